@@ -1,0 +1,141 @@
+"""Pin the CPU oracle (oracle/oracle.c) against golden vectors written by the
+compiled reference cpp/ (tests/golden/make_golden.py).  Every comparison is
+bit-exact: the oracle keeps the reference's operation order and libm calls."""
+import ast
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def load(name):
+    d = dict(np.load(os.path.join(GOLD, name + ".npz")))
+    params = dict(ast.literal_eval(str(d.pop("params_json")))) if "params_json" in d else None
+    return d, params
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    if not os.path.exists(O.ORACLE_SO):
+        O.build(with_reference=False)
+
+
+def state_eq(o, g, prefix):
+    pos, vel = o.get_fluid()
+    np.testing.assert_array_equal(pos, g[prefix + "pos"], err_msg=prefix + "pos")
+    np.testing.assert_array_equal(vel, g[prefix + "vel"], err_msg=prefix + "vel")
+    c = o.get_colloid()
+    for k, v in c.items():
+        np.testing.assert_array_equal(v, g[prefix + "c_" + k], err_msg=prefix + "c_" + k)
+
+
+def test_generator_stream():
+    g, _ = load("rng")
+    o = O.Oracle(L=7, density=1, n_colloid=1)
+    a = np.array([o.ran() for _ in range(20000)])
+    np.testing.assert_array_equal(a, g["draws_77777"])
+    more = np.array([o.ran() for _ in range(180000)])
+    assert a.sum() + more.sum() == g["sum_200k"][0]
+    assert more[-1] == g["last_200k"][0]
+    o2 = O.Oracle(L=7, density=1, n_colloid=1, seed=-12345)
+    np.testing.assert_array_equal(np.array([o2.ran() for _ in range(20000)]), g["draws_m12345"])
+
+
+def test_generator_range_and_moments():
+    # closed-form checks independent of the golden stream: U(0,1) moments
+    o = O.Oracle(L=7, density=1, n_colloid=1, seed=4242)
+    a = np.array([o.ran() for _ in range(200000)])
+    assert a.min() > 0 and a.max() < 1
+    assert abs(a.mean() - 0.5) < 4 * (1 / np.sqrt(12 * len(a)))
+    assert abs(a.var() - 1 / 12) < 0.002
+
+
+def test_small_function_trace():
+    g, p = load("small")
+    o = O.Oracle(**p)
+    o.initialize_colloid()
+    c = o.get_colloid()
+    for k in ("pos", "vel", "omega"):
+        np.testing.assert_array_equal(c[k], g["init_colloid_" + k])
+    o.initialize_fluid()
+    state_eq(o, g, "init_")
+    o.create_box()
+    np.testing.assert_array_equal(o.box_neigh(), g["box_neigh"].astype(np.int32))
+    o.neighbour_list_mpcd()
+    np.testing.assert_array_equal(o.fluid_no(), g["nl_fluid_no"])
+    np.testing.assert_array_equal(o.cell_lists(0), g["nl_box_part"])
+    cnt, lst = o.neigh_fl()
+    np.testing.assert_array_equal(cnt, g["nl_no_neigh"])
+    np.testing.assert_array_equal(lst, g["nl_neigh_fl"])
+    o.neighbour_list_md()
+    cnt, lst = o.md_list()
+    np.testing.assert_array_equal(cnt, g["md_count"])
+    np.testing.assert_array_equal(lst, g["md_list"])
+    o.compute_force_md()
+    np.testing.assert_array_equal(o.get_colloid()["f"], g["force0"])
+    assert o.potential() == g["potential0"][0]
+    o.tumble()
+    state_eq(o, g, "start_")
+    o.rotation_mpcd()
+    # the reference's fluid_no is shared: after rotation_mpcd it holds shifted counts
+    np.testing.assert_array_equal(o.cell_lists(1), g["rot_cell_part"])
+    state_eq(o, g, "rot_")
+    o.run()
+    cnt, lst = o.run_lists()
+    np.testing.assert_array_equal(cnt, g["run_cnt"])
+    np.testing.assert_array_equal(lst, g["run_nbr"])
+    state_eq(o, g, "run_")
+    seq = ["copy_old_force", "update_pos_md", "neighbour_list_md", "update_pos_mpcd",
+           "neighbour_list_mpcd", "fluid_colloid_collision", "update_activity_direction",
+           "compute_force_md", "update_velocity_colloid"]
+    for l in range(1, p["n_sub"] + 1):
+        for fn in seq:
+            getattr(o, fn)()
+            if l == 1:
+                state_eq(o, g, f"s1_{fn}_")
+                if fn == "neighbour_list_mpcd":
+                    np.testing.assert_array_equal(o.fluid_no(), g["s1_nl_fluid_no"])
+                    np.testing.assert_array_equal(o.cell_lists(0), g["s1_nl_box_part"])
+                    cnt, lst = o.neigh_fl()
+                    np.testing.assert_array_equal(cnt, g["s1_nl_no_neigh"])
+                    np.testing.assert_array_equal(lst, g["s1_nl_neigh_fl"])
+                if fn == "neighbour_list_md":
+                    cnt, lst = o.md_list()
+                    np.testing.assert_array_equal(cnt, g["s1_md_count"])
+                    np.testing.assert_array_equal(lst, g["s1_md_list"])
+        assert o.potential() == g[f"pot_sub{l}"][0]
+    state_eq(o, g, "step1_")
+    o.step(); o.step()
+    state_eq(o, g, "step3_")
+    assert o.potential() == g["step3_potential"][0]
+
+
+def test_c0like_trajectory():
+    g, p = load("c0like")
+    o = O.Oracle(**p)
+    o.startup()
+    state_eq(o, g, "start_")
+    o.step(); o.step()
+    state_eq(o, g, "step2_")
+    assert o.potential() == g["step2_potential"][0]
+    kinds, keys, vals = o.drawlog()
+    assert (kinds == O.DRAW_REFLECT).sum() > 20  # collisions were exercised
+
+
+def test_overlapping_colloids_sequential_semantics():
+    g, p = load("overlap")
+    o = O.Oracle(**p)
+    o.startup()
+    c = o.get_colloid()
+    c["pos"][0] = [5.0, 5.0, 5.0]
+    c["pos"][1] = [8.1, 5.4, 4.8]
+    o.set_colloid(**c)
+    o.neighbour_list_mpcd(); o.neighbour_list_md(); o.compute_force_md()
+    state_eq(o, g, "start_")
+    o.step()
+    state_eq(o, g, "step1_")
+    assert o.potential() == g["step1_potential"][0]
