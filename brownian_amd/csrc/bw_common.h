@@ -1,0 +1,112 @@
+// Shared host/device definitions of libbrownian.
+//
+// World: n_rep independent replicas (a plain run is one replica).  Fluid
+// particles live in tiles -- blocks of Bx x By x Bz cells of the SRD grid as it
+// was shifted at the last rotation step -- each tile owning a fixed range of
+// `cap` slots in structure-of-arrays buffers.  A particle's position is an
+// integer cell (ix, iy, iz packed as ix + L*(iy + L*iz)) plus an offset in that
+// cell stored in Real (float or double); velocities are Real.  Colloids are
+// few and kept in FP64.
+#pragma once
+#include <stdint.h>
+
+namespace bw {
+
+constexpr int EV_CAP = 64;          // reflection events per colloid per substep
+constexpr int CHAIN_CAP = 512;      // chain particles per replica per substep
+constexpr int TCL_CAP = 128;        // colloids listed for one tile
+constexpr int MD_CAP = 128;         // full MD neighbour list per colloid
+constexpr int MDC_CAP = 256;        // MD candidates (cutoff + skin) per colloid
+constexpr int MAX_CAP = 4096;       // slots per tile upper bound
+constexpr int SRD_THREADS = 512;
+constexpr int SRD_PT = MAX_CAP / SRD_THREADS;  // particles per thread in the SRD kernel
+constexpr int STREAM_THREADS = 256;
+constexpr int DEPART_THREADS = 256;
+constexpr int COLLIDE_THREADS = 256; // 4 colloids (one wave each) per workgroup
+constexpr int NDIR = 27;
+constexpr int TILE_REACH = 3;       // cells a particle may stand outside its tile box
+constexpr double CHAIN_MARGIN = 0.5; // extra radius that marks a particle as shared
+constexpr double MD_SKIN = 2.0;     // candidate-list skin for the colloid pair list
+
+// colloid phase flags (applied in this order)
+enum : unsigned {
+    PH_APPLY = 1u << 0,     // v += dv*m_f/m_c, w += dw*m_f/I   (cpp/fluid_colloid_collision.cpp:46-47)
+    PH_ACTIVITY = 1u << 1,  // cpp/update.cpp:34-46
+    PH_FORCE = 1u << 2,     // cpp/update.cpp:14-32 (force only)
+    PH_VELOCITY = 1u << 3,  // cpp/update.cpp:62-67
+    PH_KICK = 1u << 4,      // cpp/run.cpp:14 (colloid part of run) + run-time snapshot
+    PH_OLDF = 1u << 5,      // cpp/active_brownian.cpp:28
+    PH_POS = 1u << 6,       // cpp/update.cpp:48-54
+    PH_MDLIST = 1u << 7,    // filter candidates into the pair list (cpp/neighbour_list.cpp:27-39)
+    PH_POTENTIAL = 1u << 8, // potential in the reference's summation order
+    PH_RESET_EVENTS = 1u << 9,
+};
+
+struct Phys {
+    double kbt, mass_fl, mass_colloid, sig_colloid, eps, sigma, I_colloid;
+    double ndt, dt;
+    double theta, ct, st, ict;        // SRD angle and its cos/sin/(1-cos)
+    double r_cutoff, fc, ufc, sig12, sig6;
+    double lim_coll;                  // pow(sigma,2)*0.25
+    double lim_shell;                 // pow(sigma*0.5+0.5,2)
+    double lim_md;                    // pow(3*sig_colloid,2)
+    double lim_md_skin;               // (3*sig_colloid + skin)^2
+    double lim_chain;                 // (sigma/2 + margin)^2
+    double vscale_thermal;            // sqrt(kbt/mass_fl)
+    int n_sub;
+    int cap;                          // slots per tile
+    int n_rep, n_tiles, n_colloid_total;
+    int draws;                        // BW_DRAWS_*
+};
+
+struct Rep {
+    int L, nf, nc;
+    int first_colloid;
+    int B[3], nt[3];
+    int first_tile, n_tiles;
+    long long first_cell;   // offset into per-cell arrays
+    long long first_fluid;  // offset into id-indexed per-particle arrays
+    uint32_t seed_lo, seed_hi;
+};
+
+template <class Real>
+struct Fluid {
+    int *cell;
+    Real *ox, *oy, *oz;
+    Real *vx, *vy, *vz;
+    int *pid;
+};
+
+struct Dev {
+    const Rep *rep;
+    const int *tile_rep;
+    int *count;          // [n_tiles] particles in each tile (current buffer)
+    int *count_next;     // [n_tiles] for the buffer being written
+    int *dep_off;        // [n_tiles * 28]
+    int *dep_idx;        // [n_tiles * cap]
+    // colloids (global index g), FP64, 3 per colloid
+    double *cpos, *cvel, *comega, *cf, *coldf, *cra, *cdisp, *cpos_run, *cdv, *cdw, *cpos_md;
+    double *cv0;
+    int *crep, *ccell, *ccell_run, *ccnt;
+    int *md_cnt, *md_list, *mdc_cnt, *mdc_list;
+    double *cpot;        // [n_rep] potential
+    int *ev_cnt, *ev_slot, *chain_cnt, *chain_slot;
+    unsigned *ticket;    // [n_rep]
+    unsigned *status;
+    long long *step;     // MPCD step counter (device generator key)
+    double *shift;       // [n_rep*3] shift of the current step
+    // injected draws
+    const double *inj_shift;   // [n_rep*3]
+    const double *inj_axes;    // [cells_total*2]
+    const int *inj_off;        // [n_sub*n_colloid_total + 1]
+    const int *inj_pid;
+    const double *inj_val;     // 6 per tuple
+    // maps
+    int *map_count, *map_start, *map_fill, *map_list;
+    int *neigh_cnt, *neigh_list;
+    int neigh_cap;
+    int *shifted_cell;         // id-indexed
+    double *audit_partial;     // [n_tiles * 8]
+};
+
+}  // namespace bw

@@ -1,0 +1,503 @@
+// Device helpers of libbrownian (gfx950).  Included by the kernel sources only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/brownian.h"
+#include "bw_common.h"
+
+namespace bw {
+
+// ------------------------------------------------------------------ generator
+// Counter-based generator for the device draw source: Philox4x32-10
+// (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11).  A draw is a pure function of (seed, step, kind, index), so replicas,
+// tiles and graph replays need no generator state.
+__device__ __forceinline__ uint4 philox(uint4 c, uint2 k) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+        uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+// 53-bit uniform in [0,1) from two words
+__device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+enum : uint32_t { KIND_SHIFT = 1, KIND_AXIS = 2, KIND_REFL = 3 };
+
+__device__ __forceinline__ uint2 rep_key(const Rep &R) { return make_uint2(R.seed_lo, R.seed_hi); }
+
+// raw uniforms of the grid shift (cpp/rotation_mpcd.cpp:9 draws 3)
+__device__ inline void draw_shift_raw(const Dev &d, const Phys &ph, const Rep &R, int r, long long step,
+                                      double out[3]) {
+    if (ph.draws == BW_DRAWS_INJECTED) {
+        for (int k = 0; k < 3; k++) out[k] = d.inj_shift[3 * r + k];
+        return;
+    }
+    uint4 a = philox(make_uint4(KIND_SHIFT, (uint32_t)step, 0u, (uint32_t)(step >> 32)), rep_key(R));
+    uint4 b = philox(make_uint4(KIND_SHIFT | 0x100u, (uint32_t)step, 0u, (uint32_t)(step >> 32)), rep_key(R));
+    out[0] = u01(a.x, a.y);
+    out[1] = u01(a.z, a.w);
+    out[2] = u01(b.x, b.y);
+}
+
+// raw uniforms (rho, phi draws of cpp/rotation_mpcd.cpp:19) for shifted cell `cell`
+__device__ inline bool draw_axis_raw(const Dev &d, const Phys &ph, const Rep &R, long long step, int cell,
+                                     double &u1, double &u2) {
+    if (ph.draws == BW_DRAWS_INJECTED) {
+        long long i = R.first_cell + cell;
+        u1 = d.inj_axes[2 * i];
+        u2 = d.inj_axes[2 * i + 1];
+        return !(u1 != u1);  // NaN marks "no draw"
+    }
+    uint4 a = philox(make_uint4(KIND_AXIS, (uint32_t)step, (uint32_t)cell, (uint32_t)(step >> 32)), rep_key(R));
+    u1 = u01(a.x, a.y);
+    u2 = u01(a.z, a.w);
+    return true;
+}
+
+// raw reflection tuple (U_e, t.x, t.y, t.z, a, b) of cpp/fluid_colloid_collision.cpp:8-19
+struct ReflDraw {
+    double v[6];
+};
+__device__ inline bool draw_reflection(const Dev &d, const Phys &ph, const Rep &R, long long step, int substep,
+                                       int g, int pid, ReflDraw &o) {
+    if (ph.draws == BW_DRAWS_INJECTED) {
+        int base = substep * ph.n_colloid_total + g;
+        int b = d.inj_off[base], e = d.inj_off[base + 1];
+        for (int i = b; i < e; i++)
+            if (d.inj_pid[i] == pid) {
+                for (int k = 0; k < 6; k++) o.v[k] = d.inj_val[6 * i + k];
+                return true;
+            }
+        return false;
+    }
+    uint32_t c0 = KIND_REFL | ((uint32_t)substep << 8);
+    uint4 a = philox(make_uint4(c0, (uint32_t)step, (uint32_t)pid, (uint32_t)g), rep_key(R));
+    uint4 b = philox(make_uint4(c0 | (1u << 16), (uint32_t)step, (uint32_t)pid, (uint32_t)g), rep_key(R));
+    o.v[0] = u01(a.x, a.y);
+    o.v[1] = u01(a.z, a.w);
+    o.v[2] = u01(b.x, b.y);
+    o.v[3] = u01(b.z, b.w);
+    // polar pair by rejection, as the reference's loop (:17-20); z = 0 also rejected
+    for (uint32_t t = 2; t < 65535u; t++) {
+        uint4 c = philox(make_uint4(c0 | (t << 16), (uint32_t)step, (uint32_t)pid, (uint32_t)g), rep_key(R));
+        double aa = u01(c.x, c.y), bb = u01(c.z, c.w);
+        double x1 = 2.0 * aa - 1, x2 = 2.0 * bb - 1, z = x1 * x1 + x2 * x2;
+        if (z <= 1 && z > 0) {
+            o.v[4] = aa;
+            o.v[5] = bb;
+            break;
+        }
+    }
+    return true;
+}
+
+// -------------------------------------------------------------- geometry
+// img(a,b) = a - b*round(a/b) with round half away from zero (cpp/parameters.hpp:74,76)
+__device__ __forceinline__ double img1(double a, double L) { return a - L * round(a / L); }
+// mod(a,b) = a - b*round((a - b/2)/b) (cpp/parameters.hpp:75)
+__device__ __forceinline__ double mod1(double a, double L) { return a - L * round((a - L / 2) / L); }
+
+__device__ __forceinline__ void unpack_cell(int c, int L, int &ix, int &iy, int &iz) {
+    int LL = L * L;
+    iz = c / LL;
+    int rem = c - iz * LL;
+    iy = rem / L;
+    ix = rem - iy * L;
+}
+__device__ __forceinline__ int pack_cell(int ix, int iy, int iz, int L) { return ix + L * (iy + L * iz); }
+__device__ __forceinline__ int pdiff(int a, int b, int L) {
+    int d = a - b;
+    if (d > L / 2) d -= L;
+    if (d < -(L / 2)) d += L;
+    return d;
+}
+// cell (ix,iy,iz) belongs to the 7^3-1 stencil of cell (cx,cy,cz)
+// (cpp/neighbour_list.cpp:12-25); requires L >= 7
+__device__ __forceinline__ bool in_stencil(int ix, int iy, int iz, int cx, int cy, int cz, int L) {
+    int dx = pdiff(ix, cx, L), dy = pdiff(iy, cy, L), dz = pdiff(iz, cz, L);
+    return dx >= -3 && dx <= 3 && dy >= -3 && dy <= 3 && dz >= -3 && dz <= 3 && (dx | dy | dz) != 0;
+}
+// position of (dx,dy,dz) in the reference stencil order (x fastest, centre skipped)
+__device__ __forceinline__ int stencil_index(int dx, int dy, int dz) {
+    int i = (dx + 3) + 7 * (dy + 3) + 49 * (dz + 3);
+    return i < 171 ? i : i - 1;
+}
+// cell of an absolute colloid position in [0, L] (cpp/parameters.hpp:59)
+__device__ __forceinline__ int coord_cell(double x, int L) {
+    int c = (int)x;
+    if (c >= L) c -= L;
+    if (c < 0) c += L;
+    return c;
+}
+
+// shifted-cell coordinate: ix + floor(o + s) on one IEEE add, wrapped
+template <class Real>
+__device__ __forceinline__ int shifted_coord(int ix, Real o, double s, int L) {
+    int k = ix + (int)floor((double)o + s);
+    if (k >= L) k -= L;
+    if (k < 0) k += L;
+    return k;
+}
+
+// bring an offset back into [0,1), moving the cell index (periodic)
+template <class Real>
+__device__ __forceinline__ void normalise(int &i, Real &o, int L) {
+    Real f = floor(o);
+    if (f != Real(0)) {
+        o -= f;
+        i += (int)f;
+        if (o >= Real(1)) {  // o + 1 rounded up to 1 for a tiny negative o
+            o = Real(0);
+            i += 1;
+        }
+        i %= L;
+        if (i < 0) i += L;
+    }
+}
+
+// ------------------------------------------------------------ reflection
+// Stochastic (thermal-wall) reflection, cpp/fluid_colloid_collision.cpp:7-24:
+// Rayleigh normal speed along rs, tangential Gaussian along un x t.
+__device__ inline void stochastic_reflection(const Phys &ph, double L, double rfx, double rfy, double rfz,
+                                             double rsx, double rsy, double rsz, const ReflDraw &dr,
+                                             double &ux, double &uy, double &uz) {
+    double m_beta = ph.mass_fl / ph.kbt;
+    double re = (1 - dr.v[0]) * (1 - dr.v[0]);
+    double nrm = sqrt((rsx * rsx + rsy * rsy) + rsz * rsz);
+    double nx = rsx / nrm, ny = rsy / nrm, nz = rsz / nrm;
+    double val = sqrt(-log(re) / m_beta);
+    double unx = nx * val, uny = ny * val, unz = nz * val;
+    double tx = img1(dr.v[1] * L - rfx, L), ty = img1(dr.v[2] * L - rfy, L), tz = img1(dr.v[3] * L - rfz, L);
+    double utx = uny * tz - unz * ty, uty = unz * tx - unx * tz, utz = unx * ty - uny * tx;
+    double un2 = sqrt((utx * utx + uty * uty) + utz * utz);
+    utx /= un2; uty /= un2; utz /= un2;
+    double x1 = 2.0 * dr.v[4] - 1, x2 = 2.0 * dr.v[5] - 1;
+    double z = x1 * x1 + x2 * x2;
+    z = sqrt((-2.0 * log(z)) / z);
+    double v1 = x1 * z * ph.vscale_thermal;
+    ux = utx * v1 + unx;
+    uy = uty * v1 + uny;
+    uz = utz * v1 + unz;
+}
+
+// ------------------------------------------------------- block utilities
+// stable compaction index within the block for threads with pred set
+__device__ inline int block_compact(bool pred, int *s_wtot, int &total) {
+    const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned long long m = __ballot(pred);
+    int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wtot[w] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int i = 0; i < nw; i++) {
+        int v = s_wtot[i];
+        off += (i < w) ? v : 0;
+        tot += v;
+    }
+    __syncthreads();
+    total = tot;
+    return off + rank;
+}
+
+// exclusive scan of s[0..n) in place (n <= 1024), returns the total
+__device__ inline int block_exclusive_scan(int *s, int n, int *s_wtot) {
+    const int nw = blockDim.x >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int total = 0;
+    for (int base = 0; base < n; base += blockDim.x) {
+        int i = base + threadIdx.x;
+        int v = i < n ? s[i] : 0;
+        int x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_wtot[w] = x;
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int k = 0; k < nw; k++) {
+            off += (k < w) ? s_wtot[k] : 0;
+            tot += s_wtot[k];
+        }
+        if (i < n) s[i] = total + off + x - v;
+        total += tot;
+        __syncthreads();
+    }
+    return total;
+}
+
+// ------------------------------------------------ colloids listed per tile
+struct TileColl {
+    int n;
+    int g[TCL_CAP];
+    double px[TCL_CAP], py[TCL_CAP], pz[TCL_CAP];
+    int cx[TCL_CAP], cy[TCL_CAP], cz[TCL_CAP];
+    double rx[TCL_CAP], ry[TCL_CAP], rz[TCL_CAP];    // position at run() time
+    int qx[TCL_CAP], qy[TCL_CAP], qz[TCL_CAP];       // its cell
+    double vx[TCL_CAP], vy[TCL_CAP], vz[TCL_CAP];    // velocity after the kick
+    double kx[TCL_CAP], ky[TCL_CAP], kz[TCL_CAP];    // per-particle kick
+    int wtot[16];
+};
+
+__device__ __forceinline__ bool near_interval(int c, int lo, int len, int L) {
+    if (len >= L) return true;
+    int d = c - lo;
+    d %= L;
+    if (d < 0) d += L;
+    return d < len;
+}
+
+// List (ascending g) the replica's colloids that can touch a particle of tile
+// (tx,ty,tz): a particle stays within TILE_REACH cells of its tile box and the
+// collision / run() stencils reach 3 cells, so the colloid cell must lie within
+// the tile box grown by TILE_REACH + 3 (per axis, periodic).
+__device__ inline void build_tile_colloids(const Dev &d, const Phys &ph, const Rep &R, int tx, int ty, int tz,
+                                           bool want_cur, bool want_run, bool kick_data, TileColl &tc) {
+    const int reach = TILE_REACH + 3;
+    int lox = tx * R.B[0] - reach, loy = ty * R.B[1] - reach, loz = tz * R.B[2] - reach;
+    int lnx = R.B[0] + 2 * reach, lny = R.B[1] + 2 * reach, lnz = R.B[2] + 2 * reach;
+    if (threadIdx.x == 0) tc.n = 0;
+    __syncthreads();
+    for (int base = 0; base < R.nc; base += blockDim.x) {
+        int i = base + threadIdx.x;
+        bool p = false;
+        int g = R.first_colloid + i;
+        if (i < R.nc) {
+            if (want_cur)
+                p = near_interval(d.ccell[3 * g], lox, lnx, R.L) && near_interval(d.ccell[3 * g + 1], loy, lny, R.L) &&
+                    near_interval(d.ccell[3 * g + 2], loz, lnz, R.L);
+            if (want_run && !p)
+                p = near_interval(d.ccell_run[3 * g], lox, lnx, R.L) &&
+                    near_interval(d.ccell_run[3 * g + 1], loy, lny, R.L) &&
+                    near_interval(d.ccell_run[3 * g + 2], loz, lnz, R.L);
+        }
+        int tot;
+        int idx = block_compact(p, tc.wtot, tot);
+        int slot = tc.n + idx;
+        if (p) {
+            if (slot < TCL_CAP) {
+                tc.g[slot] = g;
+                tc.px[slot] = d.cpos[3 * g]; tc.py[slot] = d.cpos[3 * g + 1]; tc.pz[slot] = d.cpos[3 * g + 2];
+                tc.cx[slot] = d.ccell[3 * g]; tc.cy[slot] = d.ccell[3 * g + 1]; tc.cz[slot] = d.ccell[3 * g + 2];
+                if (kick_data) {
+                    tc.rx[slot] = d.cpos_run[3 * g]; tc.ry[slot] = d.cpos_run[3 * g + 1];
+                    tc.rz[slot] = d.cpos_run[3 * g + 2];
+                    tc.qx[slot] = d.ccell_run[3 * g]; tc.qy[slot] = d.ccell_run[3 * g + 1];
+                    tc.qz[slot] = d.ccell_run[3 * g + 2];
+                    tc.vx[slot] = d.cvel[3 * g]; tc.vy[slot] = d.cvel[3 * g + 1]; tc.vz[slot] = d.cvel[3 * g + 2];
+                    // cpp/run.cpp:14,23: del*(mass_colloid/(mass_fl*cnt)) with del = ra*v0
+                    int c = d.ccnt[g];
+                    double fac = c > 0 ? ph.mass_colloid / (ph.mass_fl * c) : 0.0;
+                    double v0 = d.cv0[g];
+                    tc.kx[slot] = (d.cra[3 * g] * v0) * fac;
+                    tc.ky[slot] = (d.cra[3 * g + 1] * v0) * fac;
+                    tc.kz[slot] = (d.cra[3 * g + 2] * v0) * fac;
+                }
+            } else {
+                atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) tc.n = min(tc.n + tot, TCL_CAP);
+        __syncthreads();
+    }
+}
+
+// run() rear-shell test of listed colloid k for a particle at binned cell
+// (ix,iy,iz) and absolute position (x,y,z) (cpp/run.cpp:17-18)
+__device__ __forceinline__ bool shell_test(const Phys &ph, const TileColl &tc, int k, int L, int ix, int iy,
+                                           int iz, double x, double y, double z) {
+    if (!in_stencil(ix, iy, iz, tc.qx[k], tc.qy[k], tc.qz[k], L)) return false;
+    double Ld = (double)L;
+    double ax = img1(x - tc.rx[k], Ld), ay = img1(y - tc.ry[k], Ld), az = img1(z - tc.rz[k], Ld);
+    double d2 = (ax * ax + ay * ay) + az * az;
+    double dot = (ax * tc.vx[k] + ay * tc.vy[k]) + az * tc.vz[k];
+    return d2 <= ph.lim_shell && dot <= 0;
+}
+
+// ------------------------------------------------------------ colloid phases
+// cpp/update.cpp:34-46
+__device__ inline void activity_one(const Phys &ph, double *ra, const double *w) {
+    double bx = w[0] * ph.dt, by = w[1] * ph.dt, bz = w[2] * ph.dt;
+    double sbx = sin(bx), sby = sin(by), sbz = sin(bz), cbx = cos(bx), cby = cos(by), cbz = cos(bz);
+    double m1x = cby * cbz, m1y = -cby * sbz, m1z = sby;
+    double m2x = sbx * sby * cbz + cbx * sbz, m2y = -sbx * sby * sbz + cbx * cbz, m2z = -sbx * cby;
+    double m3x = -cbx * sby * cbz + sbx * sbz, m3y = cbx * sby * sbz + sbx * cbz, m3z = cbx * cby;
+    double a = ra[0], b = ra[1], c = ra[2];
+    ra[0] = (m1x * a + m1y * b) + m1z * c;
+    ra[1] = (m2x * a + m2y * b) + m2z * c;
+    ra[2] = (m3x * a + m3y * b) + m3z * c;
+}
+
+// cpp/update.cpp:8-13
+__device__ __forceinline__ double powi(double x, int r) {
+    double ans = 1;
+    for (int i = 1; i <= r; i++) ans *= x;
+    return ans;
+}
+
+// WCA pair force on the pair (i, j), i the lower index: temp = img(p_i - p_j)
+// (cpp/update.cpp:21-28); returns false beyond r_cutoff
+__device__ __forceinline__ bool pair_force(const Phys &ph, double L, const double *pi, const double *pj,
+                                           double &fx, double &fy, double &fz) {
+    double tx = img1(pi[0] - pj[0], L), ty = img1(pi[1] - pj[1], L), tz = img1(pi[2] - pj[2], L);
+    double r = sqrt((tx * tx + ty * ty) + tz * tz);
+    if (!(r < ph.r_cutoff)) return false;
+    double t1 = ph.sig12 / powi(r, 13), t2 = ph.sig6 / powi(r, 7);
+    double mag = 4.0 * ph.eps * (12.0 * t1 - 6.0 * t2) - ph.fc;
+    fx = (tx * mag) / r;
+    fy = (ty * mag) / r;
+    fz = (tz * mag) / r;
+    return true;
+}
+
+// All colloid work of one replica, run by one workgroup, phases in the order
+// of the PH_* flags.  Every colloid-only reference entry point is one flag.
+__device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R, int r, unsigned flags) {
+    const int g0 = R.first_colloid, nc = R.nc;
+    const double L = (double)R.L;
+    if (flags & PH_APPLY) {
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+            int g = g0 + i;
+            for (int k = 0; k < 3; k++) {
+                d.cvel[3 * g + k] = d.cvel[3 * g + k] + d.cdv[3 * g + k] * ph.mass_fl / ph.mass_colloid;
+                d.comega[3 * g + k] = d.comega[3 * g + k] + d.cdw[3 * g + k] * ph.mass_fl / ph.I_colloid;
+                d.cdv[3 * g + k] = 0;
+                d.cdw[3 * g + k] = 0;
+            }
+        }
+    }
+    if (flags & PH_ACTIVITY) {
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+            int g = g0 + i;
+            double ra[3] = {d.cra[3 * g], d.cra[3 * g + 1], d.cra[3 * g + 2]};
+            double w[3] = {d.comega[3 * g], d.comega[3 * g + 1], d.comega[3 * g + 2]};
+            activity_one(ph, ra, w);
+            for (int k = 0; k < 3; k++) d.cra[3 * g + k] = ra[k];
+        }
+    }
+    if (flags & PH_FORCE) {
+        // f_k in the reference's accumulation order: -ff(i,k) for partners
+        // i < k in ascending i, then +ff(k,j) for partners j > k ascending
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+            int g = g0 + i;
+            double f0 = 0, f1 = 0, f2 = 0;
+            int n = d.md_cnt[g];
+            const double *pk = d.cpos + 3 * g;
+            for (int m = 0; m < n; m++) {
+                int j = d.md_list[g * MD_CAP + m];
+                double fx, fy, fz;
+                if (j < g) {
+                    if (pair_force(ph, L, d.cpos + 3 * j, pk, fx, fy, fz)) { f0 = f0 - fx; f1 = f1 - fy; f2 = f2 - fz; }
+                } else {
+                    if (pair_force(ph, L, pk, d.cpos + 3 * j, fx, fy, fz)) { f0 = f0 + fx; f1 = f1 + fy; f2 = f2 + fz; }
+                }
+            }
+            d.cf[3 * g] = f0; d.cf[3 * g + 1] = f1; d.cf[3 * g + 2] = f2;
+        }
+    }
+    __syncthreads();
+    if (flags & PH_VELOCITY) {
+        double dtb2 = ph.dt / (ph.mass_colloid * 2);
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+            int g = g0 + i;
+            for (int k = 0; k < 3; k++)
+                d.cvel[3 * g + k] = d.cvel[3 * g + k] + (d.coldf[3 * g + k] + d.cf[3 * g + k]) * dtb2;
+        }
+    }
+    if (flags & PH_KICK) {
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+            int g = g0 + i;
+            double v0 = d.cv0[g];
+            for (int k = 0; k < 3; k++) {
+                d.cvel[3 * g + k] = d.cvel[3 * g + k] + d.cra[3 * g + k] * v0;
+                d.cpos_run[3 * g + k] = d.cpos[3 * g + k];
+                d.ccell_run[3 * g + k] = d.ccell[3 * g + k];
+            }
+            d.ccnt[g] = 0;
+        }
+    }
+    if (flags & PH_OLDF) {
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+            int g = g0 + i;
+            for (int k = 0; k < 3; k++) d.coldf[3 * g + k] = d.cf[3 * g + k];
+        }
+    }
+    __syncthreads();
+    if (flags & PH_POS) {
+        double dt2 = ph.dt * ph.dt, ddt = 0.5 * dt2 / ph.mass_colloid;
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+            int g = g0 + i;
+            for (int k = 0; k < 3; k++) {
+                double dx = d.cvel[3 * g + k] * ph.dt + d.cf[3 * g + k] * ddt;
+                double p = d.cpos[3 * g + k] + dx;
+                d.cdisp[3 * g + k] += dx;
+                p = mod1(p, L);
+                d.cpos[3 * g + k] = p;
+                d.ccell[3 * g + k] = coord_cell(p, R.L);
+            }
+        }
+    }
+    __syncthreads();
+    if (flags & PH_MDLIST) {
+        // exact pair list: candidates (ascending) with |img(p_i-p_j)|^2 < (3 sigma)^2
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+            int g = g0 + i;
+            const double *pi = d.cpos + 3 * g;
+            double mx = img1(pi[0] - d.cpos_md[3 * g], L), my = img1(pi[1] - d.cpos_md[3 * g + 1], L),
+                   mz = img1(pi[2] - d.cpos_md[3 * g + 2], L);
+            if ((mx * mx + my * my) + mz * mz > 0.25 * MD_SKIN * MD_SKIN) atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
+            int nc_ = d.mdc_cnt[g], w = 0;
+            for (int m = 0; m < nc_; m++) {
+                int j = d.mdc_list[g * MDC_CAP + m];
+                const double *pj = d.cpos + 3 * j;
+                double tx, ty, tz;
+                if (j > g) {
+                    tx = img1(pi[0] - pj[0], L); ty = img1(pi[1] - pj[1], L); tz = img1(pi[2] - pj[2], L);
+                } else {
+                    tx = img1(pj[0] - pi[0], L); ty = img1(pj[1] - pi[1], L); tz = img1(pj[2] - pi[2], L);
+                }
+                if ((tx * tx + ty * ty) + tz * tz < ph.lim_md) {
+                    if (w < MD_CAP) d.md_list[g * MD_CAP + w] = j;
+                    w++;
+                }
+            }
+            if (w > MD_CAP) { atomicOr(d.status, BW_FLAG_LIST_OVERFLOW); w = MD_CAP; }
+            d.md_cnt[g] = w;
+        }
+    }
+    __syncthreads();
+    if (flags & PH_POTENTIAL) {
+        // cpp/update.cpp:24, one running sum over (i ascending, j > i ascending)
+        if (threadIdx.x == 0) {
+            double pot = 0;
+            for (int i = 0; i < nc; i++) {
+                int g = g0 + i;
+                for (int m = 0; m < d.md_cnt[g]; m++) {
+                    int j = d.md_list[g * MD_CAP + m];
+                    if (j <= g) continue;
+                    const double *pi = d.cpos + 3 * g, *pj = d.cpos + 3 * j;
+                    double tx = img1(pi[0] - pj[0], L), ty = img1(pi[1] - pj[1], L), tz = img1(pi[2] - pj[2], L);
+                    double rr = sqrt((tx * tx + ty * ty) + tz * tz);
+                    if (rr < ph.r_cutoff)
+                        pot += 4 * ph.eps * (pow(ph.sig_colloid / rr, 12) - pow(ph.sig_colloid / rr, 6)) - ph.ufc +
+                               ph.fc * rr;
+                }
+            }
+            d.cpot[r] = pot;
+        }
+    }
+    if (flags & PH_RESET_EVENTS) {
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) d.ev_cnt[g0 + i] = 0;
+        if (threadIdx.x == 0) d.chain_cnt[r] = 0;
+    }
+    __syncthreads();
+}
+
+}  // namespace bw

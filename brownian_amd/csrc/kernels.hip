@@ -1,0 +1,974 @@
+// HIP kernels of libbrownian for gfx950 (CDNA4, wave64).
+//
+// Hot path per MPCD step (fused path, one replica or many):
+//   k_step_begin      step counter, grid shift of this step            (1 WG)
+//   k_md_candidates   colloid pair candidates (cutoff + skin)           (1 wave / colloid)
+//   k_depart          per tile: new shifted tile of each particle,
+//                     stable departure lists per direction              (1 WG / tile)
+//   k_srd             per tile: gather from 27 neighbour lists, LDS
+//                     counting sort by shifted cell, FP64 per-cell sums,
+//                     rotation + thermostat, run() shell counts          (1 WG / tile)
+//   n_sub x { k_stream   per tile: run() kick (first substep), streaming,
+//                        collision detection against the tile's colloids
+//             k_collide  1 wave / colloid: stochastic reflections; the
+//                        last workgroup of a replica finishes the colloid
+//                        MD of the substep }
+#include "bw_device.h"
+
+namespace bw {
+
+// ------------------------------------------------------------- step begin
+__global__ void k_step_begin(Phys ph, Dev d) {
+    long long s = *d.step + 1;
+    for (int r = threadIdx.x; r < ph.n_rep; r += blockDim.x) {
+        double u[3];
+        draw_shift_raw(d, ph, d.rep[r], r, s, u);
+        // cpp/rotation_mpcd.cpp:9: rr = (ran,ran,ran) - 0.5
+        for (int k = 0; k < 3; k++) {
+            double v = u[k] * 1 - 0.5;
+            if (!(v >= -0.5 && v < 0.5)) {  // injected draw absent (NaN) or out of range
+                atomicOr(d.status, BW_FLAG_DRAW_MISSING);
+                v = 0.0;
+            }
+            d.shift[3 * r + k] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *d.step = s;
+}
+
+// --------------------------------------------------------- colloid phases
+__global__ void k_colloid(Phys ph, Dev d, unsigned flags) {
+    int r = blockIdx.x;
+    const Rep R = d.rep[r];
+    colloid_phases(ph, d, R, r, flags);
+}
+
+// one wave per colloid: every partner within cutoff + skin, ascending
+__global__ __launch_bounds__(256) void k_md_candidates(Phys ph, Dev d, const int2 *wgt) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int2 wt = wgt[blockIdx.x];
+    const Rep R = d.rep[wt.x];
+    const int g = wt.y + w;
+    if (g >= R.first_colloid + R.nc) return;
+    const double L = R.L;
+    const double *pi = d.cpos + 3 * g;
+    int cnt = 0;
+    for (int base = 0; base < R.nc; base += 64) {
+        int j = R.first_colloid + base + lane;
+        bool p = false;
+        if (base + lane < R.nc && j != g) {
+            const double *pj = d.cpos + 3 * j;
+            double tx = img1(pi[0] - pj[0], L), ty = img1(pi[1] - pj[1], L), tz = img1(pi[2] - pj[2], L);
+            p = (tx * tx + ty * ty) + tz * tz < ph.lim_md_skin;
+        }
+        unsigned long long m = __ballot(p);
+        int rank = __popcll(m & ((1ull << lane) - 1ull));
+        if (p && cnt + rank < MDC_CAP) d.mdc_list[g * MDC_CAP + cnt + rank] = j;
+        cnt += __popcll(m);
+    }
+    if (lane == 0) {
+        if (cnt > MDC_CAP) { atomicOr(d.status, BW_FLAG_LIST_OVERFLOW); cnt = MDC_CAP; }
+        d.mdc_cnt[g] = cnt;
+        for (int k = 0; k < 3; k++) d.cpos_md[3 * g + k] = pi[k];
+    }
+}
+
+// ------------------------------------------------------------ tile geometry
+struct TileGeo {
+    int r, tx, ty, tz, L;
+};
+__device__ __forceinline__ TileGeo tile_geo(const Dev &d, int T, Rep &R) {
+    TileGeo t;
+    t.r = d.tile_rep[T];
+    R = d.rep[t.r];
+    int lt = T - R.first_tile;
+    t.tx = lt % R.nt[0];
+    t.ty = (lt / R.nt[0]) % R.nt[1];
+    t.tz = lt / (R.nt[0] * R.nt[1]);
+    t.L = R.L;
+    return t;
+}
+__device__ __forceinline__ int wrapi(int a, int n) {
+    a %= n;
+    return a < 0 ? a + n : a;
+}
+__device__ __forceinline__ int tile_delta(int tnew, int told, int nt) {
+    int dlt = tnew - told;
+    if (nt > 2) {
+        if (dlt > nt / 2) dlt -= nt;
+        if (dlt < -(nt / 2)) dlt += nt;
+    }
+    return dlt;
+}
+
+// --------------------------------------------------------------- departure
+// For each particle of tile T, its tile on the SRD grid shifted by this step's
+// shift, as a direction 0..26; stable (array-order) lists per direction.
+template <class Real>
+__global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid<Real> f) {
+    __shared__ unsigned char s_dir[MAX_CAP];
+    __shared__ int s_off[28], s_run[27];
+    __shared__ int s_w[DEPART_THREADS / 64][27];
+    const int T = blockIdx.x;
+    Rep R;
+    TileGeo tg = tile_geo(d, T, R);
+    const int L = tg.L;
+    const int n = min(d.count[T], ph.cap);
+    const double sx = d.shift[3 * tg.r], sy = d.shift[3 * tg.r + 1], sz = d.shift[3 * tg.r + 2];
+    if (threadIdx.x < 28) s_off[threadIdx.x] = 0;
+    if (threadIdx.x < 27) s_run[threadIdx.x] = 0;
+    __syncthreads();
+    bool drift = false;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int slot = T * ph.cap + i;
+        int ix, iy, iz;
+        unpack_cell(f.cell[slot], L, ix, iy, iz);
+        int kx = shifted_coord(ix, f.ox[slot], sx, L);
+        int ky = shifted_coord(iy, f.oy[slot], sy, L);
+        int kz = shifted_coord(iz, f.oz[slot], sz, L);
+        int dx = tile_delta(kx / R.B[0], tg.tx, R.nt[0]);
+        int dy = tile_delta(ky / R.B[1], tg.ty, R.nt[1]);
+        int dz = tile_delta(kz / R.B[2], tg.tz, R.nt[2]);
+        if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) {
+            drift = true;
+            dx = max(-1, min(1, dx)); dy = max(-1, min(1, dy)); dz = max(-1, min(1, dz));
+        }
+        int dir = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+        s_dir[i] = (unsigned char)dir;
+        atomicAdd(&s_off[dir], 1);
+    }
+    if (drift) atomicOr(d.status, BW_FLAG_DRIFT);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < 27; k++) {
+            int c = s_off[k];
+            s_off[k] = acc;
+            d.dep_off[T * 28 + k] = acc;
+            acc += c;
+        }
+        s_off[27] = acc;
+        d.dep_off[T * 28 + 27] = acc;
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int base = 0; base < n; base += blockDim.x) {
+        int i = base + threadIdx.x;
+        int dd = i < n ? (int)s_dir[i] : 255;
+        int myrank = 0;
+        for (int t = 0; t < 27; t++) {
+            unsigned long long m = __ballot(dd == t);
+            if (dd == t) myrank = __popcll(m & lt);
+            if (lane == 0) s_w[w][t] = __popcll(m);
+        }
+        __syncthreads();
+        if (i < n) {
+            int off = s_off[dd] + s_run[dd] + myrank;
+            for (int k = 0; k < w; k++) off += s_w[k][dd];
+            d.dep_idx[T * ph.cap + off] = i;
+        }
+        __syncthreads();
+        if (threadIdx.x < 27) {
+            int acc = 0;
+            for (int k = 0; k < DEPART_THREADS / 64; k++) acc += s_w[k][threadIdx.x];
+            s_run[threadIdx.x] += acc;
+        }
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------------- SRD
+// Destination tile T gathers the particles whose shifted cell lies in its box
+// (27 departure lists of the neighbour tiles, fixed order), sorts them by
+// shifted cell with an LDS counting sort, and per cell (particles in ascending
+// id, the reference's cell_part order) forms the FP64 mean velocity and the
+// thermostat factor of cpp/rotation_mpcd.cpp:14-49.  Output: the tile's
+// particles, rotated, in gather order, into the other buffer.
+template <class Real, bool COUNT_SHELL, bool MAP>
+__global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real> src, Fluid<Real> dst) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_pre[28], s_src[27], s_base[27];
+    __shared__ int s_wtot[16];
+    __shared__ TileColl tc;
+    __shared__ int s_shell[TCL_CAP];
+    const int T = blockIdx.x;
+    Rep R;
+    TileGeo tg = tile_geo(d, T, R);
+    const int L = tg.L;
+    const int cap = ph.cap;
+    const int B0 = R.B[0], B1 = R.B[1], B2 = R.B[2];
+    const int ncell = B0 * B1 * B2;
+    // LDS carve-up
+    double *s_u = (double *)smem;                       // 3 * ncell
+    double *s_ax = s_u + 3 * ncell;                     // 3 * ncell  (rotation axis)
+    double *s_sc = s_ax + 3 * ncell;                    // ncell      (thermostat factor)
+    Real *s_v = (Real *)(s_sc + ncell);                 // 3 * cap
+    int *s_pid = (int *)(s_v + 3 * cap);                // cap
+    int *s_start = s_pid + cap;                         // ncell + 1
+    int *s_fill = s_start + ncell + 1;                  // ncell
+    unsigned short *s_key = (unsigned short *)(s_fill + ncell);  // cap
+    unsigned short *s_ord = s_key + cap;                // cap
+
+    const long long step = *d.step;
+    const double sx = d.shift[3 * tg.r], sy = d.shift[3 * tg.r + 1], sz = d.shift[3 * tg.r + 2];
+
+    if (threadIdx.x < 27) {
+        int t = threadIdx.x;
+        int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+        int ax = wrapi(tg.tx - dx, R.nt[0]), ay = wrapi(tg.ty - dy, R.nt[1]), az = wrapi(tg.tz - dz, R.nt[2]);
+        int S = R.first_tile + ax + R.nt[0] * (ay + R.nt[1] * az);
+        s_src[t] = S;
+        s_base[t] = d.dep_off[S * 28 + t];
+        s_pre[t] = d.dep_off[S * 28 + t + 1] - d.dep_off[S * 28 + t];
+    }
+    for (int c = threadIdx.x; c < ncell + 1; c += blockDim.x) s_start[c] = 0;
+    if (COUNT_SHELL)
+        for (int k = threadIdx.x; k < TCL_CAP; k += blockDim.x) s_shell[k] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < 27; t++) {
+            int c = s_pre[t];
+            s_pre[t] = acc;
+            acc += c;
+        }
+        s_pre[27] = acc;
+    }
+    if (COUNT_SHELL) build_tile_colloids(d, ph, R, tg.tx, tg.ty, tg.tz, false, true, true, tc);
+    __syncthreads();
+    int n = s_pre[27];
+    if (n > cap) {
+        if (threadIdx.x == 0) atomicOr(d.status, BW_FLAG_TILE_OVERFLOW);
+        n = cap;
+    }
+
+    int rcell[SRD_PT], rpid[SRD_PT], rkey[SRD_PT];
+    Real rox[SRD_PT], roy[SRD_PT], roz[SRD_PT], rvx[SRD_PT], rvy[SRD_PT], rvz[SRD_PT];
+    bool drift = false;
+#pragma unroll
+    for (int k = 0; k < SRD_PT; k++) {
+        int g = threadIdx.x + k * SRD_THREADS;
+        rkey[k] = -1;
+        if (g < n) {
+            int t = 0;
+            while (t < 26 && s_pre[t + 1] <= g) t++;
+            int S = s_src[t];
+            int li = d.dep_idx[S * cap + s_base[t] + (g - s_pre[t])];
+            int slot = S * cap + li;
+            rcell[k] = src.cell[slot];
+            rox[k] = src.ox[slot]; roy[k] = src.oy[slot]; roz[k] = src.oz[slot];
+            rvx[k] = src.vx[slot]; rvy[k] = src.vy[slot]; rvz[k] = src.vz[slot];
+            rpid[k] = src.pid[slot];
+            int ix, iy, iz;
+            unpack_cell(rcell[k], L, ix, iy, iz);
+            int lx = shifted_coord(ix, rox[k], sx, L) - tg.tx * B0;
+            int ly = shifted_coord(iy, roy[k], sy, L) - tg.ty * B1;
+            int lz = shifted_coord(iz, roz[k], sz, L) - tg.tz * B2;
+            if (lx < 0 || lx >= B0 || ly < 0 || ly >= B1 || lz < 0 || lz >= B2) {
+                drift = true;
+                lx = max(0, min(B0 - 1, lx)); ly = max(0, min(B1 - 1, ly)); lz = max(0, min(B2 - 1, lz));
+            }
+            int key = lx + B0 * (ly + B1 * lz);
+            rkey[k] = key;
+            s_key[g] = (unsigned short)key;
+            s_pid[g] = rpid[k];
+            s_v[g] = rvx[k]; s_v[cap + g] = rvy[k]; s_v[2 * cap + g] = rvz[k];
+            atomicAdd(&s_start[key], 1);
+        }
+    }
+    if (drift) atomicOr(d.status, BW_FLAG_DRIFT);
+    __syncthreads();
+    block_exclusive_scan(s_start, ncell, s_wtot);
+    if (threadIdx.x == 0) s_start[ncell] = n;
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) s_fill[c] = s_start[c];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SRD_PT; k++) {
+        int g = threadIdx.x + k * SRD_THREADS;
+        if (g < n) {
+            int pos = atomicAdd(&s_fill[rkey[k]], 1);
+            s_ord[pos] = (unsigned short)g;
+        }
+    }
+    __syncthreads();
+    // per cell: canonical (ascending id) order, FP64 sums, axis, thermostat
+    bool zero_var = false, missing = false;
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        const int b = s_start[c], e = s_start[c + 1], N = e - b;
+        for (int i = b + 1; i < e; i++) {
+            unsigned short gi = s_ord[i];
+            int pi = s_pid[gi];
+            int j = i - 1;
+            while (j >= b && s_pid[s_ord[j]] > pi) {
+                s_ord[j + 1] = s_ord[j];
+                j--;
+            }
+            s_ord[j + 1] = gi;
+        }
+        if (N > 1) {
+            double ux = 0, uy = 0, uz = 0;
+            for (int i = b; i < e; i++) {
+                int gi = s_ord[i];
+                ux = ux + (double)s_v[gi] / N;
+                uy = uy + (double)s_v[cap + gi] / N;
+                uz = uz + (double)s_v[2 * cap + gi] / N;
+            }
+            int kx = tg.tx * B0 + c % B0, ky = tg.ty * B1 + (c / B0) % B1, kz = tg.tz * B2 + c / (B0 * B1);
+            int gc = kx + L * (ky + L * kz);
+            double u1, u2;
+            if (!draw_axis_raw(d, ph, R, step, gc, u1, u2)) {
+                missing = true;
+                u1 = 0.5; u2 = 0.0;
+            }
+            // cpp/rotation_mpcd.cpp:19-25
+            double rho = 2 * u1 - 1;
+            double phi = 4.0 * asin(1.0) * u2;
+            double r1 = cos(phi) * sqrt(1 - rho * rho), r2 = sin(phi) * sqrt(1 - rho * rho), r3 = rho;
+            double ir1 = ph.ict * r1, ir2 = ph.ict * r2, ir3 = ph.ict * r3;
+            double a11 = ir1 * r1 + ph.ct, a12 = ir1 * r2 - ph.st * r3, a13 = ir1 * r3 + ph.st * r2;
+            double a21 = ir2 * r1 + ph.st * r3, a22 = ir2 * r2 + ph.ct, a23 = ir2 * r3 - ph.st * r1;
+            double a31 = ir3 * r1 - ph.st * r2, a32 = ir3 * r2 + ph.st * r1, a33 = ir3 * r3 + ph.ct;
+            // thermostat variance from the rotated velocities (:37-41)
+            double var = 0.0;
+            for (int i = b; i < e; i++) {
+                int gi = s_ord[i];
+                double dx = (double)s_v[gi] - ux, dy = (double)s_v[cap + gi] - uy, dz = (double)s_v[2 * cap + gi] - uz;
+                double v1x = ux + ((a11 * dx + a12 * dy) + a13 * dz);
+                double v1y = uy + ((a21 * dx + a22 * dy) + a23 * dz);
+                double v1z = uz + ((a31 * dx + a32 * dy) + a33 * dz);
+                double ex = v1x - ux, ey = v1y - uy, ez = v1z - uz;
+                var += (ex * ex + ey * ey) + ez * ez;
+            }
+            double scale = sqrt(3.0 * (N - 1) * ph.kbt / (ph.mass_fl * var));
+            if (!(var > 0)) {
+                zero_var = true;
+                scale = 1.0;
+            }
+            s_u[3 * c] = ux; s_u[3 * c + 1] = uy; s_u[3 * c + 2] = uz;
+            s_ax[3 * c] = r1; s_ax[3 * c + 1] = r2; s_ax[3 * c + 2] = r3;
+            s_sc[c] = scale;
+        }
+    }
+    if (zero_var) atomicOr(d.status, BW_FLAG_ZERO_VARIANCE);
+    if (missing) atomicOr(d.status, BW_FLAG_DRAW_MISSING);
+    __syncthreads();
+    // apply, count run() shells, write out (coalesced, gather order)
+#pragma unroll
+    for (int k = 0; k < SRD_PT; k++) {
+        int g = threadIdx.x + k * SRD_THREADS;
+        if (g < n) {
+            int c = rkey[k];
+            int N = s_start[c + 1] - s_start[c];
+            Real vx = rvx[k], vy = rvy[k], vz = rvz[k];
+            if (N > 1) {
+                double ux = s_u[3 * c], uy = s_u[3 * c + 1], uz = s_u[3 * c + 2];
+                double r1 = s_ax[3 * c], r2 = s_ax[3 * c + 1], r3 = s_ax[3 * c + 2];
+                double ir1 = ph.ict * r1, ir2 = ph.ict * r2, ir3 = ph.ict * r3;
+                double a11 = ir1 * r1 + ph.ct, a12 = ir1 * r2 - ph.st * r3, a13 = ir1 * r3 + ph.st * r2;
+                double a21 = ir2 * r1 + ph.st * r3, a22 = ir2 * r2 + ph.ct, a23 = ir2 * r3 - ph.st * r1;
+                double a31 = ir3 * r1 - ph.st * r2, a32 = ir3 * r2 + ph.st * r1, a33 = ir3 * r3 + ph.ct;
+                double dx = (double)vx - ux, dy = (double)vy - uy, dz = (double)vz - uz;
+                double v1x = ux + ((a11 * dx + a12 * dy) + a13 * dz);
+                double v1y = uy + ((a21 * dx + a22 * dy) + a23 * dz);
+                double v1z = uz + ((a31 * dx + a32 * dy) + a33 * dz);
+                double sc = s_sc[c];
+                vx = (Real)(ux + (v1x - ux) * sc);
+                vy = (Real)(uy + (v1y - uy) * sc);
+                vz = (Real)(uz + (v1z - uz) * sc);
+            }
+            if (COUNT_SHELL) {
+                int ix, iy, iz;
+                unpack_cell(rcell[k], L, ix, iy, iz);
+                double x = (double)ix + (double)rox[k], y = (double)iy + (double)roy[k], z = (double)iz + (double)roz[k];
+                for (int q = 0; q < tc.n; q++)
+                    if (shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
+            }
+            int o = T * cap + g;
+            dst.cell[o] = rcell[k];
+            dst.ox[o] = rox[k]; dst.oy[o] = roy[k]; dst.oz[o] = roz[k];
+            dst.vx[o] = vx; dst.vy[o] = vy; dst.vz[o] = vz;
+            dst.pid[o] = rpid[k];
+            if (MAP) {
+                int kx = tg.tx * B0 + c % B0, ky = tg.ty * B1 + (c / B0) % B1, kz = tg.tz * B2 + c / (B0 * B1);
+                d.shifted_cell[R.first_fluid + rpid[k]] = kx + L * (ky + L * kz);
+            }
+        }
+    }
+    if (threadIdx.x == 0) d.count_next[T] = n;
+    if (COUNT_SHELL) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < tc.n; q += blockDim.x)
+            if (s_shell[q]) atomicAdd(&d.ccnt[tc.g[q]], s_shell[q]);
+    }
+}
+
+// ------------------------------------------------------------------ stream
+// Per tile: [run() kick on the rear shells, cpp/run.cpp:21-24]
+//           [streaming, cpp/update.cpp:56-60, with exact int-cell binning]
+//           [collision detection, cpp/fluid_colloid_collision.cpp:32-35]
+template <class Real, bool KICK, bool STREAM, bool DETECT>
+__global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid<Real> f) {
+    __shared__ TileColl tc;
+    const int T = blockIdx.x;
+    Rep R;
+    TileGeo tg = tile_geo(d, T, R);
+    const int L = tg.L;
+    const double Ld = (double)L;
+    const int n = min(d.count[T], ph.cap);
+    if (KICK || DETECT) build_tile_colloids(d, ph, R, tg.tx, tg.ty, tg.tz, DETECT, KICK, KICK, tc);
+    const Real dt = (Real)ph.dt;
+    const int lox = tg.tx * R.B[0] - TILE_REACH, loy = tg.ty * R.B[1] - TILE_REACH, loz = tg.tz * R.B[2] - TILE_REACH;
+    const int lnx = R.B[0] + 2 * TILE_REACH, lny = R.B[1] + 2 * TILE_REACH, lnz = R.B[2] + 2 * TILE_REACH;
+    bool drift = false, ev_over = false, ch_over = false;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int slot = T * ph.cap + i;
+        int ix, iy, iz;
+        unpack_cell(f.cell[slot], L, ix, iy, iz);
+        Real ox = f.ox[slot], oy = f.oy[slot], oz = f.oz[slot];
+        Real vx = f.vx[slot], vy = f.vy[slot], vz = f.vz[slot];
+        if (KICK) {
+            double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
+            double wx = vx, wy = vy, wz = vz;
+            bool hit = false;
+            for (int k = 0; k < tc.n; k++)
+                if (shell_test(ph, tc, k, L, ix, iy, iz, x, y, z)) {
+                    wx = wx - tc.kx[k];
+                    wy = wy - tc.ky[k];
+                    wz = wz - tc.kz[k];
+                    hit = true;
+                }
+            if (hit) {
+                vx = (Real)wx; vy = (Real)wy; vz = (Real)wz;
+                f.vx[slot] = vx; f.vy[slot] = vy; f.vz[slot] = vz;
+            }
+        }
+        if (STREAM) {
+            ox = ox + vx * dt;
+            oy = oy + vy * dt;
+            oz = oz + vz * dt;
+            normalise(ix, ox, L);
+            normalise(iy, oy, L);
+            normalise(iz, oz, L);
+            f.cell[slot] = pack_cell(ix, iy, iz, L);
+            f.ox[slot] = ox; f.oy[slot] = oy; f.oz[slot] = oz;
+            if (!(near_interval(ix, lox, lnx, L) && near_interval(iy, loy, lny, L) && near_interval(iz, loz, lnz, L)))
+                drift = true;
+        }
+        if (DETECT) {
+            double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
+            int n_in = 0, n_near = 0, gin = -1;
+            for (int k = 0; k < tc.n; k++) {
+                if (!in_stencil(ix, iy, iz, tc.cx[k], tc.cy[k], tc.cz[k], L)) continue;
+                double rx = img1(tc.px[k] - x, Ld), ry = img1(tc.py[k] - y, Ld), rz = img1(tc.pz[k] - z, Ld);
+                double d2 = (rx * rx + ry * ry) + rz * rz;
+                if (d2 <= ph.lim_chain) {
+                    n_near++;
+                    if (d2 <= ph.lim_coll) {
+                        n_in++;
+                        gin = tc.g[k];
+                    }
+                }
+            }
+            if (n_in > 0) {
+                if (n_near == 1) {
+                    int e = atomicAdd(&d.ev_cnt[gin], 1);
+                    if (e < EV_CAP) d.ev_slot[gin * EV_CAP + e] = slot;
+                    else ev_over = true;
+                } else {
+                    int e = atomicAdd(&d.chain_cnt[tg.r], 1);
+                    if (e < CHAIN_CAP) d.chain_slot[tg.r * CHAIN_CAP + e] = slot;
+                    else ch_over = true;
+                }
+            }
+        }
+    }
+    if (drift) atomicOr(d.status, BW_FLAG_DRIFT);
+    if (ev_over) atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
+    if (ch_over) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
+}
+
+// --------------------------------------------------------------- collisions
+// One event: back half-step, stochastic reflection with the colloid's rigid
+// body velocity, forward half-step (cpp/fluid_colloid_collision.cpp:36-43).
+// (bx,by,bz) is the offset in the binned cell (ix,iy,iz), updated in place.
+__device__ inline void reflect_event(const Phys &ph, int L, int ix, int iy, int iz, double &bx, double &by,
+                                     double &bz, double &vx, double &vy, double &vz, const double *pc,
+                                     const double *vcol, const double *wcol, const ReflDraw &dr, double *rs,
+                                     bool fp32) {
+    const double Ld = (double)L, dt = ph.dt;
+    bx = bx - vx * dt * 0.5;
+    by = by - vy * dt * 0.5;
+    bz = bz - vz * dt * 0.5;
+    double ax = (double)ix + bx, ay = (double)iy + by, az = (double)iz + bz;
+    double rsx = img1(ax - pc[0], Ld), rsy = img1(ay - pc[1], Ld), rsz = img1(az - pc[2], Ld);
+    double ux, uy, uz;
+    stochastic_reflection(ph, Ld, ax, ay, az, rsx, rsy, rsz, dr, ux, uy, uz);
+    double nx = ux + vcol[0] + (wcol[1] * rsz - wcol[2] * rsy);
+    double ny = uy + vcol[1] + (wcol[2] * rsx - wcol[0] * rsz);
+    double nz = uz + vcol[2] + (wcol[0] * rsy - wcol[1] * rsx);
+    if (fp32) {  // what the velocity store keeps
+        nx = (double)(float)nx; ny = (double)(float)ny; nz = (double)(float)nz;
+    }
+    rs[0] = rsx; rs[1] = rsy; rs[2] = rsz;
+    vx = nx; vy = ny; vz = nz;
+    bx = bx + vx * dt * 0.5;
+    by = by + vy * dt * 0.5;
+    bz = bz + vz * dt * 0.5;
+}
+
+// momentum and angular-momentum transfer of one event: dump - vel and
+// rs x (dump - vel) (cpp/fluid_colloid_collision.cpp:40-42)
+__device__ __forceinline__ void transfer(const double *dump, double vx, double vy, double vz, const double *rs,
+                                         double *c) {
+    double ex = dump[0] - vx, ey = dump[1] - vy, ez = dump[2] - vz;
+    c[0] = ex; c[1] = ey; c[2] = ez;
+    c[3] = rs[1] * ez - rs[2] * ey;
+    c[4] = rs[2] * ex - rs[0] * ez;
+    c[5] = rs[0] * ey - rs[1] * ex;
+}
+
+// Particles inside one colloid and near no other: one wave per colloid, events
+// ranked in the reference's neigh_fl order (stencil position, then id) and the
+// momentum/torque sums taken in that order.  Particles shared between colloids
+// ("chain" particles) are replayed serially, colloid by colloid, by the last
+// workgroup of the replica, which then runs the colloid phases `tail`.
+template <class Real>
+__global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Fluid<Real> f, int substep,
+                                                             unsigned tail, const int2 *wgt) {
+    __shared__ double s_c[COLLIDE_THREADS / 64][EV_CAP][6];
+    __shared__ int s_ne[COLLIDE_THREADS / 64];
+    __shared__ int s_last;
+    __shared__ int s_chain[CHAIN_CAP];
+    const bool fp32 = sizeof(Real) == 4;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int2 wt = wgt[blockIdx.x];
+    const int r = wt.x;
+    const Rep R = d.rep[r];
+    const int L = R.L;
+    const int g = wt.y + w;
+    const int gend = R.first_colloid + R.nc;
+    const long long step = *d.step;
+    int ne = 0;
+    if (g < gend) {
+        ne = d.ev_cnt[g];
+        if (ne > EV_CAP) {
+            if (lane == 0) atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
+            ne = EV_CAP;
+        }
+    }
+    int slot = -1, pid = 0, ix = 0, iy = 0, iz = 0;
+    long long key = 0x7fffffffffffffffll;
+    if (lane < ne) {
+        slot = d.ev_slot[g * EV_CAP + lane];
+        pid = f.pid[slot];
+        unpack_cell(f.cell[slot], L, ix, iy, iz);
+        int dx = pdiff(ix, d.ccell[3 * g], L), dy = pdiff(iy, d.ccell[3 * g + 1], L),
+            dz = pdiff(iz, d.ccell[3 * g + 2], L);
+        key = ((long long)stencil_index(dx, dy, dz) << 32) | (unsigned)pid;
+    }
+    int rank = 0;
+    for (int m = 0; m < ne; m++) {
+        long long km = __shfl(key, m);
+        rank += (km < key) ? 1 : 0;
+    }
+    if (lane < ne) {
+        double bx = f.ox[slot], by = f.oy[slot], bz = f.oz[slot];
+        double vx = f.vx[slot], vy = f.vy[slot], vz = f.vz[slot];
+        double bx0 = bx, by0 = by, bz0 = bz;
+        ReflDraw dr;
+        double cont[6] = {0, 0, 0, 0, 0, 0};
+        const double dump[3] = {vx, vy, vz};
+        if (draw_reflection(d, ph, R, step, substep, g, pid, dr)) {
+            double rs[3];
+            reflect_event(ph, L, ix, iy, iz, bx, by, bz, vx, vy, vz, d.cpos + 3 * g, d.cvel + 3 * g,
+                          d.comega + 3 * g, dr, rs, fp32);
+            transfer(dump, vx, vy, vz, rs, cont);
+            f.ox[slot] = (Real)bx; f.oy[slot] = (Real)by; f.oz[slot] = (Real)bz;
+            f.vx[slot] = (Real)vx; f.vy[slot] = (Real)vy; f.vz[slot] = (Real)vz;
+            double mx = bx - bx0, my = by - by0, mz = bz - bz0;
+            if ((mx * mx + my * my) + mz * mz > 0.25 * CHAIN_MARGIN * CHAIN_MARGIN)
+                atomicOr(d.status, BW_FLAG_CHAIN_DISPLACEMENT);
+        } else {
+            atomicOr(d.status, BW_FLAG_DRAW_MISSING);
+        }
+        for (int k = 0; k < 6; k++) s_c[w][rank][k] = cont[k];
+    }
+    if (lane == 0) s_ne[w] = ne;
+    __syncthreads();
+    if (lane == 0 && g < gend) {
+        double a[6] = {0, 0, 0, 0, 0, 0};
+        for (int e = 0; e < s_ne[w]; e++)
+            for (int k = 0; k < 6; k++) a[k] = a[k] + s_c[w][e][k];
+        for (int k = 0; k < 3; k++) {
+            d.cdv[3 * g + k] = a[k];
+            d.cdw[3 * g + k] = a[3 + k];
+        }
+    }
+    // last workgroup of the replica
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int mine = min(COLLIDE_THREADS / 64, gend - wt.y);
+        unsigned t = atomicAdd(&d.ticket[r], (unsigned)mine);
+        s_last = (t + (unsigned)mine == (unsigned)R.nc) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // chain particles, serially in ascending id, each tested against every
+    // colloid of the replica in ascending order (the reference's colloid loop)
+    int nch = min(d.chain_cnt[r], CHAIN_CAP);
+    if (d.chain_cnt[r] > CHAIN_CAP && threadIdx.x == 0) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
+    for (int i = threadIdx.x; i < nch; i += blockDim.x) s_chain[i] = d.chain_slot[r * CHAIN_CAP + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < nch; i++) {
+            int s = s_chain[i], p = f.pid[s], j = i - 1;
+            while (j >= 0 && f.pid[s_chain[j]] > p) {
+                s_chain[j + 1] = s_chain[j];
+                j--;
+            }
+            s_chain[j + 1] = s;
+        }
+        const double Ld = (double)L;
+        int n_chain_events = 0;
+        for (int i = 0; i < nch; i++) {
+            int s = s_chain[i];
+            int p = f.pid[s];
+            int cx, cy, cz;
+            unpack_cell(f.cell[s], L, cx, cy, cz);
+            double bx = f.ox[s], by = f.oy[s], bz = f.oz[s];
+            double vx = f.vx[s], vy = f.vy[s], vz = f.vz[s];
+            const double dump[3] = {vx, vy, vz};
+            for (int gg = R.first_colloid; gg < gend; gg++) {
+                if (!in_stencil(cx, cy, cz, d.ccell[3 * gg], d.ccell[3 * gg + 1], d.ccell[3 * gg + 2], L)) continue;
+                const double *pc = d.cpos + 3 * gg;
+                double ax = (double)cx + bx, ay = (double)cy + by, az = (double)cz + bz;
+                double rx = img1(pc[0] - ax, Ld), ry = img1(pc[1] - ay, Ld), rz = img1(pc[2] - az, Ld);
+                if ((rx * rx + ry * ry) + rz * rz > ph.lim_coll) continue;
+                ReflDraw dr;
+                if (!draw_reflection(d, ph, R, step, substep, gg, p, dr)) {
+                    atomicOr(d.status, BW_FLAG_DRAW_MISSING);
+                    continue;
+                }
+                double rs[3], c6[6];
+                reflect_event(ph, L, cx, cy, cz, bx, by, bz, vx, vy, vz, pc, d.cvel + 3 * gg, d.comega + 3 * gg, dr,
+                              rs, fp32);
+                transfer(dump, vx, vy, vz, rs, c6);
+                for (int k = 0; k < 3; k++) {
+                    d.cdv[3 * gg + k] += c6[k];
+                    d.cdw[3 * gg + k] += c6[3 + k];
+                }
+                n_chain_events++;
+            }
+            f.ox[s] = (Real)bx; f.oy[s] = (Real)by; f.oz[s] = (Real)bz;
+            f.vx[s] = (Real)vx; f.vy[s] = (Real)vy; f.vz[s] = (Real)vz;
+        }
+        if (ph.draws == BW_DRAWS_INJECTED) {
+            int n_inj = 0, n_ev = n_chain_events;
+            for (int gg = R.first_colloid; gg < gend; gg++) {
+                int base = substep * ph.n_colloid_total + gg;
+                n_inj += d.inj_off[base + 1] - d.inj_off[base];
+                n_ev += min(d.ev_cnt[gg], EV_CAP);
+            }
+            if (n_inj != n_ev) atomicOr(d.status, BW_FLAG_DRAW_MISMATCH);
+        }
+        d.ticket[r] = 0;
+    }
+    __syncthreads();
+    __threadfence();
+    colloid_phases(ph, d, R, r, tail | PH_RESET_EVENTS);
+}
+
+// ------------------------------------------------------- run() (standalone)
+template <class Real>
+__global__ __launch_bounds__(STREAM_THREADS) void k_run_count(Phys ph, Dev d, Fluid<Real> f) {
+    __shared__ TileColl tc;
+    __shared__ int s_shell[TCL_CAP];
+    const int T = blockIdx.x;
+    Rep R;
+    TileGeo tg = tile_geo(d, T, R);
+    const int L = tg.L;
+    const int n = min(d.count[T], ph.cap);
+    for (int k = threadIdx.x; k < TCL_CAP; k += blockDim.x) s_shell[k] = 0;
+    build_tile_colloids(d, ph, R, tg.tx, tg.ty, tg.tz, false, true, true, tc);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int slot = T * ph.cap + i;
+        int ix, iy, iz;
+        unpack_cell(f.cell[slot], L, ix, iy, iz);
+        double x = (double)ix + (double)f.ox[slot], y = (double)iy + (double)f.oy[slot],
+               z = (double)iz + (double)f.oz[slot];
+        for (int q = 0; q < tc.n; q++)
+            if (shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < tc.n; q += blockDim.x)
+        if (s_shell[q]) atomicAdd(&d.ccnt[tc.g[q]], s_shell[q]);
+}
+
+// -------------------------------------------------------------------- maps
+template <class Real>
+__global__ void k_map_count(Phys ph, Dev d, Fluid<Real> f) {
+    const int T = blockIdx.x;
+    Rep R;
+    tile_geo(d, T, R);
+    const int n = min(d.count[T], ph.cap);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&d.map_count[R.first_cell + f.cell[T * ph.cap + i]], 1);
+}
+template <class Real>
+__global__ void k_map_fill(Phys ph, Dev d, Fluid<Real> f) {
+    const int T = blockIdx.x;
+    Rep R;
+    tile_geo(d, T, R);
+    const int n = min(d.count[T], ph.cap);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int slot = T * ph.cap + i;
+        int pos = atomicAdd(&d.map_fill[R.first_cell + f.cell[slot]], 1);
+        d.map_list[pos] = f.pid[slot];
+    }
+}
+// ascending id within each cell (the reference's box_part order)
+__global__ void k_map_sort(Dev d, long long ncells) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int b = d.map_start[c], e = b + d.map_count[c];
+    for (int i = b + 1; i < e; i++) {
+        int v = d.map_list[i], j = i - 1;
+        while (j >= b && d.map_list[j] > v) {
+            d.map_list[j + 1] = d.map_list[j];
+            j--;
+        }
+        d.map_list[j + 1] = v;
+    }
+}
+// exclusive scan, three passes (blocks of 1024)
+__global__ void k_scan_block(const int *in, int *out, int *bsum, long long n) {
+    __shared__ int s[1024];
+    __shared__ int s_wtot[16];
+    long long i = (long long)blockIdx.x * 1024 + threadIdx.x;
+    s[threadIdx.x] = i < n ? in[i] : 0;
+    __syncthreads();
+    int tot = block_exclusive_scan(s, 1024, s_wtot);
+    if (i < n) out[i] = s[threadIdx.x];
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+__global__ void k_scan_sums(int *bsum, int nb) {
+    __shared__ int s_wtot[16];
+    __shared__ int s[1024];
+    int carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        int i = base + threadIdx.x;
+        s[threadIdx.x] = i < nb ? bsum[i] : 0;
+        __syncthreads();
+        int tot = block_exclusive_scan(s, 1024, s_wtot);
+        if (i < nb) bsum[i] = s[threadIdx.x] + carry;
+        carry += tot;
+        __syncthreads();
+    }
+}
+__global__ void k_scan_add(int *out, const int *bsum, long long n) {
+    long long i = (long long)blockIdx.x * 1024 + threadIdx.x;
+    if (i < n) out[i] += bsum[blockIdx.x];
+}
+// neigh_fl of colloid g: stencil cells in the reference order (dx fastest,
+// -3..3, centre skipped), each cell's particles in ascending id
+__global__ void k_neigh_fl(Phys ph, Dev d, const int *colloid_rep) {
+    __shared__ int s_off[343];
+    __shared__ int s_wtot[16];
+    const int g = blockIdx.x;
+    const int r = colloid_rep[g];
+    const Rep R = d.rep[r];
+    const int L = R.L;
+    const int cx = d.ccell[3 * g], cy = d.ccell[3 * g + 1], cz = d.ccell[3 * g + 2];
+    for (int k = threadIdx.x; k < 342; k += blockDim.x) {
+        int i = k < 171 ? k : k + 1;
+        int dx = i % 7 - 3, dy = (i / 7) % 7 - 3, dz = i / 49 - 3;
+        int c = pack_cell(wrapi(cx + dx, L), wrapi(cy + dy, L), wrapi(cz + dz, L), L);
+        s_off[k] = d.map_count[R.first_cell + c];
+    }
+    __syncthreads();
+    int tot = block_exclusive_scan(s_off, 342, s_wtot);
+    if (threadIdx.x == 0) {
+        if (tot > d.neigh_cap) atomicOr(d.status, BW_FLAG_NEIGH_OVERFLOW);
+        d.neigh_cnt[g] = min(tot, d.neigh_cap);
+    }
+    for (int k = threadIdx.x; k < 342; k += blockDim.x) {
+        int i = k < 171 ? k : k + 1;
+        int dx = i % 7 - 3, dy = (i / 7) % 7 - 3, dz = i / 49 - 3;
+        int c = pack_cell(wrapi(cx + dx, L), wrapi(cy + dy, L), wrapi(cz + dz, L), L);
+        long long gc = R.first_cell + c;
+        int b = d.map_start[gc], cnt = d.map_count[gc];
+        for (int e = 0; e < cnt; e++) {
+            int o = s_off[k] + e;
+            if (o < d.neigh_cap) d.neigh_list[(long long)g * d.neigh_cap + o] = d.map_list[b + e];
+        }
+    }
+}
+
+// ------------------------------------------------------------------- audit
+template <class Real>
+__global__ void k_audit(Phys ph, Dev d, Fluid<Real> f) {
+    __shared__ double s[256][5];
+    const int T = blockIdx.x;
+    const int n = min(d.count[T], ph.cap);
+    double a[5] = {0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int slot = T * ph.cap + i;
+        double vx = f.vx[slot], vy = f.vy[slot], vz = f.vz[slot];
+        a[0] += vx; a[1] += vy; a[2] += vz;
+        a[3] += (vx * vx + vy * vy) + vz * vz;
+        a[4] += (fabs(vx) + fabs(vy)) + fabs(vz);
+    }
+    for (int k = 0; k < 5; k++) s[threadIdx.x][k] = a[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 5; k++) s[threadIdx.x][k] += s[threadIdx.x + h][k];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) d.audit_partial[T * 8 + threadIdx.x] = s[0][threadIdx.x];
+}
+
+// ------------------------------------------------------- gather (download)
+template <class Real>
+__global__ void k_gather(Phys ph, Dev d, Fluid<Real> f, double *pos, double *vel, int *cell) {
+    const int T = blockIdx.x;
+    Rep R;
+    TileGeo tg = tile_geo(d, T, R);
+    const int L = tg.L;
+    const int n = min(d.count[T], ph.cap);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int slot = T * ph.cap + i;
+        long long id = R.first_fluid + f.pid[slot];
+        int c = f.cell[slot], ix, iy, iz;
+        unpack_cell(c, L, ix, iy, iz);
+        double p[3] = {(double)ix + (double)f.ox[slot], (double)iy + (double)f.oy[slot],
+                       (double)iz + (double)f.oz[slot]};
+        for (int k = 0; k < 3; k++) {
+            if (p[k] >= (double)L) p[k] -= (double)L;
+            if (p[k] < 0) p[k] += (double)L;
+            pos[3 * id + k] = p[k];
+        }
+        vel[3 * id] = f.vx[slot]; vel[3 * id + 1] = f.vy[slot]; vel[3 * id + 2] = f.vz[slot];
+        cell[id] = c;
+    }
+}
+
+// plain copy for HBM counter calibration
+__global__ void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) b[i] = a[i];
+}
+
+// -------------------------------------------------------- launch wrappers
+#define LAUNCH(kern, grid, block, lds, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, st, __VA_ARGS__)
+
+size_t srd_lds_bytes(int ncell, int cap, bool fp64) {
+    size_t rb = fp64 ? 8 : 4;
+    return (size_t)ncell * 7 * 8 + 3 * (size_t)cap * rb + (size_t)cap * 4 + (size_t)(2 * ncell + 1) * 4 +
+           (size_t)cap * 4 + 16;
+}
+
+template <class Real>
+void launch_depart(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream_t st) {
+    LAUNCH(k_depart<Real>, ph.n_tiles, DEPART_THREADS, 0, st, ph, d, f);
+}
+template <class Real>
+void launch_srd(const Phys &ph, const Dev &d, const Fluid<Real> &src, const Fluid<Real> &dst, bool count_shell,
+                bool map, size_t lds, hipStream_t st) {
+    static bool attr = false;
+    if (!attr) {
+        int bytes = 160 * 1024 - 24 * 1024;
+        (void)hipFuncSetAttribute((const void *)k_srd<Real, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void *)k_srd<Real, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void *)k_srd<Real, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void *)k_srd<Real, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        attr = true;
+    }
+    if (count_shell) {
+        if (map) LAUNCH((k_srd<Real, true, true>), ph.n_tiles, SRD_THREADS, lds, st, ph, d, src, dst);
+        else LAUNCH((k_srd<Real, true, false>), ph.n_tiles, SRD_THREADS, lds, st, ph, d, src, dst);
+    } else {
+        if (map) LAUNCH((k_srd<Real, false, true>), ph.n_tiles, SRD_THREADS, lds, st, ph, d, src, dst);
+        else LAUNCH((k_srd<Real, false, false>), ph.n_tiles, SRD_THREADS, lds, st, ph, d, src, dst);
+    }
+}
+template <class Real>
+void launch_stream(const Phys &ph, const Dev &d, const Fluid<Real> &f, bool kick, bool stream, bool detect,
+                   hipStream_t st) {
+    int m = (kick ? 4 : 0) | (stream ? 2 : 0) | (detect ? 1 : 0);
+    switch (m) {
+        case 7: LAUNCH((k_stream<Real, true, true, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
+        case 6: LAUNCH((k_stream<Real, true, true, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
+        case 4: LAUNCH((k_stream<Real, true, false, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
+        case 3: LAUNCH((k_stream<Real, false, true, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
+        case 2: LAUNCH((k_stream<Real, false, true, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
+        case 1: LAUNCH((k_stream<Real, false, false, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
+        default: break;
+    }
+}
+template <class Real>
+void launch_collide(const Phys &ph, const Dev &d, const Fluid<Real> &f, int substep, unsigned tail,
+                    const int2 *wgt, int nwg, hipStream_t st) {
+    LAUNCH(k_collide<Real>, nwg, COLLIDE_THREADS, 0, st, ph, d, f, substep, tail, wgt);
+}
+template <class Real>
+void launch_run_count(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream_t st) {
+    LAUNCH(k_run_count<Real>, ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f);
+}
+template <class Real>
+void launch_maps(const Phys &ph, const Dev &d, const Fluid<Real> &f, long long ncells, int *bsum,
+                 const int *colloid_rep, hipStream_t st) {
+    (void)hipMemsetAsync(d.map_count, 0, sizeof(int) * ncells, st);
+    LAUNCH(k_map_count<Real>, ph.n_tiles, 256, 0, st, ph, d, f);
+    long long nb = (ncells + 1023) / 1024;
+    LAUNCH(k_scan_block, nb, 1024, 0, st, d.map_count, d.map_start, bsum, ncells);
+    LAUNCH(k_scan_sums, 1, 1024, 0, st, bsum, (int)nb);
+    LAUNCH(k_scan_add, nb, 1024, 0, st, d.map_start, bsum, ncells);
+    (void)hipMemcpyAsync(d.map_fill, d.map_start, sizeof(int) * ncells, hipMemcpyDeviceToDevice, st);
+    LAUNCH(k_map_fill<Real>, ph.n_tiles, 256, 0, st, ph, d, f);
+    LAUNCH(k_map_sort, (ncells + 255) / 256, 256, 0, st, d, ncells);
+    if (ph.n_colloid_total > 0) LAUNCH(k_neigh_fl, ph.n_colloid_total, 256, 0, st, ph, d, colloid_rep);
+}
+template <class Real>
+void launch_audit(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream_t st) {
+    LAUNCH(k_audit<Real>, ph.n_tiles, 256, 0, st, ph, d, f);
+}
+template <class Real>
+void launch_gather(const Phys &ph, const Dev &d, const Fluid<Real> &f, double *pos, double *vel, int *cell,
+                   hipStream_t st) {
+    LAUNCH(k_gather<Real>, ph.n_tiles, 256, 0, st, ph, d, f, pos, vel, cell);
+}
+void launch_step_begin(const Phys &ph, const Dev &d, hipStream_t st) { LAUNCH(k_step_begin, 1, 256, 0, st, ph, d); }
+void launch_colloid(const Phys &ph, const Dev &d, unsigned flags, hipStream_t st) {
+    LAUNCH(k_colloid, ph.n_rep, 256, 0, st, ph, d, flags);
+}
+void launch_md_candidates(const Phys &ph, const Dev &d, const int2 *wgt, int nwg, hipStream_t st) {
+    LAUNCH(k_md_candidates, nwg, 256, 0, st, ph, d, wgt);
+}
+void launch_copy(const void *a, void *b, long long nbytes, hipStream_t st) {
+    long long n = nbytes / 16;
+    LAUNCH(k_copy, 2048, 256, 0, st, (const float4 *)a, (float4 *)b, n);
+}
+
+#define INST(Real)                                                                                                 \
+    template void launch_depart<Real>(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);               \
+    template void launch_srd<Real>(const Phys &, const Dev &, const Fluid<Real> &, const Fluid<Real> &, bool,    \
+                                   bool, size_t, hipStream_t);                                                     \
+    template void launch_stream<Real>(const Phys &, const Dev &, const Fluid<Real> &, bool, bool, bool,           \
+                                      hipStream_t);                                                                \
+    template void launch_collide<Real>(const Phys &, const Dev &, const Fluid<Real> &, int, unsigned,             \
+                                       const int2 *, int, hipStream_t);                                            \
+    template void launch_run_count<Real>(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);            \
+    template void launch_maps<Real>(const Phys &, const Dev &, const Fluid<Real> &, long long, int *,             \
+                                    const int *, hipStream_t);                                                     \
+    template void launch_audit<Real>(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);                \
+    template void launch_gather<Real>(const Phys &, const Dev &, const Fluid<Real> &, double *, double *, int *,  \
+                                      hipStream_t);
+INST(float)
+INST(double)
+
+}  // namespace bw
