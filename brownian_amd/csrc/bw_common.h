@@ -13,7 +13,8 @@
 namespace bw {
 
 constexpr int EV_CAP = 64;          // reflection events per colloid per substep
-constexpr int CHAIN_CAP = 512;      // chain particles per replica per substep
+constexpr int CHAIN_CAP = 1024;     // serially replayed particles per replica per substep
+constexpr int SERIAL_CAP = 2048;    // serial list incl. entries of overflowed buckets
 constexpr int TCL_CAP = 128;        // colloids listed for one tile
 constexpr int MD_CAP = 128;         // full MD neighbour list per colloid
 constexpr int MDC_CAP = 256;        // MD candidates (cutoff + skin) per colloid

@@ -77,16 +77,19 @@ __device__ inline bool draw_reflection(const Dev &d, const Phys &ph, const Rep &
             }
         return false;
     }
+    // keyed by the replica-local colloid index: a replica draws the same
+    // numbers alone or inside an ensemble
     uint32_t c0 = KIND_REFL | ((uint32_t)substep << 8);
-    uint4 a = philox(make_uint4(c0, (uint32_t)step, (uint32_t)pid, (uint32_t)g), rep_key(R));
-    uint4 b = philox(make_uint4(c0 | (1u << 16), (uint32_t)step, (uint32_t)pid, (uint32_t)g), rep_key(R));
+    const uint32_t jl = (uint32_t)(g - R.first_colloid);
+    uint4 a = philox(make_uint4(c0, (uint32_t)step, (uint32_t)pid, jl), rep_key(R));
+    uint4 b = philox(make_uint4(c0 | (1u << 16), (uint32_t)step, (uint32_t)pid, jl), rep_key(R));
     o.v[0] = u01(a.x, a.y);
     o.v[1] = u01(a.z, a.w);
     o.v[2] = u01(b.x, b.y);
     o.v[3] = u01(b.z, b.w);
     // polar pair by rejection, as the reference's loop (:17-20); z = 0 also rejected
     for (uint32_t t = 2; t < 65535u; t++) {
-        uint4 c = philox(make_uint4(c0 | (t << 16), (uint32_t)step, (uint32_t)pid, (uint32_t)g), rep_key(R));
+        uint4 c = philox(make_uint4(c0 | (t << 16), (uint32_t)step, (uint32_t)pid, jl), rep_key(R));
         double aa = u01(c.x, c.y), bb = u01(c.z, c.w);
         double x1 = 2.0 * aa - 1, x2 = 2.0 * bb - 1, z = x1 * x1 + x2 * x2;
         if (z <= 1 && z > 0) {

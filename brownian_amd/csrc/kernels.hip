@@ -421,7 +421,7 @@ __global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid
     const Real dt = (Real)ph.dt;
     const int lox = tg.tx * R.B[0] - TILE_REACH, loy = tg.ty * R.B[1] - TILE_REACH, loz = tg.tz * R.B[2] - TILE_REACH;
     const int lnx = R.B[0] + 2 * TILE_REACH, lny = R.B[1] + 2 * TILE_REACH, lnz = R.B[2] + 2 * TILE_REACH;
-    bool drift = false, ev_over = false, ch_over = false;
+    bool drift = false, ch_over = false;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int slot = T * ph.cap + i;
         int ix, iy, iz;
@@ -472,11 +472,10 @@ __global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid
                 }
             }
             if (n_in > 0) {
-                if (n_near == 1) {
-                    int e = atomicAdd(&d.ev_cnt[gin], 1);
-                    if (e < EV_CAP) d.ev_slot[gin * EV_CAP + e] = slot;
-                    else ev_over = true;
-                } else {
+                int e = n_near == 1 ? atomicAdd(&d.ev_cnt[gin], 1) : EV_CAP;
+                if (e < EV_CAP) {
+                    d.ev_slot[gin * EV_CAP + e] = slot;
+                } else {  // shared particle, or a colloid with a full bucket: serial replay
                     int e = atomicAdd(&d.chain_cnt[tg.r], 1);
                     if (e < CHAIN_CAP) d.chain_slot[tg.r * CHAIN_CAP + e] = slot;
                     else ch_over = true;
@@ -485,7 +484,6 @@ __global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid
         }
     }
     if (drift) atomicOr(d.status, BW_FLAG_DRIFT);
-    if (ev_over) atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
     if (ch_over) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
 }
 
@@ -540,7 +538,8 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     __shared__ double s_c[COLLIDE_THREADS / 64][EV_CAP][6];
     __shared__ int s_ne[COLLIDE_THREADS / 64];
     __shared__ int s_last;
-    __shared__ int s_chain[CHAIN_CAP];
+    __shared__ int s_chain[SERIAL_CAP];
+    __shared__ int s_nser;
     const bool fp32 = sizeof(Real) == 4;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int2 wt = wgt[blockIdx.x];
@@ -553,10 +552,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     int ne = 0;
     if (g < gend) {
         ne = d.ev_cnt[g];
-        if (ne > EV_CAP) {
-            if (lane == 0) atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
-            ne = EV_CAP;
-        }
+        if (ne > EV_CAP) ne = 0;  // full bucket: all its events are replayed serially
     }
     int slot = -1, pid = 0, ix = 0, iy = 0, iz = 0;
     long long key = 0x7fffffffffffffffll;
@@ -619,10 +615,25 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     __threadfence();
     // chain particles, serially in ascending id, each tested against every
     // colloid of the replica in ascending order (the reference's colloid loop)
-    int nch = min(d.chain_cnt[r], CHAIN_CAP);
-    if (d.chain_cnt[r] > CHAIN_CAP && threadIdx.x == 0) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
-    for (int i = threadIdx.x; i < nch; i += blockDim.x) s_chain[i] = d.chain_slot[r * CHAIN_CAP + i];
+    // serial list: shared particles plus every event of a colloid whose bucket overflowed
+    if (threadIdx.x == 0) {
+        int n = min(d.chain_cnt[r], CHAIN_CAP);
+        if (d.chain_cnt[r] > CHAIN_CAP) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
+        for (int i = 0; i < n; i++) s_chain[i] = d.chain_slot[r * CHAIN_CAP + i];
+        for (int gg = R.first_colloid; gg < gend; gg++)
+            if (d.ev_cnt[gg] > EV_CAP)
+                for (int e = 0; e < EV_CAP; e++) {
+                    if (n < SERIAL_CAP) s_chain[n] = d.ev_slot[gg * EV_CAP + e];
+                    n++;
+                }
+        if (n > SERIAL_CAP) {
+            atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
+            n = SERIAL_CAP;
+        }
+        s_nser = n;
+    }
     __syncthreads();
+    const int nch = s_nser;
     if (threadIdx.x == 0) {
         for (int i = 1; i < nch; i++) {
             int s = s_chain[i], p = f.pid[s], j = i - 1;
@@ -671,7 +682,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             for (int gg = R.first_colloid; gg < gend; gg++) {
                 int base = substep * ph.n_colloid_total + gg;
                 n_inj += d.inj_off[base + 1] - d.inj_off[base];
-                n_ev += min(d.ev_cnt[gg], EV_CAP);
+                n_ev += d.ev_cnt[gg] > EV_CAP ? 0 : d.ev_cnt[gg];
             }
             if (n_inj != n_ev) atomicOr(d.status, BW_FLAG_DRAW_MISMATCH);
         }
