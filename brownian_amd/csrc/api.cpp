@@ -161,7 +161,7 @@ Dev dev_for(bw_ctx *c, int buf) {
 
 int check_status_code(unsigned s) {
     if (s & (BW_FLAG_DRAW_MISSING | BW_FLAG_DRAW_MISMATCH)) return BW_ERR_DRAWS;
-    if (s & (BW_FLAG_DRIFT | BW_FLAG_ZERO_VARIANCE | BW_FLAG_CHAIN_DISPLACEMENT)) return BW_ERR_PHYSICS;
+    if (s & (BW_FLAG_DRIFT | BW_FLAG_ZERO_VARIANCE)) return BW_ERR_PHYSICS;
     return BW_ERR_CAPACITY;
 }
 
@@ -390,7 +390,6 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
     ph.lim_shell = pow(p0.sigma * 0.5 + 0.5, 2);        // cpp/run.cpp:18
     ph.lim_md = pow(3.0 * p0.sig_colloid, 2);           // cpp/neighbour_list.cpp:28,34
     ph.lim_md_skin = pow(3.0 * p0.sig_colloid + MD_SKIN, 2);
-    ph.lim_chain = pow(p0.sigma * 0.5 + CHAIN_MARGIN, 2);
     ph.vscale_thermal = sqrt(p0.kbt / p0.mass_fl);
     ph.draws = p0.draws;
     ph.n_rep = n_rep;
@@ -519,6 +518,9 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         (rc = dalloc(c, &d.mdc_list, NC * MDC_CAP)) || (rc = dalloc(c, &d.cpot, n_rep)) ||
         (rc = dalloc(c, &d.ev_cnt, NC)) || (rc = dalloc(c, &d.ev_slot, NC * EV_CAP)) ||
         (rc = dalloc(c, &d.chain_cnt, n_rep)) || (rc = dalloc(c, &d.chain_slot, (size_t)n_rep * CHAIN_CAP)) ||
+        (rc = dalloc(c, &d.cont_cnt, n_rep)) || (rc = dalloc(c, &d.cont_slot, (size_t)n_rep * CONT_CAP)) ||
+        (rc = dalloc(c, &d.cont_from, (size_t)n_rep * CONT_CAP)) ||
+        (rc = dalloc(c, &d.cont_dump, (size_t)n_rep * CONT_CAP * 3)) ||
         (rc = dalloc(c, &d.ticket, n_rep)) || (rc = dalloc(c, &d.status, 1)) || (rc = dalloc(c, &d.step, 1)) ||
         (rc = dalloc(c, &d.shift, 3 * n_rep)) || (rc = dalloc(c, &d.audit_partial, (size_t)tiles * 8))) {
         *out = c;
@@ -753,6 +755,7 @@ BW_API int initialize(bw_ctx *c) {
     for (double *a : arrs) HIPCHK(c, hipMemset(a, 0, sizeof(double) * 3 * NC));
     HIPCHK(c, hipMemset(c->dv.ev_cnt, 0, sizeof(int) * NC));
     HIPCHK(c, hipMemset(c->dv.chain_cnt, 0, sizeof(int) * c->n_rep));
+    HIPCHK(c, hipMemset(c->dv.cont_cnt, 0, sizeof(int) * c->n_rep));
     HIPCHK(c, hipMemset(c->dv.ticket, 0, sizeof(unsigned) * c->n_rep));
     c->pending_prologue = true;
     c->substep = 0;

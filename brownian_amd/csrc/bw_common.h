@@ -14,6 +14,7 @@ namespace bw {
 
 constexpr int EV_CAP = 64;          // reflection events per colloid per substep
 constexpr int CHAIN_CAP = 1024;     // serially replayed particles per replica per substep
+constexpr int CONT_CAP = 512;       // continuations per replica per substep
 constexpr int SERIAL_CAP = 2048;    // serial list incl. entries of overflowed buckets
 constexpr int TCL_CAP = 128;        // colloids listed for one tile
 constexpr int MD_CAP = 128;         // full MD neighbour list per colloid
@@ -26,7 +27,6 @@ constexpr int DEPART_THREADS = 256;
 constexpr int COLLIDE_THREADS = 256; // 4 colloids (one wave each) per workgroup
 constexpr int NDIR = 27;
 constexpr int TILE_REACH = 3;       // cells a particle may stand outside its tile box
-constexpr double CHAIN_MARGIN = 0.5; // extra radius that marks a particle as shared
 constexpr double MD_SKIN = 2.0;     // candidate-list skin for the colloid pair list
 
 // colloid phase flags (applied in this order)
@@ -52,7 +52,6 @@ struct Phys {
     double lim_shell;                 // pow(sigma*0.5+0.5,2)
     double lim_md;                    // pow(3*sig_colloid,2)
     double lim_md_skin;               // (3*sig_colloid + skin)^2
-    double lim_chain;                 // (sigma/2 + margin)^2
     double vscale_thermal;            // sqrt(kbt/mass_fl)
     int n_sub;
     int cap;                          // slots per tile
@@ -92,6 +91,8 @@ struct Dev {
     int *md_cnt, *md_list, *mdc_cnt, *mdc_list;
     double *cpot;        // [n_rep] potential
     int *ev_cnt, *ev_slot, *chain_cnt, *chain_slot;
+    int *cont_cnt, *cont_slot, *cont_from;  // continuations of the colloid loop
+    double *cont_dump;
     unsigned *ticket;    // [n_rep]
     unsigned *status;
     long long *step;     // MPCD step counter (device generator key)

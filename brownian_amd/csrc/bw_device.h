@@ -449,16 +449,24 @@ __device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R
     }
     __syncthreads();
     if (flags & PH_MDLIST) {
-        // exact pair list: candidates (ascending) with |img(p_i-p_j)|^2 < (3 sigma)^2
+        // exact pair list: partners (ascending) with |img(p_i-p_j)|^2 < (3 sigma)^2,
+        // filtered from the candidate list while no colloid of the replica has
+        // moved more than half the skin since it was built, else by a full scan
+        bool moved = false;
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+            int g = g0 + i;
+            double mx = img1(d.cpos[3 * g] - d.cpos_md[3 * g], L), my = img1(d.cpos[3 * g + 1] - d.cpos_md[3 * g + 1], L),
+                   mz = img1(d.cpos[3 * g + 2] - d.cpos_md[3 * g + 2], L);
+            if ((mx * mx + my * my) + mz * mz > 0.25 * MD_SKIN * MD_SKIN) moved = true;
+        }
+        const bool full = __syncthreads_or(moved);
         for (int i = threadIdx.x; i < nc; i += blockDim.x) {
             int g = g0 + i;
             const double *pi = d.cpos + 3 * g;
-            double mx = img1(pi[0] - d.cpos_md[3 * g], L), my = img1(pi[1] - d.cpos_md[3 * g + 1], L),
-                   mz = img1(pi[2] - d.cpos_md[3 * g + 2], L);
-            if ((mx * mx + my * my) + mz * mz > 0.25 * MD_SKIN * MD_SKIN) atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
-            int nc_ = d.mdc_cnt[g], w = 0;
-            for (int m = 0; m < nc_; m++) {
-                int j = d.mdc_list[g * MDC_CAP + m];
+            int ncand = full ? nc : d.mdc_cnt[g], w = 0;
+            for (int m = 0; m < ncand; m++) {
+                int j = full ? g0 + m : d.mdc_list[g * MDC_CAP + m];
+                if (j == g) continue;
                 const double *pj = d.cpos + 3 * j;
                 double tx, ty, tz;
                 if (j > g) {
@@ -498,7 +506,10 @@ __device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R
     }
     if (flags & PH_RESET_EVENTS) {
         for (int i = threadIdx.x; i < nc; i += blockDim.x) d.ev_cnt[g0 + i] = 0;
-        if (threadIdx.x == 0) d.chain_cnt[r] = 0;
+        if (threadIdx.x == 0) {
+            d.chain_cnt[r] = 0;
+            d.cont_cnt[r] = 0;
+        }
     }
     __syncthreads();
 }

@@ -458,26 +458,22 @@ __global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid
         }
         if (DETECT) {
             double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
-            int n_in = 0, n_near = 0, gin = -1;
+            int n_in = 0, gin = -1;
             for (int k = 0; k < tc.n; k++) {
                 if (!in_stencil(ix, iy, iz, tc.cx[k], tc.cy[k], tc.cz[k], L)) continue;
                 double rx = img1(tc.px[k] - x, Ld), ry = img1(tc.py[k] - y, Ld), rz = img1(tc.pz[k] - z, Ld);
-                double d2 = (rx * rx + ry * ry) + rz * rz;
-                if (d2 <= ph.lim_chain) {
-                    n_near++;
-                    if (d2 <= ph.lim_coll) {
-                        n_in++;
-                        gin = tc.g[k];
-                    }
+                if ((rx * rx + ry * ry) + rz * rz <= ph.lim_coll) {
+                    n_in++;
+                    gin = tc.g[k];
                 }
             }
             if (n_in > 0) {
-                int e = n_near == 1 ? atomicAdd(&d.ev_cnt[gin], 1) : EV_CAP;
+                int e = n_in == 1 ? atomicAdd(&d.ev_cnt[gin], 1) : EV_CAP;
                 if (e < EV_CAP) {
                     d.ev_slot[gin * EV_CAP + e] = slot;
-                } else {  // shared particle, or a colloid with a full bucket: serial replay
-                    int e = atomicAdd(&d.chain_cnt[tg.r], 1);
-                    if (e < CHAIN_CAP) d.chain_slot[tg.r * CHAIN_CAP + e] = slot;
+                } else {  // inside several colloids, or a colloid with a full bucket: serial replay
+                    int e2 = atomicAdd(&d.chain_cnt[tg.r], 1);
+                    if (e2 < CHAIN_CAP) d.chain_slot[tg.r * CHAIN_CAP + e2] = slot;
                     else ch_over = true;
                 }
             }
@@ -538,7 +534,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     __shared__ double s_c[COLLIDE_THREADS / 64][EV_CAP][6];
     __shared__ int s_ne[COLLIDE_THREADS / 64];
     __shared__ int s_last;
-    __shared__ int s_chain[SERIAL_CAP];
+    __shared__ int s_chain[SERIAL_CAP], s_src[SERIAL_CAP];
     __shared__ int s_nser;
     const bool fp32 = sizeof(Real) == 4;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -572,7 +568,6 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     if (lane < ne) {
         double bx = f.ox[slot], by = f.oy[slot], bz = f.oz[slot];
         double vx = f.vx[slot], vy = f.vy[slot], vz = f.vz[slot];
-        double bx0 = bx, by0 = by, bz0 = bz;
         ReflDraw dr;
         double cont[6] = {0, 0, 0, 0, 0, 0};
         const double dump[3] = {vx, vy, vz};
@@ -583,9 +578,31 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             transfer(dump, vx, vy, vz, rs, cont);
             f.ox[slot] = (Real)bx; f.oy[slot] = (Real)by; f.oz[slot] = (Real)bz;
             f.vx[slot] = (Real)vx; f.vy[slot] = (Real)vy; f.vz[slot] = (Real)vz;
-            double mx = bx - bx0, my = by - by0, mz = bz - bz0;
-            if ((mx * mx + my * my) + mz * mz > 0.25 * CHAIN_MARGIN * CHAIN_MARGIN)
-                atomicOr(d.status, BW_FLAG_CHAIN_DISPLACEMENT);
+            // The reference goes on to test this particle, now moved, against
+            // every later colloid (cpp/fluid_colloid_collision.cpp:30-35).  A
+            // later colloid that now contains it is a pair partner of g; if
+            // there is one, the rest of the colloid loop is replayed serially.
+            const double ax = (double)ix + bx, ay = (double)iy + by, az = (double)iz + bz;
+            const double Ld = (double)L;
+            for (int m = 0; m < d.md_cnt[g]; m++) {
+                int j = d.md_list[g * MD_CAP + m];
+                if (j <= g) continue;
+                if (!in_stencil(ix, iy, iz, d.ccell[3 * j], d.ccell[3 * j + 1], d.ccell[3 * j + 2], L)) continue;
+                double rx = img1(d.cpos[3 * j] - ax, Ld), ry = img1(d.cpos[3 * j + 1] - ay, Ld),
+                       rz = img1(d.cpos[3 * j + 2] - az, Ld);
+                if ((rx * rx + ry * ry) + rz * rz <= ph.lim_coll) {
+                    int e = atomicAdd(&d.cont_cnt[r], 1);
+                    if (e < CONT_CAP) {
+                        int q = r * CONT_CAP + e;
+                        d.cont_slot[q] = slot;
+                        d.cont_from[q] = g + 1;
+                        for (int kk = 0; kk < 3; kk++) d.cont_dump[3 * q + kk] = dump[kk];
+                    } else {
+                        atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
+                    }
+                    break;
+                }
+            }
         } else {
             atomicOr(d.status, BW_FLAG_DRAW_MISSING);
         }
@@ -615,45 +632,53 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     __threadfence();
     // chain particles, serially in ascending id, each tested against every
     // colloid of the replica in ascending order (the reference's colloid loop)
-    // serial list: shared particles plus every event of a colloid whose bucket overflowed
+    // Serial list, replayed in ascending particle id: particles inside several
+    // colloids and every event of a colloid whose bucket overflowed (tested
+    // from the replica's first colloid on, pre-collision velocity as dump), and
+    // continuations (from the colloid after the one that reflected them, with
+    // the velocity they had before that reflection as dump).
     if (threadIdx.x == 0) {
-        int n = min(d.chain_cnt[r], CHAIN_CAP);
+        int n = 0;
+        int nch = min(d.chain_cnt[r], CHAIN_CAP);
         if (d.chain_cnt[r] > CHAIN_CAP) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
-        for (int i = 0; i < n; i++) s_chain[i] = d.chain_slot[r * CHAIN_CAP + i];
+        for (int i = 0; i < nch && n < SERIAL_CAP; i++, n++) { s_chain[n] = d.chain_slot[r * CHAIN_CAP + i]; s_src[n] = -1; }
         for (int gg = R.first_colloid; gg < gend; gg++)
             if (d.ev_cnt[gg] > EV_CAP)
-                for (int e = 0; e < EV_CAP; e++) {
-                    if (n < SERIAL_CAP) s_chain[n] = d.ev_slot[gg * EV_CAP + e];
-                    n++;
-                }
-        if (n > SERIAL_CAP) {
-            atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
-            n = SERIAL_CAP;
-        }
+                for (int e = 0; e < EV_CAP && n < SERIAL_CAP; e++, n++) { s_chain[n] = d.ev_slot[gg * EV_CAP + e]; s_src[n] = -1; }
+        int ncont = min(d.cont_cnt[r], CONT_CAP);
+        for (int i = 0; i < ncont && n < SERIAL_CAP; i++, n++) { s_chain[n] = d.cont_slot[r * CONT_CAP + i]; s_src[n] = r * CONT_CAP + i; }
+        if (n >= SERIAL_CAP) atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
         s_nser = n;
     }
     __syncthreads();
     const int nch = s_nser;
     if (threadIdx.x == 0) {
         for (int i = 1; i < nch; i++) {
-            int s = s_chain[i], p = f.pid[s], j = i - 1;
+            int s = s_chain[i], q = s_src[i], p = f.pid[s], j = i - 1;
             while (j >= 0 && f.pid[s_chain[j]] > p) {
                 s_chain[j + 1] = s_chain[j];
+                s_src[j + 1] = s_src[j];
                 j--;
             }
             s_chain[j + 1] = s;
+            s_src[j + 1] = q;
         }
         const double Ld = (double)L;
         int n_chain_events = 0;
         for (int i = 0; i < nch; i++) {
-            int s = s_chain[i];
+            int s = s_chain[i], q = s_src[i];
             int p = f.pid[s];
             int cx, cy, cz;
             unpack_cell(f.cell[s], L, cx, cy, cz);
             double bx = f.ox[s], by = f.oy[s], bz = f.oz[s];
             double vx = f.vx[s], vy = f.vy[s], vz = f.vz[s];
-            const double dump[3] = {vx, vy, vz};
-            for (int gg = R.first_colloid; gg < gend; gg++) {
+            double dump[3] = {vx, vy, vz};
+            int from = R.first_colloid;
+            if (q >= 0) {
+                from = d.cont_from[q];
+                for (int kk = 0; kk < 3; kk++) dump[kk] = d.cont_dump[3 * q + kk];
+            }
+            for (int gg = from; gg < gend; gg++) {
                 if (!in_stencil(cx, cy, cz, d.ccell[3 * gg], d.ccell[3 * gg + 1], d.ccell[3 * gg + 2], L)) continue;
                 const double *pc = d.cpos + 3 * gg;
                 double ax = (double)cx + bx, ay = (double)cy + by, az = (double)cz + bz;
@@ -668,9 +693,9 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
                 reflect_event(ph, L, cx, cy, cz, bx, by, bz, vx, vy, vz, pc, d.cvel + 3 * gg, d.comega + 3 * gg, dr,
                               rs, fp32);
                 transfer(dump, vx, vy, vz, rs, c6);
-                for (int k = 0; k < 3; k++) {
-                    d.cdv[3 * gg + k] += c6[k];
-                    d.cdw[3 * gg + k] += c6[3 + k];
+                for (int kk = 0; kk < 3; kk++) {
+                    d.cdv[3 * gg + kk] += c6[kk];
+                    d.cdw[3 * gg + kk] += c6[3 + kk];
                 }
                 n_chain_events++;
             }

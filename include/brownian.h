@@ -34,7 +34,7 @@ extern "C" {
 
 /* device status bits (bw_status) */
 #define BW_FLAG_EVENT_OVERFLOW 0x1u    /* > event capacity for one colloid in a substep */
-#define BW_FLAG_CHAIN_OVERFLOW 0x2u    /* > chain-particle capacity                      */
+#define BW_FLAG_CHAIN_OVERFLOW 0x2u    /* > serial-replay capacity (shared particles)    */
 #define BW_FLAG_TILE_OVERFLOW 0x4u     /* a tile received more particles than its slots  */
 #define BW_FLAG_LIST_OVERFLOW 0x8u     /* colloid list of a tile / MD list overflow       */
 #define BW_FLAG_DRIFT 0x10u            /* particle moved more than a tile in one step     */
@@ -42,7 +42,6 @@ extern "C" {
 #define BW_FLAG_DRAW_MISMATCH 0x40u    /* injected reflection list differs from events    */
 #define BW_FLAG_ZERO_VARIANCE 0x80u    /* thermostat on a cell with zero relative KE      */
 #define BW_FLAG_NEIGH_OVERFLOW 0x100u  /* neigh_fl capacity exceeded                      */
-#define BW_FLAG_CHAIN_DISPLACEMENT 0x200u /* chain particle displaced beyond the margin   */
 
 enum bw_precision { BW_FP32 = 0, BW_FP64 = 1 };
 enum bw_vel_law { BW_VEL_REFERENCE = 0, BW_VEL_MAXWELL = 1 };

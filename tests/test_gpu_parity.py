@@ -195,6 +195,7 @@ def test_fp32_per_step_vs_oracle():
     s = B.System(precision=B.BW_FP32, draws=B.BW_DRAWS_INJECTED, **prm)
     s.startup()
     o = O.Oracle(**prm)
+    o.create_box()
     o.set_rng_state(s.rng_state())
     for step in range(3):
         pos, vel, _ = s.download_fluid()
