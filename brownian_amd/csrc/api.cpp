@@ -32,6 +32,7 @@ void launch_gather(const Phys &, const Dev &, const Fluid<Real> &, double *, dou
 void launch_step_begin(const Phys &, const Dev &, hipStream_t);
 void launch_colloid(const Phys &, const Dev &, unsigned, hipStream_t);
 void launch_md_candidates(const Phys &, const Dev &, const int2 *, int, hipStream_t);
+void launch_tile_colloids(const Phys &, const Dev &, hipStream_t);
 void launch_copy(const void *, void *, long long, hipStream_t);
 }  // namespace bw
 
@@ -260,7 +261,10 @@ void enqueue_step(bw_ctx *c, int cur, bool final, Prof *prof) {
     Fluid<Real> &src = fluid_of<Real>(c, cur), &dst = fluid_of<Real>(c, 1 - cur);
     if (prof) prof->mark(st, BW_KERNEL_OTHER);
     launch_step_begin(ph, d0, st);
-    if (c->nc_total > 0) launch_md_candidates(ph, d0, c->wgt, c->nwg, st);
+    if (c->nc_total > 0) {
+        launch_md_candidates(ph, d0, c->wgt, c->nwg, st);
+        launch_tile_colloids(ph, d0, st);
+    }
     if (prof) prof->mark(st, BW_KERNEL_COLLOID);
     launch_depart<Real>(ph, d0, src, st);
     if (prof) prof->mark(st, BW_KERNEL_DEPART);
@@ -513,7 +517,9 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         (rc = dalloc(c, &d.cdisp, 3 * NC)) || (rc = dalloc(c, &d.cpos_run, 3 * NC)) ||
         (rc = dalloc(c, &d.cdv, 3 * NC)) || (rc = dalloc(c, &d.cdw, 3 * NC)) || (rc = dalloc(c, &d.cpos_md, 3 * NC)) ||
         (rc = dalloc(c, &d.cv0, NC)) || (rc = dalloc(c, &d.crep, NC)) || (rc = dalloc(c, &d.ccell, 3 * NC)) ||
-        (rc = dalloc(c, &d.ccell_run, 3 * NC)) || (rc = dalloc(c, &d.ccnt, NC)) || (rc = dalloc(c, &d.md_cnt, NC)) ||
+        (rc = dalloc(c, &d.ccell_run, 3 * NC)) || (rc = dalloc(c, &d.ccell_md, 3 * NC)) ||
+        (rc = dalloc(c, &d.tcl_cnt, tiles)) || (rc = dalloc(c, &d.tcl_list, (size_t)tiles * TCL_CAP)) ||
+        (rc = dalloc(c, &d.ccnt, NC)) || (rc = dalloc(c, &d.md_cnt, NC)) ||
         (rc = dalloc(c, &d.md_list, NC * MD_CAP)) || (rc = dalloc(c, &d.mdc_cnt, NC)) ||
         (rc = dalloc(c, &d.mdc_list, NC * MDC_CAP)) || (rc = dalloc(c, &d.cpot, n_rep)) ||
         (rc = dalloc(c, &d.ev_cnt, NC)) || (rc = dalloc(c, &d.ev_slot, NC * EV_CAP)) ||
@@ -667,6 +673,7 @@ int set_colloid_cells(bw_ctx *c, int r) {
     size_t o = 3 * (size_t)R.first_colloid;
     HIPCHK(c, hipMemcpy(c->dv.ccell + o, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dv.ccell_run + o, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->dv.ccell_md + o, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dv.cpos_run + o, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dv.cpos_md + o, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice));
     return BW_OK;
@@ -868,6 +875,7 @@ BW_API int run(bw_ctx *c) {
     Dev d = dev_for(c, c->cur);
     launch_colloid(c->ph, d, PH_KICK, c->st);
     if (c->nc_total == 0) return BW_OK;
+    launch_tile_colloids(c->ph, d, c->st);
     if (c->fp64) {
         launch_run_count<double>(c->ph, d, c->fl[c->cur].f64, c->st);
         launch_stream<double>(c->ph, d, c->fl[c->cur].f64, true, false, false, c->st);
@@ -893,6 +901,7 @@ BW_API int fluid_colloid_collision(bw_ctx *c) {
     if (c->nc_total == 0) return BW_OK;
     Dev d = dev_for(c, c->cur);
     int l = c->substep % c->ph.n_sub;
+    launch_tile_colloids(c->ph, d, c->st);
     if (c->fp64) {
         launch_stream<double>(c->ph, d, c->fl[c->cur].f64, false, false, true, c->st);
         launch_collide<double>(c->ph, d, c->fl[c->cur].f64, l, PH_APPLY, c->wgt, c->nwg, c->st);

@@ -87,7 +87,8 @@ struct Dev {
     // colloids (global index g), FP64, 3 per colloid
     double *cpos, *cvel, *comega, *cf, *coldf, *cra, *cdisp, *cpos_run, *cdv, *cdw, *cpos_md;
     double *cv0;
-    int *crep, *ccell, *ccell_run, *ccnt;
+    int *crep, *ccell, *ccell_run, *ccell_md, *ccnt;
+    int *tcl_cnt, *tcl_list;  // per tile: colloids that can touch it this step
     int *md_cnt, *md_list, *mdc_cnt, *mdc_list;
     double *cpot;        // [n_rep] potential
     int *ev_cnt, *ev_slot, *chain_cnt, *chain_slot;

@@ -256,60 +256,36 @@ __device__ __forceinline__ bool near_interval(int c, int lo, int len, int L) {
     return d < len;
 }
 
-// List (ascending g) the replica's colloids that can touch a particle of tile
-// (tx,ty,tz): a particle stays within TILE_REACH cells of its tile box and the
-// collision / run() stencils reach 3 cells, so the colloid cell must lie within
-// the tile box grown by TILE_REACH + 3 (per axis, periodic).
-__device__ inline void build_tile_colloids(const Dev &d, const Phys &ph, const Rep &R, int tx, int ty, int tz,
-                                           bool want_cur, bool want_run, bool kick_data, TileColl &tc) {
-    const int reach = TILE_REACH + 3;
-    int lox = tx * R.B[0] - reach, loy = ty * R.B[1] - reach, loz = tz * R.B[2] - reach;
-    int lnx = R.B[0] + 2 * reach, lny = R.B[1] + 2 * reach, lnz = R.B[2] + 2 * reach;
-    if (threadIdx.x == 0) tc.n = 0;
-    __syncthreads();
-    for (int base = 0; base < R.nc; base += blockDim.x) {
-        int i = base + threadIdx.x;
-        bool p = false;
-        int g = R.first_colloid + i;
-        if (i < R.nc) {
-            if (want_cur)
-                p = near_interval(d.ccell[3 * g], lox, lnx, R.L) && near_interval(d.ccell[3 * g + 1], loy, lny, R.L) &&
-                    near_interval(d.ccell[3 * g + 2], loz, lnz, R.L);
-            if (want_run && !p)
-                p = near_interval(d.ccell_run[3 * g], lox, lnx, R.L) &&
-                    near_interval(d.ccell_run[3 * g + 1], loy, lny, R.L) &&
-                    near_interval(d.ccell_run[3 * g + 2], loz, lnz, R.L);
+// Colloids that can touch a particle of a tile during one MPCD step: a
+// particle stays within TILE_REACH cells of its tile box, the collision and
+// run() stencils reach 3 cells and a colloid's cell moves by at most one cell
+// within a step, so the colloid's cell at the start of the step lies within the
+// tile box grown by TILE_REACH + 4 (per axis, periodic).  Built once per step
+// by k_tile_colloids (ascending g); loaded here with the colloids' current data.
+constexpr int TILE_LIST_REACH = TILE_REACH + 4;
+
+__device__ inline void load_tile_colloids(const Dev &d, const Phys &ph, int T, bool kick_data, TileColl &tc) {
+    const int n = min(d.tcl_cnt[T], TCL_CAP);
+    for (int slot = threadIdx.x; slot < n; slot += blockDim.x) {
+        int g = d.tcl_list[T * TCL_CAP + slot];
+        tc.g[slot] = g;
+        tc.px[slot] = d.cpos[3 * g]; tc.py[slot] = d.cpos[3 * g + 1]; tc.pz[slot] = d.cpos[3 * g + 2];
+        tc.cx[slot] = d.ccell[3 * g]; tc.cy[slot] = d.ccell[3 * g + 1]; tc.cz[slot] = d.ccell[3 * g + 2];
+        if (kick_data) {
+            tc.rx[slot] = d.cpos_run[3 * g]; tc.ry[slot] = d.cpos_run[3 * g + 1]; tc.rz[slot] = d.cpos_run[3 * g + 2];
+            tc.qx[slot] = d.ccell_run[3 * g]; tc.qy[slot] = d.ccell_run[3 * g + 1]; tc.qz[slot] = d.ccell_run[3 * g + 2];
+            tc.vx[slot] = d.cvel[3 * g]; tc.vy[slot] = d.cvel[3 * g + 1]; tc.vz[slot] = d.cvel[3 * g + 2];
+            // cpp/run.cpp:14,23: del*(mass_colloid/(mass_fl*cnt)) with del = ra*v0
+            int c = d.ccnt[g];
+            double fac = c > 0 ? ph.mass_colloid / (ph.mass_fl * c) : 0.0;
+            double v0 = d.cv0[g];
+            tc.kx[slot] = (d.cra[3 * g] * v0) * fac;
+            tc.ky[slot] = (d.cra[3 * g + 1] * v0) * fac;
+            tc.kz[slot] = (d.cra[3 * g + 2] * v0) * fac;
         }
-        int tot;
-        int idx = block_compact(p, tc.wtot, tot);
-        int slot = tc.n + idx;
-        if (p) {
-            if (slot < TCL_CAP) {
-                tc.g[slot] = g;
-                tc.px[slot] = d.cpos[3 * g]; tc.py[slot] = d.cpos[3 * g + 1]; tc.pz[slot] = d.cpos[3 * g + 2];
-                tc.cx[slot] = d.ccell[3 * g]; tc.cy[slot] = d.ccell[3 * g + 1]; tc.cz[slot] = d.ccell[3 * g + 2];
-                if (kick_data) {
-                    tc.rx[slot] = d.cpos_run[3 * g]; tc.ry[slot] = d.cpos_run[3 * g + 1];
-                    tc.rz[slot] = d.cpos_run[3 * g + 2];
-                    tc.qx[slot] = d.ccell_run[3 * g]; tc.qy[slot] = d.ccell_run[3 * g + 1];
-                    tc.qz[slot] = d.ccell_run[3 * g + 2];
-                    tc.vx[slot] = d.cvel[3 * g]; tc.vy[slot] = d.cvel[3 * g + 1]; tc.vz[slot] = d.cvel[3 * g + 2];
-                    // cpp/run.cpp:14,23: del*(mass_colloid/(mass_fl*cnt)) with del = ra*v0
-                    int c = d.ccnt[g];
-                    double fac = c > 0 ? ph.mass_colloid / (ph.mass_fl * c) : 0.0;
-                    double v0 = d.cv0[g];
-                    tc.kx[slot] = (d.cra[3 * g] * v0) * fac;
-                    tc.ky[slot] = (d.cra[3 * g + 1] * v0) * fac;
-                    tc.kz[slot] = (d.cra[3 * g + 2] * v0) * fac;
-                }
-            } else {
-                atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) tc.n = min(tc.n + tot, TCL_CAP);
-        __syncthreads();
     }
+    if (threadIdx.x == 0) tc.n = n;
+    __syncthreads();
 }
 
 // run() rear-shell test of listed colloid k for a particle at binned cell
@@ -443,7 +419,10 @@ __device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R
                 d.cdisp[3 * g + k] += dx;
                 p = mod1(p, L);
                 d.cpos[3 * g + k] = p;
-                d.ccell[3 * g + k] = coord_cell(p, R.L);
+                int cc = coord_cell(p, R.L);
+                d.ccell[3 * g + k] = cc;
+                int dm = pdiff(cc, d.ccell_md[3 * g + k], R.L);
+                if (dm > 1 || dm < -1) atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
             }
         }
     }

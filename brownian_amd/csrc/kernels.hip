@@ -70,8 +70,50 @@ __global__ __launch_bounds__(256) void k_md_candidates(Phys ph, Dev d, const int
     if (lane == 0) {
         if (cnt > MDC_CAP) { atomicOr(d.status, BW_FLAG_LIST_OVERFLOW); cnt = MDC_CAP; }
         d.mdc_cnt[g] = cnt;
-        for (int k = 0; k < 3; k++) d.cpos_md[3 * g + k] = pi[k];
+        for (int k = 0; k < 3; k++) {
+            d.cpos_md[3 * g + k] = pi[k];
+            d.ccell_md[3 * g + k] = d.ccell[3 * g + k];
+        }
     }
+}
+
+// per tile: the colloids (ascending) whose cell at the start of this step, or
+// at run() time, lies within the tile box grown by TILE_LIST_REACH
+__global__ __launch_bounds__(256) void k_tile_colloids(Phys ph, Dev d) {
+    __shared__ int s_wtot[4];
+    __shared__ int s_n;
+    const int T = blockIdx.x;
+    const int r = d.tile_rep[T];
+    const Rep R = d.rep[r];
+    const int lt = T - R.first_tile;
+    const int tx = lt % R.nt[0], ty = (lt / R.nt[0]) % R.nt[1], tz = lt / (R.nt[0] * R.nt[1]);
+    const int reach = TILE_LIST_REACH;
+    const int lox = tx * R.B[0] - reach, loy = ty * R.B[1] - reach, loz = tz * R.B[2] - reach;
+    const int lnx = R.B[0] + 2 * reach, lny = R.B[1] + 2 * reach, lnz = R.B[2] + 2 * reach;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (int base = 0; base < R.nc; base += blockDim.x) {
+        int i = base + threadIdx.x;
+        int g = R.first_colloid + i;
+        bool p = false;
+        if (i < R.nc) {
+            p = near_interval(d.ccell[3 * g], lox, lnx, R.L) && near_interval(d.ccell[3 * g + 1], loy, lny, R.L) &&
+                near_interval(d.ccell[3 * g + 2], loz, lnz, R.L);
+            p = p || (near_interval(d.ccell_run[3 * g], lox, lnx, R.L) &&
+                      near_interval(d.ccell_run[3 * g + 1], loy, lny, R.L) &&
+                      near_interval(d.ccell_run[3 * g + 2], loz, lnz, R.L));
+        }
+        int tot;
+        int idx = block_compact(p, s_wtot, tot);
+        if (p) {
+            if (s_n + idx < TCL_CAP) d.tcl_list[T * TCL_CAP + s_n + idx] = g;
+            else atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_n = min(s_n + tot, TCL_CAP);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d.tcl_cnt[T] = s_n;
 }
 
 // ------------------------------------------------------------ tile geometry
@@ -236,7 +278,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
         }
         s_pre[27] = acc;
     }
-    if (COUNT_SHELL) build_tile_colloids(d, ph, R, tg.tx, tg.ty, tg.tz, false, true, true, tc);
+    if (COUNT_SHELL) load_tile_colloids(d, ph, T, true, tc);
     __syncthreads();
     int n = s_pre[27];
     if (n > cap) {
@@ -417,7 +459,7 @@ __global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid
     const int L = tg.L;
     const double Ld = (double)L;
     const int n = min(d.count[T], ph.cap);
-    if (KICK || DETECT) build_tile_colloids(d, ph, R, tg.tx, tg.ty, tg.tz, DETECT, KICK, KICK, tc);
+    if (KICK || DETECT) load_tile_colloids(d, ph, T, KICK, tc);
     const Real dt = (Real)ph.dt;
     const int lox = tg.tx * R.B[0] - TILE_REACH, loy = tg.ty * R.B[1] - TILE_REACH, loz = tg.tz * R.B[2] - TILE_REACH;
     const int lnx = R.B[0] + 2 * TILE_REACH, lny = R.B[1] + 2 * TILE_REACH, lnz = R.B[2] + 2 * TILE_REACH;
@@ -535,7 +577,9 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     __shared__ int s_ne[COLLIDE_THREADS / 64];
     __shared__ int s_last;
     __shared__ int s_chain[SERIAL_CAP], s_src[SERIAL_CAP];
-    __shared__ int s_nser;
+    __shared__ int s_nser, s_cand_n, s_serial_events, s_inj_tot;
+    __shared__ int s_cand[TCL_CAP];
+    __shared__ int s_wt[COLLIDE_THREADS / 64];
     const bool fp32 = sizeof(Real) == 4;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int2 wt = wgt[blockIdx.x];
@@ -630,6 +674,10 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     __syncthreads();
     if (!s_last) return;
     __threadfence();
+    if (threadIdx.x == 0) {
+        s_serial_events = 0;
+        s_inj_tot = 0;
+    }
     // chain particles, serially in ascending id, each tested against every
     // colloid of the replica in ascending order (the reference's colloid loop)
     // Serial list, replayed in ascending particle id: particles inside several
@@ -638,80 +686,122 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     // continuations (from the colloid after the one that reflected them, with
     // the velocity they had before that reflection as dump).
     if (threadIdx.x == 0) {
-        int n = 0;
-        int nch = min(d.chain_cnt[r], CHAIN_CAP);
-        if (d.chain_cnt[r] > CHAIN_CAP) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
-        for (int i = 0; i < nch && n < SERIAL_CAP; i++, n++) { s_chain[n] = d.chain_slot[r * CHAIN_CAP + i]; s_src[n] = -1; }
-        for (int gg = R.first_colloid; gg < gend; gg++)
-            if (d.ev_cnt[gg] > EV_CAP)
-                for (int e = 0; e < EV_CAP && n < SERIAL_CAP; e++, n++) { s_chain[n] = d.ev_slot[gg * EV_CAP + e]; s_src[n] = -1; }
-        int ncont = min(d.cont_cnt[r], CONT_CAP);
-        for (int i = 0; i < ncont && n < SERIAL_CAP; i++, n++) { s_chain[n] = d.cont_slot[r * CONT_CAP + i]; s_src[n] = r * CONT_CAP + i; }
-        if (n >= SERIAL_CAP) atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
-        s_nser = n;
+        d.ticket[r] = 0;
+        s_nser = 0;
     }
     __syncthreads();
-    const int nch = s_nser;
-    if (threadIdx.x == 0) {
-        for (int i = 1; i < nch; i++) {
-            int s = s_chain[i], q = s_src[i], p = f.pid[s], j = i - 1;
-            while (j >= 0 && f.pid[s_chain[j]] > p) {
-                s_chain[j + 1] = s_chain[j];
-                s_src[j + 1] = s_src[j];
-                j--;
-            }
-            s_chain[j + 1] = s;
-            s_src[j + 1] = q;
+    bool over = false;
+    int my_ev = 0;
+    for (int i = threadIdx.x; i < R.nc; i += blockDim.x) {
+        int c = d.ev_cnt[R.first_colloid + i];
+        if (c > EV_CAP) over = true;
+        else my_ev += c;
+    }
+    const bool any_over = __syncthreads_or(over);
+    const int nchain = d.chain_cnt[r], ncont = d.cont_cnt[r];
+    if (any_over || nchain > 0 || ncont > 0) {
+        if (threadIdx.x == 0) {
+            int n = 0;
+            int nch = min(nchain, CHAIN_CAP);
+            if (nchain > CHAIN_CAP) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
+            for (int i = 0; i < nch && n < SERIAL_CAP; i++, n++) { s_chain[n] = d.chain_slot[r * CHAIN_CAP + i]; s_src[n] = -1; }
+            int nco = min(ncont, CONT_CAP);
+            if (ncont > CONT_CAP) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
+            for (int i = 0; i < nco && n < SERIAL_CAP; i++, n++) { s_chain[n] = d.cont_slot[r * CONT_CAP + i]; s_src[n] = r * CONT_CAP + i; }
+            s_nser = n;
         }
+        __syncthreads();
+        if (any_over)
+            for (int i = threadIdx.x; i < R.nc; i += blockDim.x) {
+                int gg = R.first_colloid + i;
+                if (d.ev_cnt[gg] > EV_CAP) {
+                    int b = atomicAdd(&s_nser, EV_CAP);
+                    for (int e = 0; e < EV_CAP; e++)
+                        if (b + e < SERIAL_CAP) { s_chain[b + e] = d.ev_slot[gg * EV_CAP + e]; s_src[b + e] = -1; }
+                }
+            }
+        __syncthreads();
+        if (s_nser > SERIAL_CAP) {
+            if (threadIdx.x == 0) atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
+        }
+        const int nser = min(s_nser, SERIAL_CAP);
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < nser; i++) {
+                int s = s_chain[i], q = s_src[i], p = f.pid[s], j = i - 1;
+                while (j >= 0 && f.pid[s_chain[j]] > p) {
+                    s_chain[j + 1] = s_chain[j];
+                    s_src[j + 1] = s_src[j];
+                    j--;
+                }
+                s_chain[j + 1] = s;
+                s_src[j + 1] = q;
+            }
+        }
+        __syncthreads();
         const double Ld = (double)L;
-        int n_chain_events = 0;
-        for (int i = 0; i < nch; i++) {
-            int s = s_chain[i], q = s_src[i];
-            int p = f.pid[s];
+        for (int i = 0; i < nser; i++) {
+            const int s = s_chain[i], q = s_src[i];
             int cx, cy, cz;
             unpack_cell(f.cell[s], L, cx, cy, cz);
-            double bx = f.ox[s], by = f.oy[s], bz = f.oz[s];
-            double vx = f.vx[s], vy = f.vy[s], vz = f.vz[s];
-            double dump[3] = {vx, vy, vz};
-            int from = R.first_colloid;
-            if (q >= 0) {
-                from = d.cont_from[q];
-                for (int kk = 0; kk < 3; kk++) dump[kk] = d.cont_dump[3 * q + kk];
+            // colloids whose stencil holds the particle's binned cell, ascending
+            const int from = q >= 0 ? d.cont_from[q] : R.first_colloid;
+            s_cand_n = 0;
+            __syncthreads();
+            for (int base = from; base < gend; base += blockDim.x) {
+                int gg = base + threadIdx.x;
+                bool p = gg < gend && in_stencil(cx, cy, cz, d.ccell[3 * gg], d.ccell[3 * gg + 1], d.ccell[3 * gg + 2], L);
+                int tot;
+                int idx = block_compact(p, s_wt, tot);
+                if (p && s_cand_n + idx < TCL_CAP) s_cand[s_cand_n + idx] = gg;
+                __syncthreads();
+                if (threadIdx.x == 0) s_cand_n = min(s_cand_n + tot, TCL_CAP);
+                __syncthreads();
             }
-            for (int gg = from; gg < gend; gg++) {
-                if (!in_stencil(cx, cy, cz, d.ccell[3 * gg], d.ccell[3 * gg + 1], d.ccell[3 * gg + 2], L)) continue;
-                const double *pc = d.cpos + 3 * gg;
-                double ax = (double)cx + bx, ay = (double)cy + by, az = (double)cz + bz;
-                double rx = img1(pc[0] - ax, Ld), ry = img1(pc[1] - ay, Ld), rz = img1(pc[2] - az, Ld);
-                if ((rx * rx + ry * ry) + rz * rz > ph.lim_coll) continue;
-                ReflDraw dr;
-                if (!draw_reflection(d, ph, R, step, substep, gg, p, dr)) {
-                    atomicOr(d.status, BW_FLAG_DRAW_MISSING);
-                    continue;
+            if (threadIdx.x == 0) {
+                const int p = f.pid[s];
+                double bx = f.ox[s], by = f.oy[s], bz = f.oz[s];
+                double vx = f.vx[s], vy = f.vy[s], vz = f.vz[s];
+                double dump[3] = {vx, vy, vz};
+                if (q >= 0)
+                    for (int kk = 0; kk < 3; kk++) dump[kk] = d.cont_dump[3 * q + kk];
+                for (int m = 0; m < s_cand_n; m++) {
+                    const int gg = s_cand[m];
+                    const double *pc = d.cpos + 3 * gg;
+                    double ax = (double)cx + bx, ay = (double)cy + by, az = (double)cz + bz;
+                    double rx = img1(pc[0] - ax, Ld), ry = img1(pc[1] - ay, Ld), rz = img1(pc[2] - az, Ld);
+                    if ((rx * rx + ry * ry) + rz * rz > ph.lim_coll) continue;
+                    ReflDraw dr;
+                    if (!draw_reflection(d, ph, R, step, substep, gg, p, dr)) {
+                        atomicOr(d.status, BW_FLAG_DRAW_MISSING);
+                        continue;
+                    }
+                    double rs[3], c6[6];
+                    reflect_event(ph, L, cx, cy, cz, bx, by, bz, vx, vy, vz, pc, d.cvel + 3 * gg, d.comega + 3 * gg,
+                                  dr, rs, fp32);
+                    transfer(dump, vx, vy, vz, rs, c6);
+                    for (int kk = 0; kk < 3; kk++) {
+                        d.cdv[3 * gg + kk] += c6[kk];
+                        d.cdw[3 * gg + kk] += c6[3 + kk];
+                    }
+                    s_serial_events++;
                 }
-                double rs[3], c6[6];
-                reflect_event(ph, L, cx, cy, cz, bx, by, bz, vx, vy, vz, pc, d.cvel + 3 * gg, d.comega + 3 * gg, dr,
-                              rs, fp32);
-                transfer(dump, vx, vy, vz, rs, c6);
-                for (int kk = 0; kk < 3; kk++) {
-                    d.cdv[3 * gg + kk] += c6[kk];
-                    d.cdw[3 * gg + kk] += c6[3 + kk];
-                }
-                n_chain_events++;
+                f.ox[s] = (Real)bx; f.oy[s] = (Real)by; f.oz[s] = (Real)bz;
+                f.vx[s] = (Real)vx; f.vy[s] = (Real)vy; f.vz[s] = (Real)vz;
             }
-            f.ox[s] = (Real)bx; f.oy[s] = (Real)by; f.oz[s] = (Real)bz;
-            f.vx[s] = (Real)vx; f.vy[s] = (Real)vy; f.vz[s] = (Real)vz;
+            __syncthreads();
         }
-        if (ph.draws == BW_DRAWS_INJECTED) {
-            int n_inj = 0, n_ev = n_chain_events;
-            for (int gg = R.first_colloid; gg < gend; gg++) {
-                int base = substep * ph.n_colloid_total + gg;
-                n_inj += d.inj_off[base + 1] - d.inj_off[base];
-                n_ev += d.ev_cnt[gg] > EV_CAP ? 0 : d.ev_cnt[gg];
-            }
-            if (n_inj != n_ev) atomicOr(d.status, BW_FLAG_DRAW_MISMATCH);
+    }
+    if (ph.draws == BW_DRAWS_INJECTED) {
+        // every GPU event found its (colloid, particle) tuple; equal totals
+        // then mean equal sets
+        int my_inj = 0;
+        for (int i = threadIdx.x; i < R.nc; i += blockDim.x) {
+            int base = substep * ph.n_colloid_total + R.first_colloid + i;
+            my_inj += d.inj_off[base + 1] - d.inj_off[base];
         }
-        d.ticket[r] = 0;
+        atomicAdd(&s_inj_tot, my_inj - my_ev);
+        __syncthreads();
+        if (threadIdx.x == 0 && s_inj_tot != s_serial_events) atomicOr(d.status, BW_FLAG_DRAW_MISMATCH);
     }
     __syncthreads();
     __threadfence();
@@ -729,7 +819,7 @@ __global__ __launch_bounds__(STREAM_THREADS) void k_run_count(Phys ph, Dev d, Fl
     const int L = tg.L;
     const int n = min(d.count[T], ph.cap);
     for (int k = threadIdx.x; k < TCL_CAP; k += blockDim.x) s_shell[k] = 0;
-    build_tile_colloids(d, ph, R, tg.tx, tg.ty, tg.tz, false, true, true, tc);
+    load_tile_colloids(d, ph, T, true, tc);
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int slot = T * ph.cap + i;
         int ix, iy, iz;
@@ -984,6 +1074,9 @@ void launch_colloid(const Phys &ph, const Dev &d, unsigned flags, hipStream_t st
 }
 void launch_md_candidates(const Phys &ph, const Dev &d, const int2 *wgt, int nwg, hipStream_t st) {
     LAUNCH(k_md_candidates, nwg, 256, 0, st, ph, d, wgt);
+}
+void launch_tile_colloids(const Phys &ph, const Dev &d, hipStream_t st) {
+    LAUNCH(k_tile_colloids, ph.n_tiles, 256, 0, st, ph, d);
 }
 void launch_copy(const void *a, void *b, long long nbytes, hipStream_t st) {
     long long n = nbytes / 16;
