@@ -264,6 +264,43 @@ __device__ __forceinline__ bool near_interval(int c, int lo, int len, int L) {
 // by k_tile_colloids (ascending g); loaded here with the colloids' current data.
 constexpr int TILE_LIST_REACH = TILE_REACH + 4;
 
+// per tile: the colloids (ascending) whose cell at the start of this step, or
+// at run() time, lies within the tile box grown by TILE_LIST_REACH
+__device__ inline void build_tile_list(const Dev &d, const Rep &R, int T) {
+    __shared__ int s_wtot[16];
+    __shared__ int s_n;
+    const int lt = T - R.first_tile;
+    const int tx = lt % R.nt[0], ty = (lt / R.nt[0]) % R.nt[1], tz = lt / (R.nt[0] * R.nt[1]);
+    const int reach = TILE_LIST_REACH;
+    const int lox = tx * R.B[0] - reach, loy = ty * R.B[1] - reach, loz = tz * R.B[2] - reach;
+    const int lnx = R.B[0] + 2 * reach, lny = R.B[1] + 2 * reach, lnz = R.B[2] + 2 * reach;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (int base = 0; base < R.nc; base += blockDim.x) {
+        int i = base + threadIdx.x;
+        int g = R.first_colloid + i;
+        bool p = false;
+        if (i < R.nc) {
+            p = near_interval(d.ccell[3 * g], lox, lnx, R.L) && near_interval(d.ccell[3 * g + 1], loy, lny, R.L) &&
+                near_interval(d.ccell[3 * g + 2], loz, lnz, R.L);
+            p = p || (near_interval(d.ccell_run[3 * g], lox, lnx, R.L) &&
+                      near_interval(d.ccell_run[3 * g + 1], loy, lny, R.L) &&
+                      near_interval(d.ccell_run[3 * g + 2], loz, lnz, R.L));
+        }
+        int tot;
+        int idx = block_compact(p, s_wtot, tot);
+        if (p) {
+            if (s_n + idx < TCL_CAP) d.tcl_list[T * TCL_CAP + s_n + idx] = g;
+            else atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_n = min(s_n + tot, TCL_CAP);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d.tcl_cnt[T] = s_n;
+    __syncthreads();
+}
+
 __device__ inline void load_tile_colloids(const Dev &d, const Phys &ph, int T, bool kick_data, TileColl &tc) {
     const int n = min(d.tcl_cnt[T], TCL_CAP);
     for (int slot = threadIdx.x; slot < n; slot += blockDim.x) {
@@ -410,6 +447,7 @@ __device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R
     }
     __syncthreads();
     if (flags & PH_POS) {
+        bool far = false;
         double dt2 = ph.dt * ph.dt, ddt = 0.5 * dt2 / ph.mass_colloid;
         for (int i = threadIdx.x; i < nc; i += blockDim.x) {
             int g = g0 + i;
@@ -422,8 +460,15 @@ __device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R
                 int cc = coord_cell(p, R.L);
                 d.ccell[3 * g + k] = cc;
                 int dm = pdiff(cc, d.ccell_md[3 * g + k], R.L);
-                if (dm > 1 || dm < -1) atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
+                if (dm > 1 || dm < -1) far = true;
             }
+        }
+        // a colloid that left the one-cell margin of this step's tile lists:
+        // rebuild the replica's lists here (physically this needs speeds > 10)
+        if (__syncthreads_or(far)) {
+            for (int t = 0; t < R.n_tiles; t++) build_tile_list(d, R, R.first_tile + t);
+            for (int i = threadIdx.x; i < nc; i += blockDim.x)
+                for (int k = 0; k < 3; k++) d.ccell_md[3 * (g0 + i) + k] = d.ccell[3 * (g0 + i) + k];
         }
     }
     __syncthreads();

@@ -77,43 +77,9 @@ __global__ __launch_bounds__(256) void k_md_candidates(Phys ph, Dev d, const int
     }
 }
 
-// per tile: the colloids (ascending) whose cell at the start of this step, or
-// at run() time, lies within the tile box grown by TILE_LIST_REACH
 __global__ __launch_bounds__(256) void k_tile_colloids(Phys ph, Dev d) {
-    __shared__ int s_wtot[4];
-    __shared__ int s_n;
     const int T = blockIdx.x;
-    const int r = d.tile_rep[T];
-    const Rep R = d.rep[r];
-    const int lt = T - R.first_tile;
-    const int tx = lt % R.nt[0], ty = (lt / R.nt[0]) % R.nt[1], tz = lt / (R.nt[0] * R.nt[1]);
-    const int reach = TILE_LIST_REACH;
-    const int lox = tx * R.B[0] - reach, loy = ty * R.B[1] - reach, loz = tz * R.B[2] - reach;
-    const int lnx = R.B[0] + 2 * reach, lny = R.B[1] + 2 * reach, lnz = R.B[2] + 2 * reach;
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
-    for (int base = 0; base < R.nc; base += blockDim.x) {
-        int i = base + threadIdx.x;
-        int g = R.first_colloid + i;
-        bool p = false;
-        if (i < R.nc) {
-            p = near_interval(d.ccell[3 * g], lox, lnx, R.L) && near_interval(d.ccell[3 * g + 1], loy, lny, R.L) &&
-                near_interval(d.ccell[3 * g + 2], loz, lnz, R.L);
-            p = p || (near_interval(d.ccell_run[3 * g], lox, lnx, R.L) &&
-                      near_interval(d.ccell_run[3 * g + 1], loy, lny, R.L) &&
-                      near_interval(d.ccell_run[3 * g + 2], loz, lnz, R.L));
-        }
-        int tot;
-        int idx = block_compact(p, s_wtot, tot);
-        if (p) {
-            if (s_n + idx < TCL_CAP) d.tcl_list[T * TCL_CAP + s_n + idx] = g;
-            else atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) s_n = min(s_n + tot, TCL_CAP);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) d.tcl_cnt[T] = s_n;
+    build_tile_list(d, d.rep[d.tile_rep[T]], T);
 }
 
 // ------------------------------------------------------------ tile geometry
