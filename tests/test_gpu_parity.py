@@ -160,13 +160,19 @@ def test_overlapping_colloids_chain_particles():
     inject_step(s, shift, axes, refl)
     s.step(1)
     assert s.status() == 0  # every GPU event found its injected draw; counts equal
-    # Overlapping colloids push each other with a WCA force ~500 and spin up
-    # from ~150 reflections in one substep, which amplifies ulp-level libm
-    # differences (device log/cos vs glibc) by orders of magnitude: 1e-7 here.
-    compare_state(s, o, p["L"], 1e-7, 1e-7, 1e-7, label="overlap")
+    # This start is violent: ~300 reflections per substep (the event-bucket
+    # overflow and serial-replay paths) spin the colloids up to |w| ~ 2e4 and
+    # the fluid to |v| ~ 4e4 within the step, so FP64 rounding grows in
+    # absolute terms; compare relative to the state's magnitude (1e-9).
     pos, vel, _ = s.download_fluid()
-    assert np.abs(pdiff(pos, g["step1_pos"], p["L"])).max() < 1e-7
-    assert np.abs(vel - g["step1_vel"]).max() < 1e-7
+    opos, ovel = o.get_fluid()
+    vs = np.abs(ovel).max()
+    assert np.abs(vel - ovel).max() <= 1e-9 * vs
+    assert np.abs(pdiff(pos, opos, p["L"])).max() <= 1e-9 * p["L"]
+    assert np.abs(vel - g["step1_vel"]).max() <= 1e-9 * vs
+    c, oc = s.download_colloid(), o.get_colloid()
+    for k in ("vel", "omega"):
+        assert np.abs(c[k] - oc[k]).max() <= 1e-9 * np.abs(oc[k]).max(), k
 
 
 @pytest.mark.parametrize("precision", [B.BW_FP32, B.BW_FP64])
