@@ -11,7 +11,8 @@ uniform on the sphere.  Synthetic initial state from a fixed seed.
 One line of JSON: fluid particle-steps/s of the fused, graph-replayed step
 (inputs resident in HBM), the HBM roofline of the dominant kernel (streaming)
 from HIP events, and the reference cpp/ program (compiled from its sources
-into oracle/_ref) timed on one host core on one MPCD step of the same state.
+into oracle/_ref) timed on one host core on a bounded sample of the same state
+(rotation + run() + the first substeps, extrapolated to one MPCD step).
 """
 from __future__ import annotations
 
@@ -42,45 +43,66 @@ def workload_params(args):
                 mass_colloid=50.0, sig_colloid=4.2, eps=1.0, sigma=4.0, theta_deg=90.0, v0=0.0, seed=args.seed)
 
 
-def cpu_reference_baseline(sysm, prm, v0, seconds_note):
-    """Time one MPCD step of the reference cpp/ (oracle/_ref) on this host's
-    core, from the GPU run's initial state."""
-    import multiprocessing as mp
-    ctx = mp.get_context("spawn")
-    pos, vel, _ = sysm.download_fluid()
-    c = sysm.download_colloid()
-    q = ctx.Queue()
-    p = ctx.Process(target=_ref_worker, args=(q, prm, float(np.mean(v0)), pos, vel, c))
-    p.start()
-    out = q.get()
-    p.join()
-    return out
+def cpu_reference_baseline(args):
+    """Time the reference cpp/ (compiled from its sources into oracle/_ref) on
+    one host core, in a child process, on a bounded sample of this workload:
+    the same start-up state (recomputed on the host), one rotation_mpcd + run()
+    and the first substep(s); the step time is rotation + run + n_sub x the mean
+    substep."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--cpu-worker", "--L", str(args.L), "--ncol",
+           str(args.ncol), "--nsub", str(args.nsub), "--seed", str(args.seed)]
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
+    if p.returncode != 0:
+        return dict(error=f"worker exit {p.returncode}")
+    return json.loads(p.stdout.strip().splitlines()[-1])
 
 
-def _ref_worker(q, prm, v0_mean, pos, vel, c):
-    try:
-        import oracle as O
-        if not os.path.exists(O.REF_SO):
-            q.put(None)
-            return
-        r = O.Reference(**{**prm, "v0": v0_mean})
-        r.initialize()
-        r.set_fluid(pos, vel)
-        r.set_colloid(c["pos"], c["vel"], c["omega"], c["f"], c["old_f"], c["ra"])
-        t0 = time.process_time()
-        r.create_box()
-        r.neighbour_list_mpcd()
-        r.neighbour_list_md()
-        r.compute_force_md()
-        t_setup = time.process_time() - t0
-        t0 = time.perf_counter()
+def cpu_worker(args):
+    """Child process: no GPU use.  Prints progress on stderr, one JSON line."""
+    import oracle as O
+    import brownian_amd as B
+    if not os.path.exists(O.REF_SO):
+        print(json.dumps(dict(error="oracle/_ref not built")))
+        return
+    prm = workload_params(args)
+    t0 = time.time()
+    st = B.host_initialize(vel_law=B.BW_VEL_MAXWELL, orient_law=B.BW_ORIENT_SPHERE, **prm)
+    v0 = np.random.default_rng(args.seed).uniform(0.0, 0.05, args.ncol)
+    log(f"cpu worker: start-up state recomputed on the host ({time.time() - t0:.1f} s)")
+    r = O.Reference(**{**prm, "v0": float(np.mean(v0)), "theta_deg": 130.0})
+    r.initialize()
+    r.set_fluid(st["pos"], st["vel"])
+    z = np.zeros_like(st["c_pos"])
+    r.set_colloid(st["c_pos"], st["c_vel"], st["c_omega"], z, z, st["c_ra"])
+    del st
+    r.create_box()
+    r.neighbour_list_mpcd()
+    r.neighbour_list_md()
+    r.compute_force_md()
+    log(f"cpu worker: reference loaded ({time.time() - t0:.1f} s); timing rotation_mpcd + run")
+    c0 = time.process_time()
+    r.rotation_mpcd_bigstack()
+    r.run()
+    t_rot = time.process_time() - c0
+    log(f"cpu worker: rotation_mpcd + run {t_rot:.2f} s; timing substeps")
+    subs = []
+    # one substep as in cpp/active_brownian.cpp:27-37 (no updown_velocity: nn <= 10000)
+    while len(subs) < args.nsub and (not subs or sum(subs) < 10.0):
         c0 = time.process_time()
-        r.steps(1)
-        wall = time.perf_counter() - t0
-        cpu = time.process_time() - c0
-        q.put(dict(step_s=cpu, wall_s=wall, setup_s=t_setup))
-    except Exception as e:  # report, do not fail the GPU number
-        q.put(dict(error=repr(e)))
+        r.copy_old_force()
+        r.update_pos_md()
+        r.neighbour_list_md()
+        r.update_pos_mpcd()
+        r.neighbour_list_mpcd()
+        r.fluid_colloid_collision()
+        r.update_activity_direction()
+        r.compute_force_md()
+        r.update_velocity_colloid()
+        subs.append(time.process_time() - c0)
+        log(f"cpu worker: substep {len(subs)} {subs[-1]:.2f} s")
+    step_s = t_rot + args.nsub * float(np.mean(subs))
+    print(json.dumps(dict(step_s=step_s, rot_run_s=t_rot, substep_s=subs)))
 
 
 def main():
@@ -94,7 +116,11 @@ def main():
     ap.add_argument("--nsub", type=int, default=20)
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; a multi-GPU figure is not produced "
               "(the slab decomposition is described in DESIGN.md).", file=sys.stderr)
@@ -154,19 +180,20 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         s_ref = time.time()
-        log("reference cpp/ on one host core: one MPCD step (prints when done)")
-        r = cpu_reference_baseline(s, prm, v0, None)
-        log(f"reference done in {time.time() - s_ref:.1f} s: {r}")
+        log("reference cpp/ on one host core: bounded sample (progress follows)")
+        r = cpu_reference_baseline(args)
+        log(f"reference sample done in {time.time() - s_ref:.1f} s: {r}")
         if r and "step_s" in r:
             cpu = {"value": nf / r["step_s"], "unit": "fluid particle-steps/s", "cores": 1, "kind": "reference",
-                   "sample": f"1 MPCD step ({args.nsub} substeps) of this workload's initial state, reference "
-                             f"cpp/ compiled -O2 (oracle/_ref), host process CPU time {r['step_s']:.1f} s "
-                             f"(wall {r['wall_s']:.1f} s); angle fixed at 130 deg and mean v0 in the reference",
-                   "host_cores_total": os.cpu_count(), "gpu_over_cpu": value / (nf / r["step_s"])}
+                   "sample": f"reference cpp/ compiled -O2 (oracle/_ref), one host core, process CPU time, on this "
+                             f"workload's start-up state: rotation_mpcd + run {r['rot_run_s']:.2f} s and "
+                             f"{len(r['substep_s'])} substep(s) of mean {np.mean(r['substep_s']):.2f} s, step time = "
+                             f"rotation + run + {args.nsub} x substep = {r['step_s']:.1f} s; the reference fixes the "
+                             f"SRD angle at 130 deg and one v0 (the mean)",
+                   "gpu_over_cpu": value / (nf / r["step_s"])}
         elif r:
             cpu = {"value": None, "unit": "fluid particle-steps/s", "cores": 1, "kind": "reference",
                    "sample": "failed: " + str(r.get("error"))}
-        cpu_wall = time.time() - s_ref
     out = {
         "metric": METRIC,
         "value": value,

@@ -16,13 +16,14 @@
 namespace bw {
 // kernel launchers (kernels.hip)
 size_t srd_lds_bytes(int ncell, int cap, bool fp64);
+size_t srd_static_lds();
 template <class Real> void launch_depart(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
 template <class Real>
 void launch_srd(const Phys &, const Dev &, const Fluid<Real> &, const Fluid<Real> &, bool, bool, size_t,
                 hipStream_t);
 template <class Real> void launch_stream(const Phys &, const Dev &, const Fluid<Real> &, bool, bool, bool, hipStream_t);
 template <class Real>
-void launch_collide(const Phys &, const Dev &, const Fluid<Real> &, int, unsigned, const int2 *, int, hipStream_t);
+void launch_collide(const Phys &, const Dev &, const Fluid<Real> &, int, const int2 *, int, hipStream_t);
 template <class Real> void launch_run_count(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
 template <class Real>
 void launch_maps(const Phys &, const Dev &, const Fluid<Real> &, long long, int *, const int *, hipStream_t);
@@ -31,6 +32,7 @@ template <class Real>
 void launch_gather(const Phys &, const Dev &, const Fluid<Real> &, double *, double *, int *, hipStream_t);
 void launch_step_begin(const Phys &, const Dev &, hipStream_t);
 void launch_colloid(const Phys &, const Dev &, unsigned, hipStream_t);
+void launch_colloid_update(const Phys &, const Dev &, unsigned, const int2 *, int, hipStream_t);
 void launch_md_candidates(const Phys &, const Dev &, const int2 *, int, hipStream_t);
 void launch_tile_colloids(const Phys &, const Dev &, hipStream_t);
 void launch_copy(const void *, void *, long long, hipStream_t);
@@ -70,13 +72,16 @@ struct bw_ctx {
     Phys ph{};
     Dev dv{};
     hipStream_t st = nullptr;
-    int cur = 0;
+    int cur = 0;                   // fluid buffer holding the state
+    int cb = 0;                    // colloid position/cell buffer holding the state
     int substep = 0;
     bool pending_prologue = true;  // next fused step needs the kick/POS prologue
     long long total_slots = 0, cells_total = 0, nf_total = 0;
     int nc_total = 0;
     FluidAlloc fl[2];
     int *counts[2] = {nullptr, nullptr};
+    double *cposb[2] = {nullptr, nullptr};
+    int *ccellb[2] = {nullptr, nullptr};
     std::vector<void *> allocs;
     int2 *wgt = nullptr;
     int nwg = 0;
@@ -87,7 +92,7 @@ struct bw_ctx {
     double *stage_pos = nullptr, *stage_vel = nullptr;
     int *stage_cell = nullptr;
     bool inj_dirty = false;
-    hipGraphExec_t graphs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [cur][final]
+    hipGraphExec_t graphs[2][2][2] = {};  // [cur][cb][final]
 };
 
 namespace {
@@ -115,6 +120,12 @@ int dalloc(bw_ctx *c, T **p, size_t n) {
 
 bool divides(int a, int b) { return b > 0 && a % b == 0; }
 
+// cells of a tile's region: its box grown by TILE_REACH per side, at most L
+long long region_cells(int L, int bx, int by, int bz) {
+    auto ext = [&](int b) { return (long long)std::min(b + 2 * TILE_REACH, L); };
+    return ext(bx) * ext(by) * ext(bz);
+}
+
 // tile shape: blocks of at least 4 cells per axis dividing L, about 256 cells
 void choose_tile(int L, int density, bool fp64, int out[3]) {
     std::vector<int> divs;
@@ -130,6 +141,7 @@ void choose_tile(int L, int density, bool fp64, int out[3]) {
                 if (by > bx || bz > by) continue;
                 int n = bx * by * bz;
                 if (n > 512 || (long long)n * density * 5 / 4 + 64 > MAX_CAP) continue;
+                if (region_cells(L, bx, by, bz) > MASK_MAX) continue;
                 double score = fabs(log((double)n / target)) + 0.01 * (bx - bz);
                 if (score < best) {
                     best = score;
@@ -153,12 +165,18 @@ Fluid<Real> carve(void *base, long long n) {
     return f;
 }
 
-Dev dev_for(bw_ctx *c, int buf) {
+// device view with fluid buffer `buf` and colloid position buffer `cb` current
+Dev dev_at(bw_ctx *c, int buf, int cb) {
     Dev d = c->dv;
     d.count = c->counts[buf];
     d.count_next = c->counts[1 - buf];
+    d.cpos = c->cposb[cb];
+    d.cpos_nxt = c->cposb[1 - cb];
+    d.ccell = c->ccellb[cb];
+    d.ccell_nxt = c->ccellb[1 - cb];
     return d;
 }
+Dev dev_for(bw_ctx *c, int buf) { return dev_at(c, buf, c->cb); }
 
 int check_status_code(unsigned s) {
     if (s & (BW_FLAG_DRAW_MISSING | BW_FLAG_DRAW_MISMATCH)) return BW_ERR_DRAWS;
@@ -222,10 +240,11 @@ int sync_injected(bw_ctx *c) {
 void destroy_graphs(bw_ctx *c) {
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++)
-            if (c->graphs[a][b]) {
-                hipGraphExecDestroy(c->graphs[a][b]);
-                c->graphs[a][b] = nullptr;
-            }
+            for (int k = 0; k < 2; k++)
+                if (c->graphs[a][b][k]) {
+                    hipGraphExecDestroy(c->graphs[a][b][k]);
+                    c->graphs[a][b][k] = nullptr;
+                }
 }
 
 // ---------------------------------------------------------------- the step
@@ -249,58 +268,71 @@ struct Prof {
     }
 };
 
-// One MPCD step (cpp/active_brownian.cpp:25-38) on buffer `cur`, fused.
+// One MPCD step (cpp/active_brownian.cpp:25-38) on fluid buffer `cur` and
+// colloid buffer `cb`, fused; returns the colloid buffer current afterwards.
 // Entry state: the run() colloid kick, old_force copy and first update_pos_md
-// of this step done (by the previous step's last tail or the prologue).
-// `final`: the last tail stops after update_velocity_colloid.
+// of this step done (by the previous step's last colloid update or the
+// prologue).  `final`: the last colloid update stops after
+// update_velocity_colloid.
 template <class Real>
-void enqueue_step(bw_ctx *c, int cur, bool final, Prof *prof) {
+int enqueue_step(bw_ctx *c, int cur, int cb, bool final, Prof *prof) {
     hipStream_t st = c->st;
     const Phys &ph = c->ph;
-    Dev d0 = dev_for(c, cur), d1 = dev_for(c, 1 - cur);
+    const bool col = c->nc_total > 0;
     Fluid<Real> &src = fluid_of<Real>(c, cur), &dst = fluid_of<Real>(c, 1 - cur);
+    Dev d0 = dev_at(c, cur, cb);
     if (prof) prof->mark(st, BW_KERNEL_OTHER);
     launch_step_begin(ph, d0, st);
-    if (c->nc_total > 0) {
+    if (col) {
         launch_md_candidates(ph, d0, c->wgt, c->nwg, st);
         launch_tile_colloids(ph, d0, st);
     }
     if (prof) prof->mark(st, BW_KERNEL_COLLOID);
     launch_depart<Real>(ph, d0, src, st);
     if (prof) prof->mark(st, BW_KERNEL_DEPART);
-    launch_srd<Real>(ph, d0, src, dst, true, false, c->srd_lds, st);
+    launch_srd<Real>(ph, d0, src, dst, col, false, c->srd_lds, st);
     if (prof) prof->mark(st, BW_KERNEL_SRD);
     for (int l = 0; l < ph.n_sub; l++) {
-        launch_stream<Real>(ph, d1, dst, l == 0, true, c->nc_total > 0, st);
+        Dev d1 = dev_at(c, 1 - cur, cb);
+        launch_stream<Real>(ph, d1, dst, col && l == 0, true, col, st);
         if (prof) prof->mark(st, BW_KERNEL_STREAM);
-        if (c->nc_total > 0) {
-            unsigned tail = PH_APPLY | PH_ACTIVITY | PH_FORCE | PH_VELOCITY;
-            if (l < ph.n_sub - 1) tail |= PH_OLDF | PH_POS | PH_MDLIST;
-            else if (!final) tail |= PH_KICK | PH_OLDF | PH_POS | PH_MDLIST;
-            launch_collide<Real>(ph, d1, dst, l, tail, c->wgt, c->nwg, st);
-            if (prof) prof->mark(st, BW_KERNEL_COLLIDE);
-        }
+        if (!col) continue;
+        launch_collide<Real>(ph, d1, dst, l, c->wgt, c->nwg, st);
+        if (prof) prof->mark(st, BW_KERNEL_COLLIDE);
+        unsigned flags = PH_APPLY | PH_ACTIVITY | PH_FORCE | PH_VELOCITY;
+        if (l < ph.n_sub - 1) flags |= PH_OLDF | PH_POS;
+        else if (!final) flags |= PH_KICK | PH_OLDF | PH_POS;
+        launch_colloid_update(ph, d1, flags, c->wgt, c->nwg, st);
+        if (prof) prof->mark(st, BW_KERNEL_COLLOID);
+        if (flags & PH_POS) cb ^= 1;
     }
+    return cb;
 }
 
+// run() colloid kick, old_force copy and update_pos_md ahead of a first step
 template <class Real>
 void enqueue_prologue(bw_ctx *c) {
-    Dev d = dev_for(c, c->cur);
-    if (c->nc_total > 0) {
-        launch_md_candidates(c->ph, d, c->wgt, c->nwg, c->st);
-        launch_colloid(c->ph, d, PH_KICK | PH_OLDF | PH_POS | PH_MDLIST, c->st);
-    }
+    if (c->nc_total == 0) return;
+    launch_colloid(c->ph, dev_for(c, c->cur), PH_KICK | PH_OLDF | PH_POS, c->st);
+    c->cb ^= 1;
 }
 
 template <class Real>
-int build_graph(bw_ctx *c, int cur, int final) {
+int build_graph(bw_ctx *c, int cur, int cb, int final) {
     hipGraph_t g;
     HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    enqueue_step<Real>(c, cur, final != 0, nullptr);
+    enqueue_step<Real>(c, cur, cb, final != 0, nullptr);
     HIPCHK(c, hipStreamEndCapture(c->st, &g));
-    HIPCHK(c, hipGraphInstantiate(&c->graphs[cur][final], g, nullptr, nullptr, 0));
+    HIPCHK(c, hipGraphInstantiate(&c->graphs[cur][cb][final], g, nullptr, nullptr, 0));
     hipGraphDestroy(g);
     return BW_OK;
+}
+
+// colloid buffer after one step from buffer cb: one flip per update_pos_md
+int cb_after(const bw_ctx *c, int cb, bool final) {
+    if (c->nc_total == 0) return cb;
+    int npos = c->ph.n_sub - 1 + (final ? 0 : 1);
+    return cb ^ (npos & 1);
 }
 
 template <class Real>
@@ -312,9 +344,10 @@ int do_steps(bw_ctx *c, int n) {
     c->pending_prologue = false;
     for (int s = 0; s < n; s++) {
         int fin = (s == n - 1) ? 1 : 0;
-        if (!c->graphs[c->cur][fin])
-            if ((rc = build_graph<Real>(c, c->cur, fin))) return rc;
-        HIPCHK(c, hipGraphLaunch(c->graphs[c->cur][fin], c->st));
+        if (!c->graphs[c->cur][c->cb][fin])
+            if ((rc = build_graph<Real>(c, c->cur, c->cb, fin))) return rc;
+        HIPCHK(c, hipGraphLaunch(c->graphs[c->cur][c->cb][fin], c->st));
+        c->cb = cb_after(c, c->cb, fin != 0);
         c->cur ^= 1;
     }
     c->pending_prologue = true;
@@ -331,11 +364,12 @@ int do_profile(bw_ctx *c, int n, double *ms, int *launches) {
     for (int s = 0; s < n; s++) {
         if (c->pending_prologue) enqueue_prologue<Real>(c);
         c->pending_prologue = false;
-        enqueue_step<Real>(c, c->cur, s == n - 1, &prof);
+        c->cb = enqueue_step<Real>(c, c->cur, c->cb, s == n - 1, &prof);
         c->cur ^= 1;
     }
     prof.mark(c->st, BW_KERNEL_OTHER);
     c->pending_prologue = true;
+    c->substep = 0;
     HIPCHK(c, hipStreamSynchronize(c->st));
     for (size_t i = 0; i + 1 < prof.ev.size(); i++) {
         float t = 0;
@@ -405,7 +439,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         const bw_params &p = ps[r];
         ReplicaHost &R = c->reps[r];
         R.p = p;
-        if (p.L < 7 || p.density < 1 || p.n_colloid < 0 || p.n_sub < 1) {
+        if (p.L < 7 || p.L >= (1 << CELL_BITS) || p.density < 1 || p.n_colloid < 0 || p.n_sub < 1) {
             delete c;
             return BW_ERR_ARG;
         }
@@ -423,7 +457,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
             R.B[k] = B[k];
             R.nt[k] = p.L / B[k];
         }
-        if (B[0] * B[1] * B[2] > 512) {
+        if (B[0] * B[1] * B[2] > 512 || region_cells(p.L, B[0], B[1], B[2]) > MASK_MAX) {
             delete c;
             return BW_ERR_ARG;
         }
@@ -460,7 +494,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
     int maxcell = 0;
     for (auto &R : c->reps) maxcell = std::max(maxcell, R.B[0] * R.B[1] * R.B[2]);
     c->srd_lds = srd_lds_bytes(maxcell, cap, c->fp64);
-    if (c->srd_lds > 160 * 1024 - 24 * 1024) {
+    if (c->srd_lds > 160 * 1024 - srd_static_lds()) {
         delete c;
         return BW_ERR_ARG;
     }
@@ -512,11 +546,16 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
     d.tile_rep = dtrep;
     size_t NC = std::max(nc, 1);
     if ((rc = dalloc(c, &d.dep_off, (size_t)tiles * 28)) || (rc = dalloc(c, &d.dep_idx, (size_t)c->total_slots)) ||
-        (rc = dalloc(c, &d.cpos, 3 * NC)) || (rc = dalloc(c, &d.cvel, 3 * NC)) || (rc = dalloc(c, &d.comega, 3 * NC)) ||
+        (rc = dalloc(c, &c->cposb[0], 3 * NC)) || (rc = dalloc(c, &c->cposb[1], 3 * NC)) ||
+        (rc = dalloc(c, &c->ccellb[0], 3 * NC)) || (rc = dalloc(c, &c->ccellb[1], 3 * NC)) ||
+        (rc = dalloc(c, &d.md_stale, n_rep)) || (rc = dalloc(c, &d.tcl_stale, n_rep)) ||
+        (rc = dalloc(c, &d.ovf_cnt, n_rep)) || (rc = dalloc(c, &d.ovf_list, (size_t)n_rep * OVF_CAP)) ||
+        (rc = dalloc(c, &d.ev_total, n_rep)) ||
+        (rc = dalloc(c, &d.cvel, 3 * NC)) || (rc = dalloc(c, &d.comega, 3 * NC)) ||
         (rc = dalloc(c, &d.cf, 3 * NC)) || (rc = dalloc(c, &d.coldf, 3 * NC)) || (rc = dalloc(c, &d.cra, 3 * NC)) ||
         (rc = dalloc(c, &d.cdisp, 3 * NC)) || (rc = dalloc(c, &d.cpos_run, 3 * NC)) ||
         (rc = dalloc(c, &d.cdv, 3 * NC)) || (rc = dalloc(c, &d.cdw, 3 * NC)) || (rc = dalloc(c, &d.cpos_md, 3 * NC)) ||
-        (rc = dalloc(c, &d.cv0, NC)) || (rc = dalloc(c, &d.crep, NC)) || (rc = dalloc(c, &d.ccell, 3 * NC)) ||
+        (rc = dalloc(c, &d.cv0, NC)) || (rc = dalloc(c, &d.crep, NC)) ||
         (rc = dalloc(c, &d.ccell_run, 3 * NC)) || (rc = dalloc(c, &d.ccell_md, 3 * NC)) ||
         (rc = dalloc(c, &d.tcl_cnt, tiles)) || (rc = dalloc(c, &d.tcl_list, (size_t)tiles * TCL_CAP)) ||
         (rc = dalloc(c, &d.ccnt, NC)) || (rc = dalloc(c, &d.md_cnt, NC)) ||
@@ -606,7 +645,7 @@ int upload_fluid_impl(bw_ctx *c, int r, const double *pos, const double *vel) {
             ic[k] = (int)q;
             o[3 * i + k] = ro;
         }
-        cell[i] = ic[0] + L * (ic[1] + L * ic[2]);
+        cell[i] = ic[0] | (ic[1] << CELL_BITS) | (ic[2] << (2 * CELL_BITS));
         tile[i] = (ic[0] / R.B[0]) + R.nt[0] * ((ic[1] / R.B[1]) + R.nt[1] * (ic[2] / R.B[2]));
     }
     std::vector<int> cnt(R.n_tiles, 0);
@@ -662,7 +701,7 @@ int set_colloid_cells(bw_ctx *c, int r) {
     ReplicaHost &R = c->reps[r];
     if (R.nc == 0) return BW_OK;
     std::vector<double> pos(3 * R.nc);
-    HIPCHK(c, hipMemcpy(pos.data(), c->dv.cpos + 3 * R.first_colloid, sizeof(double) * pos.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(pos.data(), c->cposb[c->cb] + 3 * R.first_colloid, sizeof(double) * pos.size(), hipMemcpyDeviceToHost));
     std::vector<int> cc(3 * R.nc);
     for (int i = 0; i < 3 * R.nc; i++) {
         int q = (int)pos[i];
@@ -671,7 +710,9 @@ int set_colloid_cells(bw_ctx *c, int r) {
         cc[i] = q;
     }
     size_t o = 3 * (size_t)R.first_colloid;
-    HIPCHK(c, hipMemcpy(c->dv.ccell + o, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice));
+    for (int b = 0; b < 2; b++)
+        HIPCHK(c, hipMemcpy(c->ccellb[b] + o, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->cposb[1 - c->cb] + o, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dv.ccell_run + o, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dv.ccell_md + o, cc.data(), sizeof(int) * cc.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->dv.cpos_run + o, pos.data(), sizeof(double) * pos.size(), hipMemcpyHostToDevice));
@@ -758,12 +799,15 @@ BW_API int initialize(bw_ctx *c) {
     HIPCHK(c, hipMemset(c->dv.step, 0, sizeof(long long)));
     HIPCHK(c, hipMemset(c->dv.status, 0, sizeof(unsigned)));
     size_t NC = std::max(c->nc_total, 1);
-    double *arrs[] = {c->dv.cpos, c->dv.cvel, c->dv.comega, c->dv.cf, c->dv.coldf, c->dv.cra, c->dv.cdisp, c->dv.cdv, c->dv.cdw};
+    double *arrs[] = {c->cposb[0], c->cposb[1], c->dv.cvel, c->dv.comega, c->dv.cf, c->dv.coldf,
+                      c->dv.cra, c->dv.cdisp, c->dv.cdv, c->dv.cdw};
     for (double *a : arrs) HIPCHK(c, hipMemset(a, 0, sizeof(double) * 3 * NC));
+    for (int b = 0; b < 2; b++) HIPCHK(c, hipMemset(c->ccellb[b], 0, sizeof(int) * 3 * NC));
     HIPCHK(c, hipMemset(c->dv.ev_cnt, 0, sizeof(int) * NC));
-    HIPCHK(c, hipMemset(c->dv.chain_cnt, 0, sizeof(int) * c->n_rep));
-    HIPCHK(c, hipMemset(c->dv.cont_cnt, 0, sizeof(int) * c->n_rep));
+    int *per_rep[] = {c->dv.chain_cnt, c->dv.cont_cnt, c->dv.ovf_cnt, c->dv.ev_total, c->dv.md_stale, c->dv.tcl_stale};
+    for (int *a : per_rep) HIPCHK(c, hipMemset(a, 0, sizeof(int) * c->n_rep));
     HIPCHK(c, hipMemset(c->dv.ticket, 0, sizeof(unsigned) * c->n_rep));
+    c->cb = 0;
     c->pending_prologue = true;
     c->substep = 0;
     return BW_OK;
@@ -788,7 +832,7 @@ BW_API int initialize_fluid(bw_ctx *c) {
         ReplicaHost &R = c->reps[r];
         std::vector<double> cpos(3 * std::max(R.nc, 1));
         if (R.nc)
-            HIPCHK(c, hipMemcpy(cpos.data(), c->dv.cpos + 3 * R.first_colloid, sizeof(double) * 3 * R.nc,
+            HIPCHK(c, hipMemcpy(cpos.data(), c->cposb[c->cb] + 3 * R.first_colloid, sizeof(double) * 3 * R.nc,
                                 hipMemcpyDeviceToHost));
         std::vector<double> pos(3 * (size_t)R.nf), vel(3 * (size_t)R.nf);
         init_fluid(init_params(R), R.rng, cpos.data(), pos.data(), vel.data());
@@ -810,7 +854,7 @@ BW_API int tumble(bw_ctx *c) {
         ReplicaHost &R = c->reps[r];
         if (!R.nc) continue;
         std::vector<double> cpos(3 * R.nc), ra(3 * R.nc);
-        HIPCHK(c, hipMemcpy(cpos.data(), c->dv.cpos + 3 * R.first_colloid, sizeof(double) * 3 * R.nc,
+        HIPCHK(c, hipMemcpy(cpos.data(), c->cposb[c->cb] + 3 * R.first_colloid, sizeof(double) * 3 * R.nc,
                             hipMemcpyDeviceToHost));
         init_orientation(init_params(R), R.rng, cpos.data(), ra.data());
         HIPCHK(c, hipMemcpy(c->dv.cra + 3 * R.first_colloid, ra.data(), sizeof(double) * 3 * R.nc,
@@ -848,8 +892,15 @@ BW_API int neighbour_list_md(bw_ctx *c) {
 COLLOID_ENTRY(compute_force_md, PH_FORCE)
 COLLOID_ENTRY(update_activity_direction, PH_ACTIVITY)
 COLLOID_ENTRY(update_velocity_colloid, PH_VELOCITY)
-COLLOID_ENTRY(update_pos_md, PH_POS)
 COLLOID_ENTRY(bw_copy_old_force, PH_OLDF)
+
+BW_API int update_pos_md(bw_ctx *c) {
+    if (!c) return BW_ERR_ARG;
+    if (c->nc_total == 0) return BW_OK;
+    launch_colloid(c->ph, dev_for(c, c->cur), PH_POS, c->st);
+    c->cb ^= 1;  // the new positions are in the other buffer
+    return BW_OK;
+}
 
 BW_API int rotation_mpcd(bw_ctx *c) {
     if (!c) return BW_ERR_ARG;
@@ -904,11 +955,12 @@ BW_API int fluid_colloid_collision(bw_ctx *c) {
     launch_tile_colloids(c->ph, d, c->st);
     if (c->fp64) {
         launch_stream<double>(c->ph, d, c->fl[c->cur].f64, false, false, true, c->st);
-        launch_collide<double>(c->ph, d, c->fl[c->cur].f64, l, PH_APPLY, c->wgt, c->nwg, c->st);
+        launch_collide<double>(c->ph, d, c->fl[c->cur].f64, l, c->wgt, c->nwg, c->st);
     } else {
         launch_stream<float>(c->ph, d, c->fl[c->cur].f32, false, false, true, c->st);
-        launch_collide<float>(c->ph, d, c->fl[c->cur].f32, l, PH_APPLY, c->wgt, c->nwg, c->st);
+        launch_collide<float>(c->ph, d, c->fl[c->cur].f32, l, c->wgt, c->nwg, c->st);
     }
+    launch_colloid(c->ph, d, PH_APPLY, c->st);
     c->substep++;
     return BW_OK;
 }
@@ -979,7 +1031,7 @@ BW_API int bw_upload_colloid(bw_ctx *c, int r, const double *pos, const double *
     HIPCHK(c, hipStreamSynchronize(c->st));
     size_t o = 3 * (size_t)R->first_colloid, n = sizeof(double) * 3 * R->nc;
     if (!R->nc) return BW_OK;
-    if (pos) HIPCHK(c, hipMemcpy(c->dv.cpos + o, pos, n, hipMemcpyHostToDevice));
+    if (pos) HIPCHK(c, hipMemcpy(c->cposb[c->cb] + o, pos, n, hipMemcpyHostToDevice));
     if (vel) HIPCHK(c, hipMemcpy(c->dv.cvel + o, vel, n, hipMemcpyHostToDevice));
     if (omega) HIPCHK(c, hipMemcpy(c->dv.comega + o, omega, n, hipMemcpyHostToDevice));
     if (f) HIPCHK(c, hipMemcpy(c->dv.cf + o, f, n, hipMemcpyHostToDevice));
@@ -999,7 +1051,7 @@ BW_API int bw_download_colloid(bw_ctx *c, int r, double *pos, double *vel, doubl
     HIPCHK(c, hipStreamSynchronize(c->st));
     size_t o = 3 * (size_t)R->first_colloid, n = sizeof(double) * 3 * R->nc;
     if (!R->nc) return BW_OK;
-    if (pos) HIPCHK(c, hipMemcpy(pos, c->dv.cpos + o, n, hipMemcpyDeviceToHost));
+    if (pos) HIPCHK(c, hipMemcpy(pos, c->cposb[c->cb] + o, n, hipMemcpyDeviceToHost));
     if (vel) HIPCHK(c, hipMemcpy(vel, c->dv.cvel + o, n, hipMemcpyDeviceToHost));
     if (omega) HIPCHK(c, hipMemcpy(omega, c->dv.comega + o, n, hipMemcpyDeviceToHost));
     if (f) HIPCHK(c, hipMemcpy(f, c->dv.cf + o, n, hipMemcpyDeviceToHost));

@@ -4,9 +4,10 @@
 // particles live in tiles -- blocks of Bx x By x Bz cells of the SRD grid as it
 // was shifted at the last rotation step -- each tile owning a fixed range of
 // `cap` slots in structure-of-arrays buffers.  A particle's position is an
-// integer cell (ix, iy, iz packed as ix + L*(iy + L*iz)) plus an offset in that
-// cell stored in Real (float or double); velocities are Real.  Colloids are
-// few and kept in FP64.
+// integer cell, packed as ix | iy << 10 | iz << 20, plus an offset in that cell
+// stored in Real (float or double); velocities are Real.  Colloids are few and
+// kept in FP64; their positions and cells are double-buffered so that one
+// kernel can read every colloid's position (forces) while writing the next.
 #pragma once
 #include <stdint.h>
 
@@ -16,7 +17,10 @@ constexpr int EV_CAP = 64;          // reflection events per colloid per substep
 constexpr int CHAIN_CAP = 1024;     // serially replayed particles per replica per substep
 constexpr int CONT_CAP = 512;       // continuations per replica per substep
 constexpr int SERIAL_CAP = 2048;    // serial list incl. entries of overflowed buckets
-constexpr int TCL_CAP = 128;        // colloids listed for one tile
+constexpr int TCL_CAP = 64;         // colloids listed for one tile
+constexpr int MASK_BITS = 32;       // listed colloids covered by the LDS stencil masks
+constexpr int MASK_MAX = 4096;      // cells of a tile's region (box grown by TILE_REACH)
+constexpr int CELL_BITS = 10;       // packed cell coordinates: L <= 1023
 constexpr int MD_CAP = 128;         // full MD neighbour list per colloid
 constexpr int MDC_CAP = 256;        // MD candidates (cutoff + skin) per colloid
 constexpr int MAX_CAP = 4096;       // slots per tile upper bound
@@ -25,6 +29,7 @@ constexpr int SRD_PT = MAX_CAP / SRD_THREADS;  // particles per thread in the SR
 constexpr int STREAM_THREADS = 256;
 constexpr int DEPART_THREADS = 256;
 constexpr int COLLIDE_THREADS = 256; // 4 colloids (one wave each) per workgroup
+constexpr int OVF_CAP = 64;         // colloids with an overflowed event bucket, per replica
 constexpr int NDIR = 27;
 constexpr int TILE_REACH = 3;       // cells a particle may stand outside its tile box
 constexpr double MD_SKIN = 2.0;     // candidate-list skin for the colloid pair list
@@ -85,15 +90,19 @@ struct Dev {
     int *dep_off;        // [n_tiles * 28]
     int *dep_idx;        // [n_tiles * cap]
     // colloids (global index g), FP64, 3 per colloid
-    double *cpos, *cvel, *comega, *cf, *coldf, *cra, *cdisp, *cpos_run, *cdv, *cdw, *cpos_md;
+    double *cpos, *cpos_nxt;  // current / next (written by update_pos_md)
+    double *cvel, *comega, *cf, *coldf, *cra, *cdisp, *cpos_run, *cdv, *cdw, *cpos_md;
     double *cv0;
-    int *crep, *ccell, *ccell_run, *ccell_md, *ccnt;
+    int *ccell, *ccell_nxt;
+    int *crep, *ccell_run, *ccell_md, *ccnt;
+    int *md_stale, *tcl_stale;  // per replica: pair candidates / tile lists outdated
     int *tcl_cnt, *tcl_list;  // per tile: colloids that can touch it this step
     int *md_cnt, *md_list, *mdc_cnt, *mdc_list;
     double *cpot;        // [n_rep] potential
     int *ev_cnt, *ev_slot, *chain_cnt, *chain_slot;
     int *cont_cnt, *cont_slot, *cont_from;  // continuations of the colloid loop
     double *cont_dump;
+    int *ovf_cnt, *ovf_list, *ev_total;     // per replica: overflowed buckets, events seen
     unsigned *ticket;    // [n_rep]
     unsigned *status;
     long long *step;     // MPCD step counter (device generator key)

@@ -107,19 +107,31 @@ __device__ __forceinline__ double img1(double a, double L) { return a - L * roun
 // mod(a,b) = a - b*round((a - b/2)/b) (cpp/parameters.hpp:75)
 __device__ __forceinline__ double mod1(double a, double L) { return a - L * round((a - L / 2) / L); }
 
-__device__ __forceinline__ void unpack_cell(int c, int L, int &ix, int &iy, int &iz) {
-    int LL = L * L;
-    iz = c / LL;
-    int rem = c - iz * LL;
-    iy = rem / L;
-    ix = rem - iy * L;
+constexpr int CELL_MASK = (1 << CELL_BITS) - 1;
+__device__ __forceinline__ void unpack_cell(int c, int &ix, int &iy, int &iz) {
+    ix = c & CELL_MASK;
+    iy = (c >> CELL_BITS) & CELL_MASK;
+    iz = c >> (2 * CELL_BITS);
 }
-__device__ __forceinline__ int pack_cell(int ix, int iy, int iz, int L) { return ix + L * (iy + L * iz); }
+__device__ __forceinline__ int pack_cell(int ix, int iy, int iz) {
+    return ix | (iy << CELL_BITS) | (iz << (2 * CELL_BITS));
+}
+// reference cell numbering (cpp/parameters.hpp:59, 0-based)
+__device__ __forceinline__ int linear_cell(int ix, int iy, int iz, int L) { return ix + L * (iy + L * iz); }
+__device__ __forceinline__ int linear_of_packed(int c, int L) {
+    int ix, iy, iz;
+    unpack_cell(c, ix, iy, iz);
+    return linear_cell(ix, iy, iz, L);
+}
 __device__ __forceinline__ int pdiff(int a, int b, int L) {
     int d = a - b;
     if (d > L / 2) d -= L;
     if (d < -(L / 2)) d += L;
     return d;
+}
+__device__ __forceinline__ int wrapi(int a, int n) {
+    a %= n;
+    return a < 0 ? a + n : a;
 }
 // cell (ix,iy,iz) belongs to the 7^3-1 stencil of cell (cx,cy,cz)
 // (cpp/neighbour_list.cpp:12-25); requires L >= 7
@@ -144,8 +156,10 @@ __device__ __forceinline__ int coord_cell(double x, int L) {
 template <class Real>
 __device__ __forceinline__ int shifted_coord(int ix, Real o, double s, int L) {
     int k = ix + (int)floor((double)o + s);
-    if (k >= L) k -= L;
-    if (k < 0) k += L;
+    if (k >= L || k < 0) {  // a reflection may leave the offset far outside [0,1)
+        k %= L;
+        if (k < 0) k += L;
+    }
     return k;
 }
 
@@ -160,8 +174,10 @@ __device__ __forceinline__ void normalise(int &i, Real &o, int L) {
             o = Real(0);
             i += 1;
         }
-        i %= L;
-        if (i < 0) i += L;
+        if (i >= L || i < 0) {  // a particle may cross several box lengths in a substep
+            i %= L;
+            if (i < 0) i += L;
+        }
     }
 }
 
@@ -236,18 +252,6 @@ __device__ inline int block_exclusive_scan(int *s, int n, int *s_wtot) {
 }
 
 // ------------------------------------------------ colloids listed per tile
-struct TileColl {
-    int n;
-    int g[TCL_CAP];
-    double px[TCL_CAP], py[TCL_CAP], pz[TCL_CAP];
-    int cx[TCL_CAP], cy[TCL_CAP], cz[TCL_CAP];
-    double rx[TCL_CAP], ry[TCL_CAP], rz[TCL_CAP];    // position at run() time
-    int qx[TCL_CAP], qy[TCL_CAP], qz[TCL_CAP];       // its cell
-    double vx[TCL_CAP], vy[TCL_CAP], vz[TCL_CAP];    // velocity after the kick
-    double kx[TCL_CAP], ky[TCL_CAP], kz[TCL_CAP];    // per-particle kick
-    int wtot[16];
-};
-
 __device__ __forceinline__ bool near_interval(int c, int lo, int len, int L) {
     if (len >= L) return true;
     int d = c - lo;
@@ -261,11 +265,10 @@ __device__ __forceinline__ bool near_interval(int c, int lo, int len, int L) {
 // run() stencils reach 3 cells and a colloid's cell moves by at most one cell
 // within a step, so the colloid's cell at the start of the step lies within the
 // tile box grown by TILE_REACH + 4 (per axis, periodic).  Built once per step
-// by k_tile_colloids (ascending g); loaded here with the colloids' current data.
+// (ascending g).  A colloid that leaves the one-cell margin marks its
+// replica's lists stale and every workgroup then rebuilds its own list.
 constexpr int TILE_LIST_REACH = TILE_REACH + 4;
 
-// per tile: the colloids (ascending) whose cell at the start of this step, or
-// at run() time, lies within the tile box grown by TILE_LIST_REACH
 __device__ inline void build_tile_list(const Dev &d, const Rep &R, int T) {
     __shared__ int s_wtot[16];
     __shared__ int s_n;
@@ -301,7 +304,20 @@ __device__ inline void build_tile_list(const Dev &d, const Rep &R, int T) {
     __syncthreads();
 }
 
-__device__ inline void load_tile_colloids(const Dev &d, const Phys &ph, int T, bool kick_data, TileColl &tc) {
+struct TileColl {
+    int n;
+    int g[TCL_CAP];
+    double px[TCL_CAP], py[TCL_CAP], pz[TCL_CAP];
+    int cx[TCL_CAP], cy[TCL_CAP], cz[TCL_CAP];
+    double rx[TCL_CAP], ry[TCL_CAP], rz[TCL_CAP];    // position at run() time
+    int qx[TCL_CAP], qy[TCL_CAP], qz[TCL_CAP];       // its cell
+    double vx[TCL_CAP], vy[TCL_CAP], vz[TCL_CAP];    // velocity after the kick
+    double kx[TCL_CAP], ky[TCL_CAP], kz[TCL_CAP];    // per-particle kick
+};
+
+__device__ inline void load_tile_colloids(const Dev &d, const Phys &ph, const Rep &R, int r, int T, bool kick_data,
+                                          TileColl &tc) {
+    if (d.tcl_stale[r]) build_tile_list(d, R, T);
     const int n = min(d.tcl_cnt[T], TCL_CAP);
     for (int slot = threadIdx.x; slot < n; slot += blockDim.x) {
         int g = d.tcl_list[T * TCL_CAP + slot];
@@ -325,6 +341,52 @@ __device__ inline void load_tile_colloids(const Dev &d, const Phys &ph, int T, b
     __syncthreads();
 }
 
+// Stencil bit masks in LDS: for every cell of the tile's region (box grown by
+// TILE_REACH, where all of its particles stay), bit k is set when the cell is
+// in the 7^3-1 stencil of listed colloid k (k < MASK_BITS).  A particle then
+// finds its candidate colloids with one LDS read; most find none.
+struct Region {
+    int lo[3], len[3];
+};
+__device__ inline Region tile_region(const Rep &R, int tx, int ty, int tz) {
+    Region g;
+    int t[3] = {tx, ty, tz};
+    for (int k = 0; k < 3; k++) {
+        g.len[k] = R.B[k] + 2 * TILE_REACH;
+        g.lo[k] = t[k] * R.B[k] - TILE_REACH;
+        if (g.len[k] >= R.L) {
+            g.len[k] = R.L;
+            g.lo[k] = 0;
+        }
+    }
+    return g;
+}
+__device__ __forceinline__ int region_index(const Region &g, int ix, int iy, int iz, int L) {
+    int ux = ix - g.lo[0], uy = iy - g.lo[1], uz = iz - g.lo[2];
+    if (ux < 0) ux += L; else if (ux >= L) ux -= L;
+    if (uy < 0) uy += L; else if (uy >= L) uy -= L;
+    if (uz < 0) uz += L; else if (uz >= L) uz -= L;
+    if (ux >= g.len[0] || uy >= g.len[1] || uz >= g.len[2]) return -1;
+    return ux + g.len[0] * (uy + g.len[1] * uz);
+}
+// run_cells: stencils of the colloids' cells at run() time, else current cells
+__device__ inline void build_masks(const TileColl &tc, const Region &g, int L, bool run_cells, unsigned *mask) {
+    const int ncell = g.len[0] * g.len[1] * g.len[2];
+    for (int i = threadIdx.x; i < ncell; i += blockDim.x) mask[i] = 0u;
+    __syncthreads();
+    const int nk = min(tc.n, MASK_BITS);
+    for (int w = threadIdx.x; w < nk * 342; w += blockDim.x) {
+        int k = w / 342, s = w - k * 342;
+        int i = s < 171 ? s : s + 1;
+        int dx = i % 7 - 3, dy = (i / 7) % 7 - 3, dz = i / 49 - 3;
+        int cx = run_cells ? tc.qx[k] : tc.cx[k], cy = run_cells ? tc.qy[k] : tc.cy[k],
+            cz = run_cells ? tc.qz[k] : tc.cz[k];
+        int idx = region_index(g, wrapi(cx + dx, L), wrapi(cy + dy, L), wrapi(cz + dz, L), L);
+        if (idx >= 0) atomicOr(&mask[idx], 1u << k);
+    }
+    __syncthreads();
+}
+
 // run() rear-shell test of listed colloid k for a particle at binned cell
 // (ix,iy,iz) and absolute position (x,y,z) (cpp/run.cpp:17-18)
 __device__ __forceinline__ bool shell_test(const Phys &ph, const TileColl &tc, int k, int L, int ix, int iy,
@@ -336,32 +398,55 @@ __device__ __forceinline__ bool shell_test(const Phys &ph, const TileColl &tc, i
     double dot = (ax * tc.vx[k] + ay * tc.vy[k]) + az * tc.vz[k];
     return d2 <= ph.lim_shell && dot <= 0;
 }
+// candidate colloids of a particle, ascending: mask bits, then the unmasked
+// tail k >= MASK_BITS (every use applies the exact stencil test)
+template <class F>
+__device__ __forceinline__ void for_candidates(const TileColl &tc, unsigned m, F &&f) {
+    while (m) {
+        int k = __ffs(m) - 1;
+        m &= m - 1;
+        f(k);
+    }
+    for (int k = MASK_BITS; k < tc.n; k++) f(k);
+}
 
-// ------------------------------------------------------------ colloid phases
+// ------------------------------------------------------------ colloid work
+// Every colloid-only reference entry point, as per-colloid functions shared by
+// the one-workgroup-per-replica kernel (individual entry points) and the
+// wave-per-colloid kernel of the fused step; same operations in the same order.
+
+// cpp/fluid_colloid_collision.cpp:46-47 (the sums dv, dw were formed in the
+// reference's event order)
+__device__ inline void colloid_apply(const Phys &ph, const Dev &d, int g) {
+    for (int k = 0; k < 3; k++) {
+        d.cvel[3 * g + k] = d.cvel[3 * g + k] + d.cdv[3 * g + k] * ph.mass_fl / ph.mass_colloid;
+        d.comega[3 * g + k] = d.comega[3 * g + k] + d.cdw[3 * g + k] * ph.mass_fl / ph.I_colloid;
+        d.cdv[3 * g + k] = 0;
+        d.cdw[3 * g + k] = 0;
+    }
+}
 // cpp/update.cpp:34-46
-__device__ inline void activity_one(const Phys &ph, double *ra, const double *w) {
-    double bx = w[0] * ph.dt, by = w[1] * ph.dt, bz = w[2] * ph.dt;
+__device__ inline void colloid_activity(const Phys &ph, const Dev &d, int g) {
+    double bx = d.comega[3 * g] * ph.dt, by = d.comega[3 * g + 1] * ph.dt, bz = d.comega[3 * g + 2] * ph.dt;
     double sbx = sin(bx), sby = sin(by), sbz = sin(bz), cbx = cos(bx), cby = cos(by), cbz = cos(bz);
     double m1x = cby * cbz, m1y = -cby * sbz, m1z = sby;
     double m2x = sbx * sby * cbz + cbx * sbz, m2y = -sbx * sby * sbz + cbx * cbz, m2z = -sbx * cby;
     double m3x = -cbx * sby * cbz + sbx * sbz, m3y = cbx * sby * sbz + sbx * cbz, m3z = cbx * cby;
-    double a = ra[0], b = ra[1], c = ra[2];
-    ra[0] = (m1x * a + m1y * b) + m1z * c;
-    ra[1] = (m2x * a + m2y * b) + m2z * c;
-    ra[2] = (m3x * a + m3y * b) + m3z * c;
+    double a = d.cra[3 * g], b = d.cra[3 * g + 1], c = d.cra[3 * g + 2];
+    d.cra[3 * g] = (m1x * a + m1y * b) + m1z * c;
+    d.cra[3 * g + 1] = (m2x * a + m2y * b) + m2z * c;
+    d.cra[3 * g + 2] = (m3x * a + m3y * b) + m3z * c;
 }
-
 // cpp/update.cpp:8-13
 __device__ __forceinline__ double powi(double x, int r) {
     double ans = 1;
     for (int i = 1; i <= r; i++) ans *= x;
     return ans;
 }
-
 // WCA pair force on the pair (i, j), i the lower index: temp = img(p_i - p_j)
-// (cpp/update.cpp:21-28); returns false beyond r_cutoff
-__device__ __forceinline__ bool pair_force(const Phys &ph, double L, const double *pi, const double *pj,
-                                           double &fx, double &fy, double &fz) {
+// (cpp/update.cpp:21-28); false beyond r_cutoff
+__device__ __forceinline__ bool pair_force(const Phys &ph, double L, const double *pi, const double *pj, double &fx,
+                                           double &fy, double &fz) {
     double tx = img1(pi[0] - pj[0], L), ty = img1(pi[1] - pj[1], L), tz = img1(pi[2] - pj[2], L);
     double r = sqrt((tx * tx + ty * ty) + tz * tz);
     if (!(r < ph.r_cutoff)) return false;
@@ -372,141 +457,135 @@ __device__ __forceinline__ bool pair_force(const Phys &ph, double L, const doubl
     fz = (tz * mag) / r;
     return true;
 }
+// Partner j's term of f_g in the reference's accumulation (cpp/update.cpp:28):
+// f_g -= ff(j,g) for j < g, f_g += ff(g,j) for j > g.  sign carries the
+// operation; false beyond r_cutoff.
+__device__ __forceinline__ bool force_term(const Phys &ph, const Dev &d, double L, int g, int j, double *c,
+                                           bool &sub) {
+    sub = j < g;
+    return sub ? pair_force(ph, L, d.cpos + 3 * j, d.cpos + 3 * g, c[0], c[1], c[2])
+               : pair_force(ph, L, d.cpos + 3 * g, d.cpos + 3 * j, c[0], c[1], c[2]);
+}
+// cpp/update.cpp:14-32 for one colloid, serially in list order
+__device__ inline void colloid_force_serial(const Phys &ph, const Dev &d, double L, int g) {
+    double f0 = 0, f1 = 0, f2 = 0;
+    for (int m = 0; m < d.md_cnt[g]; m++) {
+        double c[3];
+        bool sub;
+        if (force_term(ph, d, L, g, d.md_list[g * MD_CAP + m], c, sub)) {
+            if (sub) { f0 = f0 - c[0]; f1 = f1 - c[1]; f2 = f2 - c[2]; }
+            else { f0 = f0 + c[0]; f1 = f1 + c[1]; f2 = f2 + c[2]; }
+        }
+    }
+    d.cf[3 * g] = f0; d.cf[3 * g + 1] = f1; d.cf[3 * g + 2] = f2;
+}
+// cpp/update.cpp:62-67
+__device__ inline void colloid_velocity(const Phys &ph, const Dev &d, int g) {
+    double dtb2 = ph.dt / (ph.mass_colloid * 2);
+    for (int k = 0; k < 3; k++) d.cvel[3 * g + k] = d.cvel[3 * g + k] + (d.coldf[3 * g + k] + d.cf[3 * g + k]) * dtb2;
+}
+// cpp/run.cpp:14 (colloid part of run) + the run-time snapshot
+__device__ inline void colloid_kick(const Phys &ph, const Dev &d, int g) {
+    double v0 = d.cv0[g];
+    for (int k = 0; k < 3; k++) {
+        d.cvel[3 * g + k] = d.cvel[3 * g + k] + d.cra[3 * g + k] * v0;
+        d.cpos_run[3 * g + k] = d.cpos[3 * g + k];
+        d.ccell_run[3 * g + k] = d.ccell[3 * g + k];
+    }
+    d.ccnt[g] = 0;
+}
+// cpp/active_brownian.cpp:28
+__device__ inline void colloid_oldf(const Dev &d, int g) {
+    for (int k = 0; k < 3; k++) d.coldf[3 * g + k] = d.cf[3 * g + k];
+}
+// cpp/update.cpp:48-54, into the other position buffer; marks the replica's
+// pair candidates / tile lists stale if the colloid left their margins
+__device__ inline void colloid_pos(const Phys &ph, const Dev &d, int L, int r, int g) {
+    double dt2 = ph.dt * ph.dt, ddt = 0.5 * dt2 / ph.mass_colloid;
+    bool far_tl = false;
+    double m2 = 0;
+    for (int k = 0; k < 3; k++) {
+        double dx = d.cvel[3 * g + k] * ph.dt + d.cf[3 * g + k] * ddt;
+        double p = d.cpos[3 * g + k] + dx;
+        d.cdisp[3 * g + k] += dx;
+        p = mod1(p, (double)L);
+        d.cpos_nxt[3 * g + k] = p;
+        int cc = coord_cell(p, L);
+        d.ccell_nxt[3 * g + k] = cc;
+        int dm = pdiff(cc, d.ccell_md[3 * g + k], L);
+        if (dm > 1 || dm < -1) far_tl = true;
+        double md = img1(p - d.cpos_md[3 * g + k], (double)L);
+        m2 += md * md;
+    }
+    if (m2 > 0.25 * MD_SKIN * MD_SKIN) atomicOr(&d.md_stale[r], 1);
+    if (far_tl) atomicOr(&d.tcl_stale[r], 1);
+}
+// pair test of cpp/neighbour_list.cpp:33-34 (symmetric: img(-a) = -img(a))
+__device__ __forceinline__ bool md_pair(const Phys &ph, const Dev &d, double L, int g, int j) {
+    const double *pi = d.cpos + 3 * g, *pj = d.cpos + 3 * j;
+    double tx, ty, tz;
+    if (j > g) {
+        tx = img1(pi[0] - pj[0], L); ty = img1(pi[1] - pj[1], L); tz = img1(pi[2] - pj[2], L);
+    } else {
+        tx = img1(pj[0] - pi[0], L); ty = img1(pj[1] - pi[1], L); tz = img1(pj[2] - pi[2], L);
+    }
+    return (tx * tx + ty * ty) + tz * tz < ph.lim_md;
+}
+// the pair list of colloid g (all partners, ascending), by one thread
+__device__ inline void colloid_mdlist_serial(const Phys &ph, const Dev &d, const Rep &R, int r, int g) {
+    const bool full = d.md_stale[r] != 0;
+    int ncand = full ? R.nc : d.mdc_cnt[g], w = 0;
+    for (int m = 0; m < ncand; m++) {
+        int j = full ? R.first_colloid + m : d.mdc_list[g * MDC_CAP + m];
+        if (j == g || !md_pair(ph, d, (double)R.L, g, j)) continue;
+        if (w < MD_CAP) d.md_list[g * MD_CAP + w] = j;
+        w++;
+    }
+    if (w > MD_CAP) { atomicOr(d.status, BW_FLAG_LIST_OVERFLOW); w = MD_CAP; }
+    d.md_cnt[g] = w;
+}
+// the same list by one wave (ballot compaction keeps the ascending order)
+__device__ inline void colloid_mdlist_wave(const Phys &ph, const Dev &d, const Rep &R, int r, int g) {
+    const int lane = threadIdx.x & 63;
+    const bool full = d.md_stale[r] != 0;
+    const int ncand = full ? R.nc : d.mdc_cnt[g];
+    int w = 0;
+    for (int base = 0; base < ncand; base += 64) {
+        int m = base + lane;
+        int j = -1;
+        bool p = false;
+        if (m < ncand) {
+            j = full ? R.first_colloid + m : d.mdc_list[g * MDC_CAP + m];
+            p = j != g && md_pair(ph, d, (double)R.L, g, j);
+        }
+        unsigned long long b = __ballot(p);
+        int rank = __popcll(b & ((1ull << lane) - 1ull));
+        if (p && w + rank < MD_CAP) d.md_list[g * MD_CAP + w + rank] = j;
+        w += __popcll(b);
+    }
+    if (lane == 0) {
+        if (w > MD_CAP) { atomicOr(d.status, BW_FLAG_LIST_OVERFLOW); w = MD_CAP; }
+        d.md_cnt[g] = w;
+    }
+}
 
-// All colloid work of one replica, run by one workgroup, phases in the order
-// of the PH_* flags.  Every colloid-only reference entry point is one flag.
+// All colloid work of one replica by one workgroup, phases in PH_* order.
 __device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R, int r, unsigned flags) {
     const int g0 = R.first_colloid, nc = R.nc;
     const double L = (double)R.L;
-    if (flags & PH_APPLY) {
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-            int g = g0 + i;
-            for (int k = 0; k < 3; k++) {
-                d.cvel[3 * g + k] = d.cvel[3 * g + k] + d.cdv[3 * g + k] * ph.mass_fl / ph.mass_colloid;
-                d.comega[3 * g + k] = d.comega[3 * g + k] + d.cdw[3 * g + k] * ph.mass_fl / ph.I_colloid;
-                d.cdv[3 * g + k] = 0;
-                d.cdw[3 * g + k] = 0;
-            }
-        }
-    }
-    if (flags & PH_ACTIVITY) {
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-            int g = g0 + i;
-            double ra[3] = {d.cra[3 * g], d.cra[3 * g + 1], d.cra[3 * g + 2]};
-            double w[3] = {d.comega[3 * g], d.comega[3 * g + 1], d.comega[3 * g + 2]};
-            activity_one(ph, ra, w);
-            for (int k = 0; k < 3; k++) d.cra[3 * g + k] = ra[k];
-        }
-    }
-    if (flags & PH_FORCE) {
-        // f_k in the reference's accumulation order: -ff(i,k) for partners
-        // i < k in ascending i, then +ff(k,j) for partners j > k ascending
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-            int g = g0 + i;
-            double f0 = 0, f1 = 0, f2 = 0;
-            int n = d.md_cnt[g];
-            const double *pk = d.cpos + 3 * g;
-            for (int m = 0; m < n; m++) {
-                int j = d.md_list[g * MD_CAP + m];
-                double fx, fy, fz;
-                if (j < g) {
-                    if (pair_force(ph, L, d.cpos + 3 * j, pk, fx, fy, fz)) { f0 = f0 - fx; f1 = f1 - fy; f2 = f2 - fz; }
-                } else {
-                    if (pair_force(ph, L, pk, d.cpos + 3 * j, fx, fy, fz)) { f0 = f0 + fx; f1 = f1 + fy; f2 = f2 + fz; }
-                }
-            }
-            d.cf[3 * g] = f0; d.cf[3 * g + 1] = f1; d.cf[3 * g + 2] = f2;
-        }
+    for (int i = threadIdx.x; i < nc; i += blockDim.x) {
+        int g = g0 + i;
+        if (flags & PH_APPLY) colloid_apply(ph, d, g);
+        if (flags & PH_ACTIVITY) colloid_activity(ph, d, g);
+        if (flags & PH_FORCE) colloid_force_serial(ph, d, L, g);
+        if (flags & PH_VELOCITY) colloid_velocity(ph, d, g);
+        if (flags & PH_KICK) colloid_kick(ph, d, g);
+        if (flags & PH_OLDF) colloid_oldf(d, g);
+        if (flags & PH_POS) colloid_pos(ph, d, R.L, r, g);
     }
     __syncthreads();
-    if (flags & PH_VELOCITY) {
-        double dtb2 = ph.dt / (ph.mass_colloid * 2);
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-            int g = g0 + i;
-            for (int k = 0; k < 3; k++)
-                d.cvel[3 * g + k] = d.cvel[3 * g + k] + (d.coldf[3 * g + k] + d.cf[3 * g + k]) * dtb2;
-        }
-    }
-    if (flags & PH_KICK) {
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-            int g = g0 + i;
-            double v0 = d.cv0[g];
-            for (int k = 0; k < 3; k++) {
-                d.cvel[3 * g + k] = d.cvel[3 * g + k] + d.cra[3 * g + k] * v0;
-                d.cpos_run[3 * g + k] = d.cpos[3 * g + k];
-                d.ccell_run[3 * g + k] = d.ccell[3 * g + k];
-            }
-            d.ccnt[g] = 0;
-        }
-    }
-    if (flags & PH_OLDF) {
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-            int g = g0 + i;
-            for (int k = 0; k < 3; k++) d.coldf[3 * g + k] = d.cf[3 * g + k];
-        }
-    }
-    __syncthreads();
-    if (flags & PH_POS) {
-        bool far = false;
-        double dt2 = ph.dt * ph.dt, ddt = 0.5 * dt2 / ph.mass_colloid;
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-            int g = g0 + i;
-            for (int k = 0; k < 3; k++) {
-                double dx = d.cvel[3 * g + k] * ph.dt + d.cf[3 * g + k] * ddt;
-                double p = d.cpos[3 * g + k] + dx;
-                d.cdisp[3 * g + k] += dx;
-                p = mod1(p, L);
-                d.cpos[3 * g + k] = p;
-                int cc = coord_cell(p, R.L);
-                d.ccell[3 * g + k] = cc;
-                int dm = pdiff(cc, d.ccell_md[3 * g + k], R.L);
-                if (dm > 1 || dm < -1) far = true;
-            }
-        }
-        // a colloid that left the one-cell margin of this step's tile lists:
-        // rebuild the replica's lists here (physically this needs speeds > 10)
-        if (__syncthreads_or(far)) {
-            for (int t = 0; t < R.n_tiles; t++) build_tile_list(d, R, R.first_tile + t);
-            for (int i = threadIdx.x; i < nc; i += blockDim.x)
-                for (int k = 0; k < 3; k++) d.ccell_md[3 * (g0 + i) + k] = d.ccell[3 * (g0 + i) + k];
-        }
-    }
-    __syncthreads();
-    if (flags & PH_MDLIST) {
-        // exact pair list: partners (ascending) with |img(p_i-p_j)|^2 < (3 sigma)^2,
-        // filtered from the candidate list while no colloid of the replica has
-        // moved more than half the skin since it was built, else by a full scan
-        bool moved = false;
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-            int g = g0 + i;
-            double mx = img1(d.cpos[3 * g] - d.cpos_md[3 * g], L), my = img1(d.cpos[3 * g + 1] - d.cpos_md[3 * g + 1], L),
-                   mz = img1(d.cpos[3 * g + 2] - d.cpos_md[3 * g + 2], L);
-            if ((mx * mx + my * my) + mz * mz > 0.25 * MD_SKIN * MD_SKIN) moved = true;
-        }
-        const bool full = __syncthreads_or(moved);
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) {
-            int g = g0 + i;
-            const double *pi = d.cpos + 3 * g;
-            int ncand = full ? nc : d.mdc_cnt[g], w = 0;
-            for (int m = 0; m < ncand; m++) {
-                int j = full ? g0 + m : d.mdc_list[g * MDC_CAP + m];
-                if (j == g) continue;
-                const double *pj = d.cpos + 3 * j;
-                double tx, ty, tz;
-                if (j > g) {
-                    tx = img1(pi[0] - pj[0], L); ty = img1(pi[1] - pj[1], L); tz = img1(pi[2] - pj[2], L);
-                } else {
-                    tx = img1(pj[0] - pi[0], L); ty = img1(pj[1] - pi[1], L); tz = img1(pj[2] - pi[2], L);
-                }
-                if ((tx * tx + ty * ty) + tz * tz < ph.lim_md) {
-                    if (w < MD_CAP) d.md_list[g * MD_CAP + w] = j;
-                    w++;
-                }
-            }
-            if (w > MD_CAP) { atomicOr(d.status, BW_FLAG_LIST_OVERFLOW); w = MD_CAP; }
-            d.md_cnt[g] = w;
-        }
-    }
+    if (flags & PH_MDLIST)
+        for (int i = threadIdx.x; i < nc; i += blockDim.x) colloid_mdlist_serial(ph, d, R, r, g0 + i);
     __syncthreads();
     if (flags & PH_POTENTIAL) {
         // cpp/update.cpp:24, one running sum over (i ascending, j > i ascending)
@@ -526,13 +605,6 @@ __device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R
                 }
             }
             d.cpot[r] = pot;
-        }
-    }
-    if (flags & PH_RESET_EVENTS) {
-        for (int i = threadIdx.x; i < nc; i += blockDim.x) d.ev_cnt[g0 + i] = 0;
-        if (threadIdx.x == 0) {
-            d.chain_cnt[r] = 0;
-            d.cont_cnt[r] = 0;
         }
     }
     __syncthreads();

@@ -1,18 +1,20 @@
 // HIP kernels of libbrownian for gfx950 (CDNA4, wave64).
 //
-// Hot path per MPCD step (fused path, one replica or many):
-//   k_step_begin      step counter, grid shift of this step            (1 WG)
-//   k_md_candidates   colloid pair candidates (cutoff + skin)           (1 wave / colloid)
-//   k_depart          per tile: new shifted tile of each particle,
-//                     stable departure lists per direction              (1 WG / tile)
-//   k_srd             per tile: gather from 27 neighbour lists, LDS
-//                     counting sort by shifted cell, FP64 per-cell sums,
-//                     rotation + thermostat, run() shell counts          (1 WG / tile)
-//   n_sub x { k_stream   per tile: run() kick (first substep), streaming,
-//                        collision detection against the tile's colloids
-//             k_collide  1 wave / colloid: stochastic reflections; the
-//                        last workgroup of a replica finishes the colloid
-//                        MD of the substep }
+// Fused MPCD step (one replica or many, one launch per kernel for all):
+//   k_step_begin       step counter, grid shift of this step               (1 WG)
+//   k_md_candidates    colloid pair candidates (cutoff + skin)              (1 wave / colloid)
+//   k_tile_colloids    per tile: colloids that can touch it this step        (1 WG / tile)
+//   k_depart           per tile: each particle's tile on the new shifted
+//                      grid, stable departure lists per direction           (1 WG / tile)
+//   k_srd              per tile: gather from 27 neighbour lists, LDS
+//                      counting sort by shifted cell, FP64 per-cell sums,
+//                      rotation + thermostat, run() shell counts             (1 WG / tile)
+//   n_sub x { k_stream          per tile: run() kick (first substep), streaming,
+//                                detection through LDS stencil masks
+//             k_collide         1 wave / colloid: pair list, reflections;
+//                                last WG of a replica: serial replays
+//             k_colloid_update  1 wave / colloid: collision impulse, activity,
+//                                WCA force, velocity, [kick], position }
 #include "bw_device.h"
 
 namespace bw {
@@ -74,12 +76,15 @@ __global__ __launch_bounds__(256) void k_md_candidates(Phys ph, Dev d, const int
             d.cpos_md[3 * g + k] = pi[k];
             d.ccell_md[3 * g + k] = d.ccell[3 * g + k];
         }
+        d.md_stale[wt.x] = 0;
     }
 }
 
 __global__ __launch_bounds__(256) void k_tile_colloids(Phys ph, Dev d) {
     const int T = blockIdx.x;
-    build_tile_list(d, d.rep[d.tile_rep[T]], T);
+    const int r = d.tile_rep[T];
+    build_tile_list(d, d.rep[r], T);
+    if (threadIdx.x == 0) d.tcl_stale[r] = 0;
 }
 
 // ------------------------------------------------------------ tile geometry
@@ -96,10 +101,6 @@ __device__ __forceinline__ TileGeo tile_geo(const Dev &d, int T, Rep &R) {
     t.tz = lt / (R.nt[0] * R.nt[1]);
     t.L = R.L;
     return t;
-}
-__device__ __forceinline__ int wrapi(int a, int n) {
-    a %= n;
-    return a < 0 ? a + n : a;
 }
 __device__ __forceinline__ int tile_delta(int tnew, int told, int nt) {
     int dlt = tnew - told;
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         int slot = T * ph.cap + i;
         int ix, iy, iz;
-        unpack_cell(f.cell[slot], L, ix, iy, iz);
+        unpack_cell(f.cell[slot], ix, iy, iz);
         int kx = shifted_coord(ix, f.ox[slot], sx, L);
         int ky = shifted_coord(iy, f.oy[slot], sy, L);
         int kz = shifted_coord(iz, f.oz[slot], sz, L);
@@ -201,6 +202,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     __shared__ int s_wtot[16];
     __shared__ TileColl tc;
     __shared__ int s_shell[TCL_CAP];
+    __shared__ unsigned s_mask[COUNT_SHELL ? MASK_MAX : 1];
     const int T = blockIdx.x;
     Rep R;
     TileGeo tg = tile_geo(d, T, R);
@@ -244,7 +246,11 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
         }
         s_pre[27] = acc;
     }
-    if (COUNT_SHELL) load_tile_colloids(d, ph, T, true, tc);
+    Region rg = tile_region(R, tg.tx, tg.ty, tg.tz);
+    if (COUNT_SHELL) {
+        load_tile_colloids(d, ph, R, tg.r, T, true, tc);
+        build_masks(tc, rg, L, true, s_mask);
+    }
     __syncthreads();
     int n = s_pre[27];
     if (n > cap) {
@@ -270,7 +276,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
             rvx[k] = src.vx[slot]; rvy[k] = src.vy[slot]; rvz[k] = src.vz[slot];
             rpid[k] = src.pid[slot];
             int ix, iy, iz;
-            unpack_cell(rcell[k], L, ix, iy, iz);
+            unpack_cell(rcell[k], ix, iy, iz);
             int lx = shifted_coord(ix, rox[k], sx, L) - tg.tx * B0;
             int ly = shifted_coord(iy, roy[k], sy, L) - tg.ty * B1;
             int lz = shifted_coord(iz, roz[k], sz, L) - tg.tz * B2;
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
                 uz = uz + (double)s_v[2 * cap + gi] / N;
             }
             int kx = tg.tx * B0 + c % B0, ky = tg.ty * B1 + (c / B0) % B1, kz = tg.tz * B2 + c / (B0 * B1);
-            int gc = kx + L * (ky + L * kz);
+            int gc = linear_cell(kx, ky, kz, L);
             double u1, u2;
             if (!draw_axis_raw(d, ph, R, step, gc, u1, u2)) {
                 missing = true;
@@ -388,10 +394,13 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
             }
             if (COUNT_SHELL) {
                 int ix, iy, iz;
-                unpack_cell(rcell[k], L, ix, iy, iz);
+                unpack_cell(rcell[k], ix, iy, iz);
+                int ri = region_index(rg, ix, iy, iz, L);
+                unsigned m = ri >= 0 ? s_mask[ri] : 0xffffffffu;
                 double x = (double)ix + (double)rox[k], y = (double)iy + (double)roy[k], z = (double)iz + (double)roz[k];
-                for (int q = 0; q < tc.n; q++)
-                    if (shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
+                for_candidates(tc, m, [&](int q) {
+                    if (q < tc.n && shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
+                });
             }
             int o = T * cap + g;
             dst.cell[o] = rcell[k];
@@ -400,7 +409,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
             dst.pid[o] = rpid[k];
             if (MAP) {
                 int kx = tg.tx * B0 + c % B0, ky = tg.ty * B1 + (c / B0) % B1, kz = tg.tz * B2 + c / (B0 * B1);
-                d.shifted_cell[R.first_fluid + rpid[k]] = kx + L * (ky + L * kz);
+                d.shifted_cell[R.first_fluid + rpid[k]] = linear_cell(kx, ky, kz, L);
             }
         }
     }
@@ -416,79 +425,118 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
 // Per tile: [run() kick on the rear shells, cpp/run.cpp:21-24]
 //           [streaming, cpp/update.cpp:56-60, with exact int-cell binning]
 //           [collision detection, cpp/fluid_colloid_collision.cpp:32-35]
+// Candidate colloids of a particle come from the tile's LDS stencil masks.
 template <class Real, bool KICK, bool STREAM, bool DETECT>
-__global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid<Real> f) {
-    __shared__ TileColl tc;
-    const int T = blockIdx.x;
-    Rep R;
-    TileGeo tg = tile_geo(d, T, R);
-    const int L = tg.L;
-    const double Ld = (double)L;
-    const int n = min(d.count[T], ph.cap);
-    if (KICK || DETECT) load_tile_colloids(d, ph, T, KICK, tc);
-    const Real dt = (Real)ph.dt;
-    const int lox = tg.tx * R.B[0] - TILE_REACH, loy = tg.ty * R.B[1] - TILE_REACH, loz = tg.tz * R.B[2] - TILE_REACH;
-    const int lnx = R.B[0] + 2 * TILE_REACH, lny = R.B[1] + 2 * TILE_REACH, lnz = R.B[2] + 2 * TILE_REACH;
-    bool drift = false, ch_over = false;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int slot = T * ph.cap + i;
+struct StreamOp {
+    const Phys &ph;
+    const Dev &d;
+    Fluid<Real> &f;
+    const TileColl &tc;
+    const Region &rg;
+    const unsigned *mdet, *mkick;
+    int L, r, lox, loy, loz, lnx, lny, lnz;
+    bool drift, ch_over;
+
+    __device__ __forceinline__ void operator()(int slot, int c, Real ox, Real oy, Real oz, Real vx, Real vy, Real vz) {
         int ix, iy, iz;
-        unpack_cell(f.cell[slot], L, ix, iy, iz);
-        Real ox = f.ox[slot], oy = f.oy[slot], oz = f.oz[slot];
-        Real vx = f.vx[slot], vy = f.vy[slot], vz = f.vz[slot];
+        unpack_cell(c, ix, iy, iz);
+        const double Ld = (double)L;
         if (KICK) {
-            double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
-            double wx = vx, wy = vy, wz = vz;
-            bool hit = false;
-            for (int k = 0; k < tc.n; k++)
-                if (shell_test(ph, tc, k, L, ix, iy, iz, x, y, z)) {
-                    wx = wx - tc.kx[k];
-                    wy = wy - tc.ky[k];
-                    wz = wz - tc.kz[k];
-                    hit = true;
+            int ri = region_index(rg, ix, iy, iz, L);
+            unsigned m = ri >= 0 ? mkick[ri] : 0xffffffffu;
+            if (m || tc.n > MASK_BITS) {
+                double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
+                double wx = vx, wy = vy, wz = vz;
+                bool hit = false;
+                for_candidates(tc, m, [&](int k) {
+                    if (k < tc.n && shell_test(ph, tc, k, L, ix, iy, iz, x, y, z)) {
+                        wx = wx - tc.kx[k];
+                        wy = wy - tc.ky[k];
+                        wz = wz - tc.kz[k];
+                        hit = true;
+                    }
+                });
+                if (hit) {
+                    vx = (Real)wx; vy = (Real)wy; vz = (Real)wz;
+                    f.vx[slot] = vx; f.vy[slot] = vy; f.vz[slot] = vz;
                 }
-            if (hit) {
-                vx = (Real)wx; vy = (Real)wy; vz = (Real)wz;
-                f.vx[slot] = vx; f.vy[slot] = vy; f.vz[slot] = vz;
             }
         }
         if (STREAM) {
+            const Real dt = (Real)ph.dt;
             ox = ox + vx * dt;
             oy = oy + vy * dt;
             oz = oz + vz * dt;
             normalise(ix, ox, L);
             normalise(iy, oy, L);
             normalise(iz, oz, L);
-            f.cell[slot] = pack_cell(ix, iy, iz, L);
+            f.cell[slot] = pack_cell(ix, iy, iz);
             f.ox[slot] = ox; f.oy[slot] = oy; f.oz[slot] = oz;
             if (!(near_interval(ix, lox, lnx, L) && near_interval(iy, loy, lny, L) && near_interval(iz, loz, lnz, L)))
                 drift = true;
         }
         if (DETECT) {
+            int ri = region_index(rg, ix, iy, iz, L);
+            unsigned m = ri >= 0 ? mdet[ri] : 0xffffffffu;
+            if (m == 0 && tc.n <= MASK_BITS) return;
             double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
             int n_in = 0, gin = -1;
-            for (int k = 0; k < tc.n; k++) {
-                if (!in_stencil(ix, iy, iz, tc.cx[k], tc.cy[k], tc.cz[k], L)) continue;
+            for_candidates(tc, m, [&](int k) {
+                if (k >= tc.n || !in_stencil(ix, iy, iz, tc.cx[k], tc.cy[k], tc.cz[k], L)) return;
                 double rx = img1(tc.px[k] - x, Ld), ry = img1(tc.py[k] - y, Ld), rz = img1(tc.pz[k] - z, Ld);
                 if ((rx * rx + ry * ry) + rz * rz <= ph.lim_coll) {
                     n_in++;
                     gin = tc.g[k];
                 }
-            }
+            });
             if (n_in > 0) {
                 int e = n_in == 1 ? atomicAdd(&d.ev_cnt[gin], 1) : EV_CAP;
                 if (e < EV_CAP) {
                     d.ev_slot[gin * EV_CAP + e] = slot;
-                } else {  // inside several colloids, or a colloid with a full bucket: serial replay
-                    int e2 = atomicAdd(&d.chain_cnt[tg.r], 1);
-                    if (e2 < CHAIN_CAP) d.chain_slot[tg.r * CHAIN_CAP + e2] = slot;
+                } else {  // inside several colloids, or beyond a full bucket: serial replay
+                    int e2 = atomicAdd(&d.chain_cnt[r], 1);
+                    if (e2 < CHAIN_CAP) d.chain_slot[r * CHAIN_CAP + e2] = slot;
                     else ch_over = true;
                 }
             }
         }
     }
-    if (drift) atomicOr(d.status, BW_FLAG_DRIFT);
-    if (ch_over) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
+};
+
+template <class Real, bool KICK, bool STREAM, bool DETECT>
+__global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid<Real> f) {
+    __shared__ TileColl tc;
+    __shared__ unsigned s_mask[(KICK ? 1 : 0) + (DETECT ? 1 : 0) + ((KICK || DETECT) ? 0 : 1)][MASK_MAX];
+    const int T = blockIdx.x;
+    Rep R;
+    TileGeo tg = tile_geo(d, T, R);
+    const int L = tg.L;
+    const int n = min(d.count[T], ph.cap);
+    Region rg = tile_region(R, tg.tx, tg.ty, tg.tz);
+    unsigned *mkick = s_mask[0], *mdet = s_mask[(KICK && DETECT) ? 1 : 0];
+    if (KICK || DETECT) {
+        load_tile_colloids(d, ph, R, tg.r, T, KICK, tc);
+        if (KICK) build_masks(tc, rg, L, true, mkick);
+        if (DETECT) build_masks(tc, rg, L, false, mdet);
+    }
+    StreamOp<Real, KICK, STREAM, DETECT> op{ph, d, f, tc, rg, mdet, mkick, L, tg.r,
+                                            tg.tx * R.B[0] - TILE_REACH, tg.ty * R.B[1] - TILE_REACH,
+                                            tg.tz * R.B[2] - TILE_REACH, R.B[0] + 2 * TILE_REACH,
+                                            R.B[1] + 2 * TILE_REACH, R.B[2] + 2 * TILE_REACH, false, false};
+    const int base_slot = T * ph.cap;
+    // two particles per thread per iteration: all loads first
+    for (int i0 = threadIdx.x; i0 < n; i0 += 2 * STREAM_THREADS) {
+        const int i1 = i0 + STREAM_THREADS;
+        const bool has1 = i1 < n;
+        const int s0 = base_slot + i0, s1 = base_slot + (has1 ? i1 : i0);
+        int c0 = f.cell[s0], c1 = f.cell[s1];
+        Real ox0 = f.ox[s0], oy0 = f.oy[s0], oz0 = f.oz[s0], vx0 = f.vx[s0], vy0 = f.vy[s0], vz0 = f.vz[s0];
+        Real ox1 = f.ox[s1], oy1 = f.oy[s1], oz1 = f.oz[s1], vx1 = f.vx[s1], vy1 = f.vy[s1], vz1 = f.vz[s1];
+        op(s0, c0, ox0, oy0, oz0, vx0, vy0, vz0);
+        if (has1) op(s1, c1, ox1, oy1, oz1, vx1, vy1, vz1);
+    }
+    if (op.drift) atomicOr(d.status, BW_FLAG_DRIFT);
+    if (op.ch_over) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
 }
 
 // --------------------------------------------------------------- collisions
@@ -531,14 +579,16 @@ __device__ __forceinline__ void transfer(const double *dump, double vx, double v
     c[5] = rs[0] * ey - rs[1] * ex;
 }
 
-// Particles inside one colloid and near no other: one wave per colloid, events
-// ranked in the reference's neigh_fl order (stencil position, then id) and the
-// momentum/torque sums taken in that order.  Particles shared between colloids
-// ("chain" particles) are replayed serially, colloid by colloid, by the last
-// workgroup of the replica, which then runs the colloid phases `tail`.
+// One wave per colloid g: its pair list for this substep, then the particles
+// inside g and no other colloid, ranked in the reference's neigh_fl order
+// (stencil position, then id) and reflected; the momentum/torque sums are taken
+// in that order.  A particle a reflection moved into a later colloid continues
+// the reference's colloid loop serially.  The last workgroup of a replica
+// replays the serial list: particles inside several colloids, every event of a
+// colloid whose bucket overflowed, and the continuations.
 template <class Real>
 __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Fluid<Real> f, int substep,
-                                                             unsigned tail, const int2 *wgt) {
+                                                             const int2 *wgt) {
     __shared__ double s_c[COLLIDE_THREADS / 64][EV_CAP][6];
     __shared__ int s_ne[COLLIDE_THREADS / 64];
     __shared__ int s_last;
@@ -552,12 +602,26 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     const int r = wt.x;
     const Rep R = d.rep[r];
     const int L = R.L;
+    const double Ld = (double)L;
     const int g = wt.y + w;
     const int gend = R.first_colloid + R.nc;
+    const bool valid = g < gend;
     const long long step = *d.step;
+    if (valid) colloid_mdlist_wave(ph, d, R, r, g);
+    __syncthreads();  // pair lists visible to every lane
     int ne = 0;
-    if (g < gend) {
+    if (valid) {
         ne = d.ev_cnt[g];
+        if (lane == 0) {
+            d.ev_cnt[g] = 0;
+            if (ne > EV_CAP) {
+                int k = atomicAdd(&d.ovf_cnt[r], 1);
+                if (k < OVF_CAP) d.ovf_list[r * OVF_CAP + k] = g;
+                else atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
+            } else if (ne > 0 && ph.draws == BW_DRAWS_INJECTED) {
+                atomicAdd(&d.ev_total[r], ne);
+            }
+        }
         if (ne > EV_CAP) ne = 0;  // full bucket: all its events are replayed serially
     }
     int slot = -1, pid = 0, ix = 0, iy = 0, iz = 0;
@@ -565,7 +629,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     if (lane < ne) {
         slot = d.ev_slot[g * EV_CAP + lane];
         pid = f.pid[slot];
-        unpack_cell(f.cell[slot], L, ix, iy, iz);
+        unpack_cell(f.cell[slot], ix, iy, iz);
         int dx = pdiff(ix, d.ccell[3 * g], L), dy = pdiff(iy, d.ccell[3 * g + 1], L),
             dz = pdiff(iz, d.ccell[3 * g + 2], L);
         key = ((long long)stencil_index(dx, dy, dz) << 32) | (unsigned)pid;
@@ -593,7 +657,6 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             // later colloid that now contains it is a pair partner of g; if
             // there is one, the rest of the colloid loop is replayed serially.
             const double ax = (double)ix + bx, ay = (double)iy + by, az = (double)iz + bz;
-            const double Ld = (double)L;
             for (int m = 0; m < d.md_cnt[g]; m++) {
                 int j = d.md_list[g * MD_CAP + m];
                 if (j <= g) continue;
@@ -620,7 +683,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     }
     if (lane == 0) s_ne[w] = ne;
     __syncthreads();
-    if (lane == 0 && g < gend) {
+    if (lane == 0 && valid) {
         double a[6] = {0, 0, 0, 0, 0, 0};
         for (int e = 0; e < s_ne[w]; e++)
             for (int k = 0; k < 6; k++) a[k] = a[k] + s_c[w][e][k];
@@ -640,32 +703,19 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     __syncthreads();
     if (!s_last) return;
     __threadfence();
-    if (threadIdx.x == 0) {
-        s_serial_events = 0;
-        s_inj_tot = 0;
-    }
-    // chain particles, serially in ascending id, each tested against every
-    // colloid of the replica in ascending order (the reference's colloid loop)
-    // Serial list, replayed in ascending particle id: particles inside several
-    // colloids and every event of a colloid whose bucket overflowed (tested
-    // from the replica's first colloid on, pre-collision velocity as dump), and
-    // continuations (from the colloid after the one that reflected them, with
-    // the velocity they had before that reflection as dump).
+    const int nchain = d.chain_cnt[r], ncont = d.cont_cnt[r], novf = min(d.ovf_cnt[r], OVF_CAP);
     if (threadIdx.x == 0) {
         d.ticket[r] = 0;
+        s_serial_events = 0;
+        s_inj_tot = 0;
         s_nser = 0;
     }
     __syncthreads();
-    bool over = false;
-    int my_ev = 0;
-    for (int i = threadIdx.x; i < R.nc; i += blockDim.x) {
-        int c = d.ev_cnt[R.first_colloid + i];
-        if (c > EV_CAP) over = true;
-        else my_ev += c;
-    }
-    const bool any_over = __syncthreads_or(over);
-    const int nchain = d.chain_cnt[r], ncont = d.cont_cnt[r];
-    if (any_over || nchain > 0 || ncont > 0) {
+    if (nchain > 0 || ncont > 0 || novf > 0) {
+        // serial list: shared particles and overflowed buckets (tested from the
+        // replica's first colloid on, dump = current velocity) and
+        // continuations (from the colloid after the reflecting one, dump =
+        // velocity before that reflection); replayed in ascending particle id
         if (threadIdx.x == 0) {
             int n = 0;
             int nch = min(nchain, CHAIN_CAP);
@@ -674,25 +724,12 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             int nco = min(ncont, CONT_CAP);
             if (ncont > CONT_CAP) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
             for (int i = 0; i < nco && n < SERIAL_CAP; i++, n++) { s_chain[n] = d.cont_slot[r * CONT_CAP + i]; s_src[n] = r * CONT_CAP + i; }
-            s_nser = n;
-        }
-        __syncthreads();
-        if (any_over)
-            for (int i = threadIdx.x; i < R.nc; i += blockDim.x) {
-                int gg = R.first_colloid + i;
-                if (d.ev_cnt[gg] > EV_CAP) {
-                    int b = atomicAdd(&s_nser, EV_CAP);
-                    for (int e = 0; e < EV_CAP; e++)
-                        if (b + e < SERIAL_CAP) { s_chain[b + e] = d.ev_slot[gg * EV_CAP + e]; s_src[b + e] = -1; }
-                }
+            for (int i = 0; i < novf; i++) {
+                int gg = d.ovf_list[r * OVF_CAP + i];
+                for (int e = 0; e < EV_CAP && n < SERIAL_CAP; e++, n++) { s_chain[n] = d.ev_slot[gg * EV_CAP + e]; s_src[n] = -1; }
             }
-        __syncthreads();
-        if (s_nser > SERIAL_CAP) {
-            if (threadIdx.x == 0) atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
-        }
-        const int nser = min(s_nser, SERIAL_CAP);
-        if (threadIdx.x == 0) {
-            for (int i = 1; i < nser; i++) {
+            if (n >= SERIAL_CAP) atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
+            for (int i = 1; i < n; i++) {
                 int s = s_chain[i], q = s_src[i], p = f.pid[s], j = i - 1;
                 while (j >= 0 && f.pid[s_chain[j]] > p) {
                     s_chain[j + 1] = s_chain[j];
@@ -702,16 +739,17 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
                 s_chain[j + 1] = s;
                 s_src[j + 1] = q;
             }
+            s_nser = n;
         }
         __syncthreads();
-        const double Ld = (double)L;
+        const int nser = s_nser;
         for (int i = 0; i < nser; i++) {
             const int s = s_chain[i], q = s_src[i];
             int cx, cy, cz;
-            unpack_cell(f.cell[s], L, cx, cy, cz);
+            unpack_cell(f.cell[s], cx, cy, cz);
             // colloids whose stencil holds the particle's binned cell, ascending
             const int from = q >= 0 ? d.cont_from[q] : R.first_colloid;
-            s_cand_n = 0;
+            if (threadIdx.x == 0) s_cand_n = 0;
             __syncthreads();
             for (int base = from; base < gend; base += blockDim.x) {
                 int gg = base + threadIdx.x;
@@ -765,13 +803,64 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             int base = substep * ph.n_colloid_total + R.first_colloid + i;
             my_inj += d.inj_off[base + 1] - d.inj_off[base];
         }
-        atomicAdd(&s_inj_tot, my_inj - my_ev);
+        atomicAdd(&s_inj_tot, my_inj);
         __syncthreads();
-        if (threadIdx.x == 0 && s_inj_tot != s_serial_events) atomicOr(d.status, BW_FLAG_DRAW_MISMATCH);
+        if (threadIdx.x == 0 && s_inj_tot != d.ev_total[r] + s_serial_events)
+            atomicOr(d.status, BW_FLAG_DRAW_MISMATCH);
     }
     __syncthreads();
-    __threadfence();
-    colloid_phases(ph, d, R, r, tail | PH_RESET_EVENTS);
+    if (threadIdx.x == 0) {
+        d.chain_cnt[r] = 0;
+        d.cont_cnt[r] = 0;
+        d.ovf_cnt[r] = 0;
+        d.ev_total[r] = 0;
+    }
+}
+
+// ----------------------------------------------------- colloid update (fused)
+// One wave per colloid: collision impulse, orientation, WCA force (pair terms
+// in parallel, summed in the reference's order by lane 0), velocity, [run()
+// kick], old force, position into the other buffer.
+__global__ __launch_bounds__(256) void k_colloid_update(Phys ph, Dev d, unsigned flags, const int2 *wgt) {
+    __shared__ double s_f[4][MD_CAP][3];
+    __shared__ unsigned char s_sub[4][MD_CAP], s_has[4][MD_CAP];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int2 wt = wgt[blockIdx.x];
+    const int r = wt.x;
+    const Rep R = d.rep[r];
+    const int g = wt.y + w;
+    const bool valid = g < R.first_colloid + R.nc;
+    const double L = (double)R.L;
+    int nmd = 0;
+    if (valid && (flags & PH_FORCE)) {
+        nmd = d.md_cnt[g];
+        for (int m = lane; m < nmd; m += 64) {
+            double c[3] = {0, 0, 0};
+            bool sub;
+            bool has = force_term(ph, d, L, g, d.md_list[g * MD_CAP + m], c, sub);
+            s_has[w][m] = has;
+            s_sub[w][m] = sub;
+            for (int k = 0; k < 3; k++) s_f[w][m][k] = c[k];
+        }
+    }
+    __syncthreads();
+    if (valid && lane == 0) {
+        if (flags & PH_APPLY) colloid_apply(ph, d, g);
+        if (flags & PH_ACTIVITY) colloid_activity(ph, d, g);
+        if (flags & PH_FORCE) {
+            double f0 = 0, f1 = 0, f2 = 0;
+            for (int m = 0; m < nmd; m++) {
+                if (!s_has[w][m]) continue;
+                if (s_sub[w][m]) { f0 = f0 - s_f[w][m][0]; f1 = f1 - s_f[w][m][1]; f2 = f2 - s_f[w][m][2]; }
+                else { f0 = f0 + s_f[w][m][0]; f1 = f1 + s_f[w][m][1]; f2 = f2 + s_f[w][m][2]; }
+            }
+            d.cf[3 * g] = f0; d.cf[3 * g + 1] = f1; d.cf[3 * g + 2] = f2;
+        }
+        if (flags & PH_VELOCITY) colloid_velocity(ph, d, g);
+        if (flags & PH_KICK) colloid_kick(ph, d, g);
+        if (flags & PH_OLDF) colloid_oldf(d, g);
+        if (flags & PH_POS) colloid_pos(ph, d, R.L, r, g);
+    }
 }
 
 // ------------------------------------------------------- run() (standalone)
@@ -779,21 +868,27 @@ template <class Real>
 __global__ __launch_bounds__(STREAM_THREADS) void k_run_count(Phys ph, Dev d, Fluid<Real> f) {
     __shared__ TileColl tc;
     __shared__ int s_shell[TCL_CAP];
+    __shared__ unsigned s_mask[MASK_MAX];
     const int T = blockIdx.x;
     Rep R;
     TileGeo tg = tile_geo(d, T, R);
     const int L = tg.L;
     const int n = min(d.count[T], ph.cap);
     for (int k = threadIdx.x; k < TCL_CAP; k += blockDim.x) s_shell[k] = 0;
-    load_tile_colloids(d, ph, T, true, tc);
+    Region rg = tile_region(R, tg.tx, tg.ty, tg.tz);
+    load_tile_colloids(d, ph, R, tg.r, T, true, tc);
+    build_masks(tc, rg, L, true, s_mask);
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int slot = T * ph.cap + i;
         int ix, iy, iz;
-        unpack_cell(f.cell[slot], L, ix, iy, iz);
+        unpack_cell(f.cell[slot], ix, iy, iz);
+        int ri = region_index(rg, ix, iy, iz, L);
+        unsigned m = ri >= 0 ? s_mask[ri] : 0xffffffffu;
         double x = (double)ix + (double)f.ox[slot], y = (double)iy + (double)f.oy[slot],
                z = (double)iz + (double)f.oz[slot];
-        for (int q = 0; q < tc.n; q++)
-            if (shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
+        for_candidates(tc, m, [&](int q) {
+            if (q < tc.n && shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
+        });
     }
     __syncthreads();
     for (int q = threadIdx.x; q < tc.n; q += blockDim.x)
@@ -807,7 +902,8 @@ __global__ void k_map_count(Phys ph, Dev d, Fluid<Real> f) {
     Rep R;
     tile_geo(d, T, R);
     const int n = min(d.count[T], ph.cap);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&d.map_count[R.first_cell + f.cell[T * ph.cap + i]], 1);
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        atomicAdd(&d.map_count[R.first_cell + linear_of_packed(f.cell[T * ph.cap + i], R.L)], 1);
 }
 template <class Real>
 __global__ void k_map_fill(Phys ph, Dev d, Fluid<Real> f) {
@@ -817,7 +913,7 @@ __global__ void k_map_fill(Phys ph, Dev d, Fluid<Real> f) {
     const int n = min(d.count[T], ph.cap);
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         int slot = T * ph.cap + i;
-        int pos = atomicAdd(&d.map_fill[R.first_cell + f.cell[slot]], 1);
+        int pos = atomicAdd(&d.map_fill[R.first_cell + linear_of_packed(f.cell[slot], R.L)], 1);
         d.map_list[pos] = f.pid[slot];
     }
 }
@@ -877,7 +973,7 @@ __global__ void k_neigh_fl(Phys ph, Dev d, const int *colloid_rep) {
     for (int k = threadIdx.x; k < 342; k += blockDim.x) {
         int i = k < 171 ? k : k + 1;
         int dx = i % 7 - 3, dy = (i / 7) % 7 - 3, dz = i / 49 - 3;
-        int c = pack_cell(wrapi(cx + dx, L), wrapi(cy + dy, L), wrapi(cz + dz, L), L);
+        int c = linear_cell(wrapi(cx + dx, L), wrapi(cy + dy, L), wrapi(cz + dz, L), L);
         s_off[k] = d.map_count[R.first_cell + c];
     }
     __syncthreads();
@@ -889,7 +985,7 @@ __global__ void k_neigh_fl(Phys ph, Dev d, const int *colloid_rep) {
     for (int k = threadIdx.x; k < 342; k += blockDim.x) {
         int i = k < 171 ? k : k + 1;
         int dx = i % 7 - 3, dy = (i / 7) % 7 - 3, dz = i / 49 - 3;
-        int c = pack_cell(wrapi(cx + dx, L), wrapi(cy + dy, L), wrapi(cz + dz, L), L);
+        int c = linear_cell(wrapi(cx + dx, L), wrapi(cy + dy, L), wrapi(cz + dz, L), L);
         long long gc = R.first_cell + c;
         int b = d.map_start[gc], cnt = d.map_count[gc];
         for (int e = 0; e < cnt; e++) {
@@ -934,8 +1030,8 @@ __global__ void k_gather(Phys ph, Dev d, Fluid<Real> f, double *pos, double *vel
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         int slot = T * ph.cap + i;
         long long id = R.first_fluid + f.pid[slot];
-        int c = f.cell[slot], ix, iy, iz;
-        unpack_cell(c, L, ix, iy, iz);
+        int ix, iy, iz;
+        unpack_cell(f.cell[slot], ix, iy, iz);
         double p[3] = {(double)ix + (double)f.ox[slot], (double)iy + (double)f.oy[slot],
                        (double)iz + (double)f.oz[slot]};
         for (int k = 0; k < 3; k++) {
@@ -944,7 +1040,7 @@ __global__ void k_gather(Phys ph, Dev d, Fluid<Real> f, double *pos, double *vel
             pos[3 * id + k] = p[k];
         }
         vel[3 * id] = f.vx[slot]; vel[3 * id + 1] = f.vy[slot]; vel[3 * id + 2] = f.vz[slot];
-        cell[id] = c;
+        cell[id] = linear_cell(ix, iy, iz, L);
     }
 }
 
@@ -963,6 +1059,8 @@ size_t srd_lds_bytes(int ncell, int cap, bool fp64) {
     return (size_t)ncell * 7 * 8 + 3 * (size_t)cap * rb + (size_t)cap * 4 + (size_t)(2 * ncell + 1) * 4 +
            (size_t)cap * 4 + 16;
 }
+// static LDS of the SRD kernel's shell-counting variant (colloid list, masks)
+size_t srd_static_lds() { return sizeof(TileColl) + 4 * TCL_CAP + 4 * MASK_MAX + 512; }
 
 template <class Real>
 void launch_depart(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream_t st) {
@@ -973,7 +1071,7 @@ void launch_srd(const Phys &ph, const Dev &d, const Fluid<Real> &src, const Flui
                 bool map, size_t lds, hipStream_t st) {
     static bool attr = false;
     if (!attr) {
-        int bytes = 160 * 1024 - 24 * 1024;
+        int bytes = 160 * 1024 - (int)srd_static_lds();
         (void)hipFuncSetAttribute((const void *)k_srd<Real, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         (void)hipFuncSetAttribute((const void *)k_srd<Real, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         (void)hipFuncSetAttribute((const void *)k_srd<Real, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -1003,9 +1101,9 @@ void launch_stream(const Phys &ph, const Dev &d, const Fluid<Real> &f, bool kick
     }
 }
 template <class Real>
-void launch_collide(const Phys &ph, const Dev &d, const Fluid<Real> &f, int substep, unsigned tail,
-                    const int2 *wgt, int nwg, hipStream_t st) {
-    LAUNCH(k_collide<Real>, nwg, COLLIDE_THREADS, 0, st, ph, d, f, substep, tail, wgt);
+void launch_collide(const Phys &ph, const Dev &d, const Fluid<Real> &f, int substep, const int2 *wgt, int nwg,
+                    hipStream_t st) {
+    LAUNCH(k_collide<Real>, nwg, COLLIDE_THREADS, 0, st, ph, d, f, substep, wgt);
 }
 template <class Real>
 void launch_run_count(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream_t st) {
@@ -1038,6 +1136,9 @@ void launch_step_begin(const Phys &ph, const Dev &d, hipStream_t st) { LAUNCH(k_
 void launch_colloid(const Phys &ph, const Dev &d, unsigned flags, hipStream_t st) {
     LAUNCH(k_colloid, ph.n_rep, 256, 0, st, ph, d, flags);
 }
+void launch_colloid_update(const Phys &ph, const Dev &d, unsigned flags, const int2 *wgt, int nwg, hipStream_t st) {
+    LAUNCH(k_colloid_update, nwg, 256, 0, st, ph, d, flags, wgt);
+}
 void launch_md_candidates(const Phys &ph, const Dev &d, const int2 *wgt, int nwg, hipStream_t st) {
     LAUNCH(k_md_candidates, nwg, 256, 0, st, ph, d, wgt);
 }
@@ -1055,8 +1156,8 @@ void launch_copy(const void *a, void *b, long long nbytes, hipStream_t st) {
                                    bool, size_t, hipStream_t);                                                     \
     template void launch_stream<Real>(const Phys &, const Dev &, const Fluid<Real> &, bool, bool, bool,           \
                                       hipStream_t);                                                                \
-    template void launch_collide<Real>(const Phys &, const Dev &, const Fluid<Real> &, int, unsigned,             \
-                                       const int2 *, int, hipStream_t);                                            \
+    template void launch_collide<Real>(const Phys &, const Dev &, const Fluid<Real> &, int, const int2 *, int,    \
+                                       hipStream_t);                                                               \
     template void launch_run_count<Real>(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);            \
     template void launch_maps<Real>(const Phys &, const Dev &, const Fluid<Real> &, long long, int *,             \
                                     const int *, hipStream_t);                                                     \
