@@ -19,9 +19,10 @@ size_t srd_lds_bytes(int ncell, int cap, bool fp64);
 size_t srd_static_lds();
 template <class Real> void launch_depart(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
 template <class Real>
-void launch_srd(const Phys &, const Dev &, const Fluid<Real> &, const Fluid<Real> &, bool, bool, size_t,
+void launch_srd(const Phys &, const Dev &, const Fluid<Real> &, const Fluid<Real> &, bool, bool, bool, size_t,
                 hipStream_t);
-template <class Real> void launch_stream(const Phys &, const Dev &, const Fluid<Real> &, bool, bool, bool, hipStream_t);
+template <class Real>
+void launch_stream(const Phys &, const Dev &, const Fluid<Real> &, bool, bool, bool, bool, hipStream_t);
 template <class Real>
 void launch_collide(const Phys &, const Dev &, const Fluid<Real> &, int, const int2 *, int, hipStream_t);
 template <class Real> void launch_run_count(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
@@ -67,6 +68,7 @@ struct FluidAlloc {
 struct bw_ctx {
     std::string err;
     bool fp64 = false;
+    bool block = false;  // blocked streaming in the fused step
     int n_rep = 0;
     std::vector<ReplicaHost> reps;
     Phys ph{};
@@ -112,7 +114,9 @@ int dalloc(bw_ctx *c, T **p, size_t n) {
     void *q = nullptr;
     hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
     if (e != hipSuccess) return fail(c, BW_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    // zero-filled before any later work on the context's (non-blocking) stream
     hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T));
+    hipDeviceSynchronize();
     c->allocs.push_back(q);
     *p = (T *)q;
     return BW_OK;
@@ -120,10 +124,21 @@ int dalloc(bw_ctx *c, T **p, size_t n) {
 
 bool divides(int a, int b) { return b > 0 && a % b == 0; }
 
-// cells of a tile's region: its box grown by TILE_REACH per side, at most L
-long long region_cells(int L, int bx, int by, int bz) {
-    auto ext = [&](int b) { return (long long)std::min(b + 2 * TILE_REACH, L); };
-    return ext(bx) * ext(by) * ext(bz);
+// slots per tile: mean occupancy + 15% (7 standard deviations of the Poisson
+// count at 10 particles per cell and 256 cells) + 64, in multiples of 64
+long long tile_slots(long long ncell, int density) {
+    long long need = (ncell * density * 23 + 19) / 20 + 64;
+    return (need + 63) / 64 * 64;
+}
+
+// tile colloid lists reach TILE_REACH + 4 cells beyond the tile box
+constexpr int TILE_LIST_REACH_HOST = TILE_REACH + 4;
+
+// a tile's region (its box grown by TILE_REACH per side, at most L) fits the
+// per-axis masks
+bool region_fits(int L, int bx, int by, int bz) {
+    auto ext = [&](int b) { return std::min(b + 2 * TILE_REACH, L); };
+    return ext(bx) <= AXIS_MAX && ext(by) <= AXIS_MAX && ext(bz) <= AXIS_MAX;
 }
 
 // tile shape: blocks of at least 4 cells per axis dividing L, about 256 cells
@@ -140,8 +155,8 @@ void choose_tile(int L, int density, bool fp64, int out[3]) {
             for (int bz : divs) {
                 if (by > bx || bz > by) continue;
                 int n = bx * by * bz;
-                if (n > 512 || (long long)n * density * 5 / 4 + 64 > MAX_CAP) continue;
-                if (region_cells(L, bx, by, bz) > MASK_MAX) continue;
+                if (n > 512 || tile_slots(n, density) > MAX_CAP) continue;
+                if (!region_fits(L, bx, by, bz)) continue;
                 double score = fabs(log((double)n / target)) + 0.01 * (bx - bz);
                 if (score < best) {
                     best = score;
@@ -180,7 +195,7 @@ Dev dev_for(bw_ctx *c, int buf) { return dev_at(c, buf, c->cb); }
 
 int check_status_code(unsigned s) {
     if (s & (BW_FLAG_DRAW_MISSING | BW_FLAG_DRAW_MISMATCH)) return BW_ERR_DRAWS;
-    if (s & (BW_FLAG_DRIFT | BW_FLAG_ZERO_VARIANCE)) return BW_ERR_PHYSICS;
+    if (s & (BW_FLAG_DRIFT | BW_FLAG_ZERO_VARIANCE | BW_FLAG_MARGIN)) return BW_ERR_PHYSICS;
     return BW_ERR_CAPACITY;
 }
 
@@ -188,6 +203,7 @@ void destroy_graphs(bw_ctx *c);
 
 int sync_injected(bw_ctx *c) {
     if (!c->inj_dirty || c->ph.draws != BW_DRAWS_INJECTED) return BW_OK;
+    HIPCHK(c, hipStreamSynchronize(c->st));  // in-flight steps read the old buffers
     int ns = c->ph.n_sub, nct = c->nc_total;
     std::vector<int> off((size_t)ns * nct + 1, 0), pids;
     std::vector<double> vals, shift(3 * c->n_rep), axes((size_t)c->cells_total * 2, NAN);
@@ -290,17 +306,18 @@ int enqueue_step(bw_ctx *c, int cur, int cb, bool final, Prof *prof) {
     if (prof) prof->mark(st, BW_KERNEL_COLLOID);
     launch_depart<Real>(ph, d0, src, st);
     if (prof) prof->mark(st, BW_KERNEL_DEPART);
-    launch_srd<Real>(ph, d0, src, dst, col, false, c->srd_lds, st);
+    launch_srd<Real>(ph, d0, src, dst, col, false, c->block, c->srd_lds, st);
     if (prof) prof->mark(st, BW_KERNEL_SRD);
     for (int l = 0; l < ph.n_sub; l++) {
         Dev d1 = dev_at(c, 1 - cur, cb);
-        launch_stream<Real>(ph, d1, dst, col && l == 0, true, col, st);
+        // blocked without colloids: the rotation pass did all the streaming
+        if (col || !c->block) launch_stream<Real>(ph, d1, dst, col && l == 0, true, col, true, st);
         if (prof) prof->mark(st, BW_KERNEL_STREAM);
         if (!col) continue;
         launch_collide<Real>(ph, d1, dst, l, c->wgt, c->nwg, st);
         if (prof) prof->mark(st, BW_KERNEL_COLLIDE);
         unsigned flags = PH_APPLY | PH_ACTIVITY | PH_FORCE | PH_VELOCITY;
-        if (l < ph.n_sub - 1) flags |= PH_OLDF | PH_POS;
+        if (l < ph.n_sub - 1) flags |= PH_OLDF | PH_POS | (c->block ? PH_MARGIN : 0u);
         else if (!final) flags |= PH_KICK | PH_OLDF | PH_POS;
         launch_colloid_update(ph, d1, flags, c->wgt, c->nwg, st);
         if (prof) prof->mark(st, BW_KERNEL_COLLOID);
@@ -397,7 +414,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         if (p.n_sub != p0.n_sub || p.ndt != p0.ndt || p.kbt != p0.kbt || p.mass_fl != p0.mass_fl ||
             p.mass_colloid != p0.mass_colloid || p.sig_colloid != p0.sig_colloid || p.eps != p0.eps ||
             p.sigma != p0.sigma || p.theta_deg != p0.theta_deg || p.precision != p0.precision ||
-            p.draws != p0.draws) {
+            p.draws != p0.draws || p.streaming != p0.streaming) {
             delete c;
             return BW_ERR_ARG;
         }
@@ -430,6 +447,14 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
     ph.lim_md_skin = pow(3.0 * p0.sig_colloid + MD_SKIN, 2);
     ph.vscale_thermal = sqrt(p0.kbt / p0.mass_fl);
     ph.draws = p0.draws;
+    // blocked streaming: near radius sigma/2 + margin + travel (<= vcap_ndt),
+    // run() shell radius sigma/2 + 0.5; 1e-3 covers FP32 offset rounding
+    ph.margin = STREAM_MARGIN;
+    ph.vcap_ndt = 0.9;  // < 1: a far particle moves less than a cell per substep
+    ph.near_base = 0.5 * p0.sigma + ph.margin + 1e-3;
+    ph.near_shell2 = pow(0.5 * p0.sigma + 0.5 + 1e-3, 2);
+    ph.near_reach = (int)floor(ph.near_base + ph.vcap_ndt) + 1;
+    c->block = p0.streaming == BW_STREAM_BLOCKED && ph.near_reach <= TILE_LIST_REACH_HOST - 1;
     ph.n_rep = n_rep;
     // replicas
     int cap = 0;
@@ -457,12 +482,11 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
             R.B[k] = B[k];
             R.nt[k] = p.L / B[k];
         }
-        if (B[0] * B[1] * B[2] > 512 || region_cells(p.L, B[0], B[1], B[2]) > MASK_MAX) {
+        if (B[0] * B[1] * B[2] > 512 || !region_fits(p.L, B[0], B[1], B[2])) {
             delete c;
             return BW_ERR_ARG;
         }
-        int need = (B[0] * B[1] * B[2] * p.density * 5) / 4 + 64;
-        need = (need + 63) / 64 * 64;
+        int need = (int)tile_slots(B[0] * B[1] * B[2], p.density);
         cap = std::max(cap, need);
         R.n_tiles = R.nt[0] * R.nt[1] * R.nt[2];
         R.first_tile = (int)tiles;
@@ -517,6 +541,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         if ((rc = dalloc(c, &c->counts[b], tiles))) { *out = c; return rc; }
     }
     Dev &d = c->dv;
+    if ((rc = dalloc(c, &d.near_cnt, tiles))) { *out = c; return rc; }
     std::vector<Rep> hrep(n_rep);
     std::vector<int> trep(tiles), crep(std::max(nc, 1));
     for (int r = 0; r < n_rep; r++) {
@@ -780,7 +805,7 @@ BW_API int bw_status(bw_ctx *c, unsigned *flags) {
     unsigned s = 0;
     HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, hipMemcpy(&s, c->dv.status, sizeof(unsigned), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemset(c->dv.status, 0, sizeof(unsigned)));
+    HIPCHK(c, hipMemsetAsync(c->dv.status, 0, sizeof(unsigned), c->st));
     if (flags) *flags = s;
     if (s) {
         char buf[96];
@@ -795,18 +820,20 @@ BW_API int initialize(bw_ctx *c) {
     if (!c) return BW_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->st));
     for (auto &R : c->reps) R.rng.reset(R.p.seed);
-    for (int b = 0; b < 2; b++) HIPCHK(c, hipMemset(c->counts[b], 0, sizeof(int) * c->ph.n_tiles));
-    HIPCHK(c, hipMemset(c->dv.step, 0, sizeof(long long)));
-    HIPCHK(c, hipMemset(c->dv.status, 0, sizeof(unsigned)));
+    // ordered on the context's stream before any later launch
+    for (int b = 0; b < 2; b++) HIPCHK(c, hipMemsetAsync(c->counts[b], 0, sizeof(int) * c->ph.n_tiles, c->st));
+    HIPCHK(c, hipMemsetAsync(c->dv.step, 0, sizeof(long long), c->st));
+    HIPCHK(c, hipMemsetAsync(c->dv.status, 0, sizeof(unsigned), c->st));
     size_t NC = std::max(c->nc_total, 1);
     double *arrs[] = {c->cposb[0], c->cposb[1], c->dv.cvel, c->dv.comega, c->dv.cf, c->dv.coldf,
                       c->dv.cra, c->dv.cdisp, c->dv.cdv, c->dv.cdw};
-    for (double *a : arrs) HIPCHK(c, hipMemset(a, 0, sizeof(double) * 3 * NC));
-    for (int b = 0; b < 2; b++) HIPCHK(c, hipMemset(c->ccellb[b], 0, sizeof(int) * 3 * NC));
-    HIPCHK(c, hipMemset(c->dv.ev_cnt, 0, sizeof(int) * NC));
+    for (double *a : arrs) HIPCHK(c, hipMemsetAsync(a, 0, sizeof(double) * 3 * NC, c->st));
+    for (int b = 0; b < 2; b++) HIPCHK(c, hipMemsetAsync(c->ccellb[b], 0, sizeof(int) * 3 * NC, c->st));
+    HIPCHK(c, hipMemsetAsync(c->dv.ev_cnt, 0, sizeof(int) * NC, c->st));
     int *per_rep[] = {c->dv.chain_cnt, c->dv.cont_cnt, c->dv.ovf_cnt, c->dv.ev_total, c->dv.md_stale, c->dv.tcl_stale};
-    for (int *a : per_rep) HIPCHK(c, hipMemset(a, 0, sizeof(int) * c->n_rep));
-    HIPCHK(c, hipMemset(c->dv.ticket, 0, sizeof(unsigned) * c->n_rep));
+    for (int *a : per_rep) HIPCHK(c, hipMemsetAsync(a, 0, sizeof(int) * c->n_rep, c->st));
+    HIPCHK(c, hipMemsetAsync(c->dv.ticket, 0, sizeof(unsigned) * c->n_rep, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
     c->cb = 0;
     c->pending_prologue = true;
     c->substep = 0;
@@ -828,6 +855,7 @@ BW_API int initialize_colloid(bw_ctx *c) {
 
 BW_API int initialize_fluid(bw_ctx *c) {
     if (!c) return BW_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
     for (int r = 0; r < c->n_rep; r++) {
         ReplicaHost &R = c->reps[r];
         std::vector<double> cpos(3 * std::max(R.nc, 1));
@@ -850,6 +878,7 @@ BW_API int create_box(bw_ctx *c) {
 
 BW_API int tumble(bw_ctx *c) {
     if (!c) return BW_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
     for (int r = 0; r < c->n_rep; r++) {
         ReplicaHost &R = c->reps[r];
         if (!R.nc) continue;
@@ -911,10 +940,10 @@ BW_API int rotation_mpcd(bw_ctx *c) {
     launch_step_begin(c->ph, d, c->st);
     if (c->fp64) {
         launch_depart<double>(c->ph, d, c->fl[c->cur].f64, c->st);
-        launch_srd<double>(c->ph, d, c->fl[c->cur].f64, c->fl[1 - c->cur].f64, false, true, c->srd_lds, c->st);
+        launch_srd<double>(c->ph, d, c->fl[c->cur].f64, c->fl[1 - c->cur].f64, false, true, false, c->srd_lds, c->st);
     } else {
         launch_depart<float>(c->ph, d, c->fl[c->cur].f32, c->st);
-        launch_srd<float>(c->ph, d, c->fl[c->cur].f32, c->fl[1 - c->cur].f32, false, true, c->srd_lds, c->st);
+        launch_srd<float>(c->ph, d, c->fl[c->cur].f32, c->fl[1 - c->cur].f32, false, true, false, c->srd_lds, c->st);
     }
     c->cur ^= 1;
     c->substep = 0;
@@ -929,10 +958,10 @@ BW_API int run(bw_ctx *c) {
     launch_tile_colloids(c->ph, d, c->st);
     if (c->fp64) {
         launch_run_count<double>(c->ph, d, c->fl[c->cur].f64, c->st);
-        launch_stream<double>(c->ph, d, c->fl[c->cur].f64, true, false, false, c->st);
+        launch_stream<double>(c->ph, d, c->fl[c->cur].f64, true, false, false, false, c->st);
     } else {
         launch_run_count<float>(c->ph, d, c->fl[c->cur].f32, c->st);
-        launch_stream<float>(c->ph, d, c->fl[c->cur].f32, true, false, false, c->st);
+        launch_stream<float>(c->ph, d, c->fl[c->cur].f32, true, false, false, false, c->st);
     }
     return BW_OK;
 }
@@ -940,8 +969,8 @@ BW_API int run(bw_ctx *c) {
 BW_API int update_pos_mpcd(bw_ctx *c) {
     if (!c) return BW_ERR_ARG;
     Dev d = dev_for(c, c->cur);
-    if (c->fp64) launch_stream<double>(c->ph, d, c->fl[c->cur].f64, false, true, false, c->st);
-    else launch_stream<float>(c->ph, d, c->fl[c->cur].f32, false, true, false, c->st);
+    if (c->fp64) launch_stream<double>(c->ph, d, c->fl[c->cur].f64, false, true, false, false, c->st);
+    else launch_stream<float>(c->ph, d, c->fl[c->cur].f32, false, true, false, false, c->st);
     return BW_OK;
 }
 
@@ -954,10 +983,10 @@ BW_API int fluid_colloid_collision(bw_ctx *c) {
     int l = c->substep % c->ph.n_sub;
     launch_tile_colloids(c->ph, d, c->st);
     if (c->fp64) {
-        launch_stream<double>(c->ph, d, c->fl[c->cur].f64, false, false, true, c->st);
+        launch_stream<double>(c->ph, d, c->fl[c->cur].f64, false, false, true, false, c->st);
         launch_collide<double>(c->ph, d, c->fl[c->cur].f64, l, c->wgt, c->nwg, c->st);
     } else {
-        launch_stream<float>(c->ph, d, c->fl[c->cur].f32, false, false, true, c->st);
+        launch_stream<float>(c->ph, d, c->fl[c->cur].f32, false, false, true, false, c->st);
         launch_collide<float>(c->ph, d, c->fl[c->cur].f32, l, c->wgt, c->nwg, c->st);
     }
     launch_colloid(c->ph, d, PH_APPLY, c->st);
@@ -1005,6 +1034,7 @@ BW_API int bw_time_steps(bw_ctx *c, int nsteps, double *ms) {
 BW_API int bw_set_v0(bw_ctx *c, int r, const double *v0) {
     ReplicaHost *R = rep_of(c, r);
     if (!R) return BW_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
     for (int j = 0; j < R->nc; j++) R->v0[j] = v0 ? v0[j] : R->p.v0;
     if (R->nc)
         HIPCHK(c, hipMemcpy(c->dv.cv0 + R->first_colloid, R->v0.data(), sizeof(double) * R->nc, hipMemcpyHostToDevice));
@@ -1267,7 +1297,7 @@ BW_API int bw_debug_copy(bw_ctx *c, long long nbytes, int reps, double *ms) {
     void *a = nullptr, *b = nullptr;
     HIPCHK(c, hipMalloc(&a, nbytes));
     HIPCHK(c, hipMalloc(&b, nbytes));
-    hipMemset(a, 0, nbytes);
+    HIPCHK(c, hipMemsetAsync(a, 0, nbytes, c->st));
     launch_copy(a, b, nbytes, c->st);
     hipEvent_t e0, e1;
     hipEventCreate(&e0);

@@ -19,12 +19,12 @@ constexpr int CONT_CAP = 512;       // continuations per replica per substep
 constexpr int SERIAL_CAP = 2048;    // serial list incl. entries of overflowed buckets
 constexpr int TCL_CAP = 64;         // colloids listed for one tile
 constexpr int MASK_BITS = 32;       // listed colloids covered by the LDS stencil masks
-constexpr int MASK_MAX = 4096;      // cells of a tile's region (box grown by TILE_REACH)
+constexpr int AXIS_MAX = 64;        // cells per axis of a tile's region (box grown by TILE_REACH)
 constexpr int CELL_BITS = 10;       // packed cell coordinates: L <= 1023
 constexpr int MD_CAP = 128;         // full MD neighbour list per colloid
 constexpr int MDC_CAP = 256;        // MD candidates (cutoff + skin) per colloid
 constexpr int MAX_CAP = 4096;       // slots per tile upper bound
-constexpr int SRD_THREADS = 512;
+constexpr int SRD_THREADS = 1024;
 constexpr int SRD_PT = MAX_CAP / SRD_THREADS;  // particles per thread in the SRD kernel
 constexpr int STREAM_THREADS = 256;
 constexpr int DEPART_THREADS = 256;
@@ -33,6 +33,7 @@ constexpr int OVF_CAP = 64;         // colloids with an overflowed event bucket,
 constexpr int NDIR = 27;
 constexpr int TILE_REACH = 3;       // cells a particle may stand outside its tile box
 constexpr double MD_SKIN = 2.0;     // candidate-list skin for the colloid pair list
+constexpr double STREAM_MARGIN = 0.25;  // colloid displacement bound within a step (blocked streaming)
 
 // colloid phase flags (applied in this order)
 enum : unsigned {
@@ -46,6 +47,7 @@ enum : unsigned {
     PH_MDLIST = 1u << 7,    // filter candidates into the pair list (cpp/neighbour_list.cpp:27-39)
     PH_POTENTIAL = 1u << 8, // potential in the reference's summation order
     PH_RESET_EVENTS = 1u << 9,
+    PH_MARGIN = 1u << 10,   // check the colloid displacement bound of blocked streaming
 };
 
 struct Phys {
@@ -58,6 +60,13 @@ struct Phys {
     double lim_md;                    // pow(3*sig_colloid,2)
     double lim_md_skin;               // (3*sig_colloid + skin)^2
     double vscale_thermal;            // sqrt(kbt/mass_fl)
+    // blocked streaming (fused step): a particle is "near" when within
+    // near_shell2 (squared) of a colloid's run() position, or within
+    // near_base + its travel in the step of a colloid's position, or when its
+    // travel exceeds vcap_ndt; colloids must stay within `margin` of their
+    // position at the step's start
+    double margin, near_base, near_shell2, vcap_ndt;
+    int near_reach;                   // per-axis cell reach of the near test
     int n_sub;
     int cap;                          // slots per tile
     int n_rep, n_tiles, n_colloid_total;
@@ -87,6 +96,7 @@ struct Dev {
     const int *tile_rep;
     int *count;          // [n_tiles] particles in each tile (current buffer)
     int *count_next;     // [n_tiles] for the buffer being written
+    int *near_cnt;       // [n_tiles] particles stored first that the substeps touch
     int *dep_off;        // [n_tiles * 28]
     int *dep_idx;        // [n_tiles * cap]
     // colloids (global index g), FP64, 3 per colloid

@@ -181,6 +181,25 @@ __device__ __forceinline__ void normalise(int &i, Real &o, int L) {
     }
 }
 
+// normalise() for an offset in (-1, 2): the same operations, selected by
+// comparisons instead of floor (o - (-1) and o + 1 are the same IEEE add)
+template <class Real>
+__device__ __forceinline__ void step_cell(int &i, Real &o, int L) {
+    if (o < Real(0)) {
+        o = o + Real(1);
+        i -= 1;
+        if (o >= Real(1)) {
+            o = Real(0);
+            i += 1;
+        }
+        if (i < 0) i += L;
+    } else if (o >= Real(1)) {
+        o = o - Real(1);
+        i += 1;
+        if (i >= L) i -= L;
+    }
+}
+
 // ------------------------------------------------------------ reflection
 // Stochastic (thermal-wall) reflection, cpp/fluid_colloid_collision.cpp:7-24:
 // Rayleigh normal speed along rs, tangential Gaussian along un x t.
@@ -315,6 +334,13 @@ struct TileColl {
     double kx[TCL_CAP], ky[TCL_CAP], kz[TCL_CAP];    // per-particle kick
 };
 
+// listed colloids' current and run() positions relative to a tile's origin,
+// FP32, for the conservative near test of blocked streaming
+struct NearColl {
+    float cx[TCL_CAP], cy[TCL_CAP], cz[TCL_CAP];
+    float rx[TCL_CAP], ry[TCL_CAP], rz[TCL_CAP];
+};
+
 __device__ inline void load_tile_colloids(const Dev &d, const Phys &ph, const Rep &R, int r, int T, bool kick_data,
                                           TileColl &tc) {
     if (d.tcl_stale[r]) build_tile_list(d, R, T);
@@ -341,10 +367,14 @@ __device__ inline void load_tile_colloids(const Dev &d, const Phys &ph, const Re
     __syncthreads();
 }
 
-// Stencil bit masks in LDS: for every cell of the tile's region (box grown by
-// TILE_REACH, where all of its particles stay), bit k is set when the cell is
-// in the 7^3-1 stencil of listed colloid k (k < MASK_BITS).  A particle then
-// finds its candidate colloids with one LDS read; most find none.
+// Candidate colloids of a cell through per-axis bit masks in LDS.  The 7^3-1
+// stencil of a colloid is a cube of cells minus its centre, so "cell in the
+// cube of colloid k" is the AND over the three axes of "coordinate within
+// `reach` cells of the colloid's".  For the tile's region (its box grown by
+// TILE_REACH per side, where all of its particles stay) bit k of
+// m[a][u] is set when axis-a coordinate u lies within reach of listed colloid
+// k (k < MASK_BITS).  Every use applies the exact test afterwards (stencil
+// centre, distance), so the masks only prune.
 struct Region {
     int lo[3], len[3];
 };
@@ -361,28 +391,35 @@ __device__ inline Region tile_region(const Rep &R, int tx, int ty, int tz) {
     }
     return g;
 }
-__device__ __forceinline__ int region_index(const Region &g, int ix, int iy, int iz, int L) {
-    int ux = ix - g.lo[0], uy = iy - g.lo[1], uz = iz - g.lo[2];
-    if (ux < 0) ux += L; else if (ux >= L) ux -= L;
-    if (uy < 0) uy += L; else if (uy >= L) uy -= L;
-    if (uz < 0) uz += L; else if (uz >= L) uz -= L;
-    if (ux >= g.len[0] || uy >= g.len[1] || uz >= g.len[2]) return -1;
-    return ux + g.len[0] * (uy + g.len[1] * uz);
+// index of coordinate i (in [0,L)) along axis a of the region, -1 outside
+__device__ __forceinline__ int axis_index(const Region &g, int a, int i, int L) {
+    int u = i - g.lo[a];
+    if (u < 0) u += L;
+    else if (u >= L) u -= L;
+    return u < g.len[a] ? u : -1;
 }
-// run_cells: stencils of the colloids' cells at run() time, else current cells
-__device__ inline void build_masks(const TileColl &tc, const Region &g, int L, bool run_cells, unsigned *mask) {
-    const int ncell = g.len[0] * g.len[1] * g.len[2];
-    for (int i = threadIdx.x; i < ncell; i += blockDim.x) mask[i] = 0u;
+struct AxisMask {
+    unsigned m[3][AXIS_MAX];
+};
+// candidate bits of cell (ix,iy,iz); all bits outside the region
+__device__ __forceinline__ unsigned cell_mask(const AxisMask &am, const Region &g, int ix, int iy, int iz, int L) {
+    int ux = axis_index(g, 0, ix, L), uy = axis_index(g, 1, iy, L), uz = axis_index(g, 2, iz, L);
+    if ((ux | uy | uz) < 0) return 0xffffffffu;
+    return am.m[0][ux] & am.m[1][uy] & am.m[2][uz];
+}
+// run_cells: the colloids' cells at run() time, else their current cells
+__device__ inline void build_axis_masks(const TileColl &tc, const Region &g, int L, bool run_cells, int reach,
+                                        AxisMask &am) {
+    for (int i = threadIdx.x; i < 3 * AXIS_MAX; i += blockDim.x) (&am.m[0][0])[i] = 0u;
     __syncthreads();
-    const int nk = min(tc.n, MASK_BITS);
-    for (int w = threadIdx.x; w < nk * 342; w += blockDim.x) {
-        int k = w / 342, s = w - k * 342;
-        int i = s < 171 ? s : s + 1;
-        int dx = i % 7 - 3, dy = (i / 7) % 7 - 3, dz = i / 49 - 3;
-        int cx = run_cells ? tc.qx[k] : tc.cx[k], cy = run_cells ? tc.qy[k] : tc.cy[k],
-            cz = run_cells ? tc.qz[k] : tc.cz[k];
-        int idx = region_index(g, wrapi(cx + dx, L), wrapi(cy + dy, L), wrapi(cz + dz, L), L);
-        if (idx >= 0) atomicOr(&mask[idx], 1u << k);
+    const int nk = min(tc.n, MASK_BITS), span = 2 * reach + 1;
+    for (int w = threadIdx.x; w < nk * 3 * span; w += blockDim.x) {
+        int k = w / (3 * span), rem = w - k * 3 * span;
+        int a = rem / span, dd = rem - a * span - reach;
+        int c = a == 0 ? (run_cells ? tc.qx[k] : tc.cx[k]) : a == 1 ? (run_cells ? tc.qy[k] : tc.cy[k])
+                                                                      : (run_cells ? tc.qz[k] : tc.cz[k]);
+        int u = axis_index(g, a, wrapi(c + dd, L), L);
+        if (u >= 0) atomicOr(&am.m[a][u], 1u << k);
     }
     __syncthreads();
 }
@@ -500,7 +537,7 @@ __device__ inline void colloid_oldf(const Dev &d, int g) {
 }
 // cpp/update.cpp:48-54, into the other position buffer; marks the replica's
 // pair candidates / tile lists stale if the colloid left their margins
-__device__ inline void colloid_pos(const Phys &ph, const Dev &d, int L, int r, int g) {
+__device__ inline void colloid_pos(const Phys &ph, const Dev &d, int L, int r, int g, bool margin) {
     double dt2 = ph.dt * ph.dt, ddt = 0.5 * dt2 / ph.mass_colloid;
     bool far_tl = false;
     double m2 = 0;
@@ -519,6 +556,7 @@ __device__ inline void colloid_pos(const Phys &ph, const Dev &d, int L, int r, i
     }
     if (m2 > 0.25 * MD_SKIN * MD_SKIN) atomicOr(&d.md_stale[r], 1);
     if (far_tl) atomicOr(&d.tcl_stale[r], 1);
+    if (margin && m2 > ph.margin * ph.margin) atomicOr(d.status, BW_FLAG_MARGIN);
 }
 // pair test of cpp/neighbour_list.cpp:33-34 (symmetric: img(-a) = -img(a))
 __device__ __forceinline__ bool md_pair(const Phys &ph, const Dev &d, double L, int g, int j) {
@@ -581,7 +619,7 @@ __device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R
         if (flags & PH_VELOCITY) colloid_velocity(ph, d, g);
         if (flags & PH_KICK) colloid_kick(ph, d, g);
         if (flags & PH_OLDF) colloid_oldf(d, g);
-        if (flags & PH_POS) colloid_pos(ph, d, R.L, r, g);
+        if (flags & PH_POS) colloid_pos(ph, d, R.L, r, g, (flags & PH_MARGIN) != 0);
     }
     __syncthreads();
     if (flags & PH_MDLIST)

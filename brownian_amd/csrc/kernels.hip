@@ -191,18 +191,28 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
 // -------------------------------------------------------------------- SRD
 // Destination tile T gathers the particles whose shifted cell lies in its box
 // (27 departure lists of the neighbour tiles, fixed order), sorts them by
-// shifted cell with an LDS counting sort, and per cell (particles in ascending
-// id, the reference's cell_part order) forms the FP64 mean velocity and the
-// thermostat factor of cpp/rotation_mpcd.cpp:14-49.  Output: the tile's
-// particles, rotated, in gather order, into the other buffer.
-template <class Real, bool COUNT_SHELL, bool MAP>
+// shifted cell with an LDS counting sort, ranks each cell's particles by id
+// (the reference's cell_part order), and per cell forms the FP64 mean velocity
+// and the thermostat factor of cpp/rotation_mpcd.cpp:14-49: the per-particle
+// terms are computed in parallel, each sum is taken serially in id order.
+// COUNT_SHELL: also counts the run() rear shells (cpp/run.cpp:17-18).
+// BLOCK (fused step): a particle that cannot meet a colloid during the whole
+// step -- farther from every colloid than its radius plus the particle's own
+// travel in the step plus the colloid displacement bound `margin` (checked
+// on every colloid position update of the step), and outside every run()
+// shell -- is streamed through all n_sub substeps here, in registers, with
+// the arithmetic of the streaming kernel.  The others ("near") are stored
+// first in the tile, and only they are touched by the per-substep kernels.
+template <class Real, bool COUNT_SHELL, bool MAP, bool BLOCK>
 __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real> src, Fluid<Real> dst) {
+    constexpr bool NEAR_TEST = COUNT_SHELL && BLOCK;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_pre[28], s_src[27], s_base[27];
     __shared__ int s_wtot[16];
     __shared__ TileColl tc;
-    __shared__ int s_shell[TCL_CAP];
-    __shared__ unsigned s_mask[COUNT_SHELL ? MASK_MAX : 1];
+    __shared__ NearColl nc[1];  // used by the near test only
+    __shared__ int s_shell[COUNT_SHELL ? TCL_CAP : 1];
+    __shared__ AxisMask s_am[COUNT_SHELL ? 2 : 1];  // [0] run shells, [1] near test
     const int T = blockIdx.x;
     Rep R;
     TileGeo tg = tile_geo(d, T, R);
@@ -210,16 +220,17 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     const int cap = ph.cap;
     const int B0 = R.B[0], B1 = R.B[1], B2 = R.B[2];
     const int ncell = B0 * B1 * B2;
+    const int ox0 = tg.tx * B0, oy0 = tg.ty * B1, oz0 = tg.tz * B2;  // tile origin (cells)
     // LDS carve-up
     double *s_u = (double *)smem;                       // 3 * ncell
     double *s_ax = s_u + 3 * ncell;                     // 3 * ncell  (rotation axis)
     double *s_sc = s_ax + 3 * ncell;                    // ncell      (thermostat factor)
-    Real *s_v = (Real *)(s_sc + ncell);                 // 3 * cap
-    int *s_pid = (int *)(s_v + 3 * cap);                // cap
-    int *s_start = s_pid + cap;                         // ncell + 1
+    Real *s_v = (Real *)(s_sc + ncell);                 // 3 * cap; later the per-particle energy terms
+    double *s_e2 = (double *)s_v;
+    int *s_pk = (int *)(s_v + 3 * cap);                 // cap: ids in cell-sorted (unranked) order
+    int *s_start = s_pk + cap;                          // ncell + 1
     int *s_fill = s_start + ncell + 1;                  // ncell
-    unsigned short *s_key = (unsigned short *)(s_fill + ncell);  // cap
-    unsigned short *s_ord = s_key + cap;                // cap
+    unsigned short *s_ord = (unsigned short *)(s_fill + ncell);  // cap: cell-sorted, id-ranked
 
     const long long step = *d.step;
     const double sx = d.shift[3 * tg.r], sy = d.shift[3 * tg.r + 1], sz = d.shift[3 * tg.r + 2];
@@ -249,7 +260,19 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     Region rg = tile_region(R, tg.tx, tg.ty, tg.tz);
     if (COUNT_SHELL) {
         load_tile_colloids(d, ph, R, tg.r, T, true, tc);
-        build_masks(tc, rg, L, true, s_mask);
+        build_axis_masks(tc, rg, L, true, 3, s_am[0]);
+        if (NEAR_TEST) {
+            build_axis_masks(tc, rg, L, false, ph.near_reach, s_am[COUNT_SHELL ? 1 : 0]);
+            const double Ld = (double)L;
+            for (int q = threadIdx.x; q < tc.n; q += blockDim.x) {
+                nc[0].cx[q] = (float)img1(tc.px[q] - ox0, Ld);
+                nc[0].cy[q] = (float)img1(tc.py[q] - oy0, Ld);
+                nc[0].cz[q] = (float)img1(tc.pz[q] - oz0, Ld);
+                nc[0].rx[q] = (float)img1(tc.rx[q] - ox0, Ld);
+                nc[0].ry[q] = (float)img1(tc.ry[q] - oy0, Ld);
+                nc[0].rz[q] = (float)img1(tc.rz[q] - oz0, Ld);
+            }
+        }
     }
     __syncthreads();
     int n = s_pre[27];
@@ -266,10 +289,14 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
         int g = threadIdx.x + k * SRD_THREADS;
         rkey[k] = -1;
         if (g < n) {
-            int t = 0;
-            while (t < 26 && s_pre[t + 1] <= g) t++;
-            int S = s_src[t];
-            int li = d.dep_idx[S * cap + s_base[t] + (g - s_pre[t])];
+            int lo = 0, hi = 26;  // the source list holding gather index g
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (s_pre[mid] <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            int S = s_src[lo];
+            int li = d.dep_idx[S * cap + s_base[lo] + (g - s_pre[lo])];
             int slot = S * cap + li;
             rcell[k] = src.cell[slot];
             rox[k] = src.ox[slot]; roy[k] = src.oy[slot]; roz[k] = src.oz[slot];
@@ -277,17 +304,15 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
             rpid[k] = src.pid[slot];
             int ix, iy, iz;
             unpack_cell(rcell[k], ix, iy, iz);
-            int lx = shifted_coord(ix, rox[k], sx, L) - tg.tx * B0;
-            int ly = shifted_coord(iy, roy[k], sy, L) - tg.ty * B1;
-            int lz = shifted_coord(iz, roz[k], sz, L) - tg.tz * B2;
+            int lx = shifted_coord(ix, rox[k], sx, L) - ox0;
+            int ly = shifted_coord(iy, roy[k], sy, L) - oy0;
+            int lz = shifted_coord(iz, roz[k], sz, L) - oz0;
             if (lx < 0 || lx >= B0 || ly < 0 || ly >= B1 || lz < 0 || lz >= B2) {
                 drift = true;
                 lx = max(0, min(B0 - 1, lx)); ly = max(0, min(B1 - 1, ly)); lz = max(0, min(B2 - 1, lz));
             }
             int key = lx + B0 * (ly + B1 * lz);
             rkey[k] = key;
-            s_key[g] = (unsigned short)key;
-            s_pid[g] = rpid[k];
             s_v[g] = rvx[k]; s_v[cap + g] = rvy[k]; s_v[2 * cap + g] = rvz[k];
             atomicAdd(&s_start[key], 1);
         }
@@ -301,81 +326,99 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
 #pragma unroll
     for (int k = 0; k < SRD_PT; k++) {
         int g = threadIdx.x + k * SRD_THREADS;
+        if (g < n) s_pk[atomicAdd(&s_fill[rkey[k]], 1)] = rpid[k];
+    }
+    __syncthreads();
+    // rank of each particle in its cell by id: the cell's canonical order
+#pragma unroll
+    for (int k = 0; k < SRD_PT; k++) {
+        int g = threadIdx.x + k * SRD_THREADS;
         if (g < n) {
-            int pos = atomicAdd(&s_fill[rkey[k]], 1);
-            s_ord[pos] = (unsigned short)g;
+            const int b = s_start[rkey[k]], e = s_start[rkey[k] + 1];
+            int rank = 0;
+            for (int j = b; j < e; j++) rank += s_pk[j] < rpid[k] ? 1 : 0;
+            s_ord[b + rank] = (unsigned short)g;
         }
     }
     __syncthreads();
-    // per cell: canonical (ascending id) order, FP64 sums, axis, thermostat
-    bool zero_var = false, missing = false;
-    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    // per cell: FP64 mean in id order (one thread per component) and the
+    // rotation axis (a fourth thread)
+    bool missing = false;
+    for (int w = threadIdx.x; w < 4 * ncell; w += blockDim.x) {
+        const int comp = w / ncell, c = w - comp * ncell;
         const int b = s_start[c], e = s_start[c + 1], N = e - b;
-        for (int i = b + 1; i < e; i++) {
-            unsigned short gi = s_ord[i];
-            int pi = s_pid[gi];
-            int j = i - 1;
-            while (j >= b && s_pid[s_ord[j]] > pi) {
-                s_ord[j + 1] = s_ord[j];
-                j--;
-            }
-            s_ord[j + 1] = gi;
-        }
         if (N > 1) {
-            double ux = 0, uy = 0, uz = 0;
-            for (int i = b; i < e; i++) {
-                int gi = s_ord[i];
-                ux = ux + (double)s_v[gi] / N;
-                uy = uy + (double)s_v[cap + gi] / N;
-                uz = uz + (double)s_v[2 * cap + gi] / N;
+            if (comp < 3) {
+                const Real *sv = s_v + comp * cap;
+                double u = 0;
+                for (int i = b; i < e; i++) u = u + (double)sv[s_ord[i]] / N;
+                s_u[3 * c + comp] = u;
+            } else {
+                int kx = ox0 + c % B0, ky = oy0 + (c / B0) % B1, kz = oz0 + c / (B0 * B1);
+                double u1, u2;
+                if (!draw_axis_raw(d, ph, R, step, linear_cell(kx, ky, kz, L), u1, u2)) {
+                    missing = true;
+                    u1 = 0.5; u2 = 0.0;
+                }
+                // cpp/rotation_mpcd.cpp:19-25
+                double rho = 2 * u1 - 1;
+                double phi = 4.0 * asin(1.0) * u2;
+                s_ax[3 * c] = cos(phi) * sqrt(1 - rho * rho);
+                s_ax[3 * c + 1] = sin(phi) * sqrt(1 - rho * rho);
+                s_ax[3 * c + 2] = rho;
             }
-            int kx = tg.tx * B0 + c % B0, ky = tg.ty * B1 + (c / B0) % B1, kz = tg.tz * B2 + c / (B0 * B1);
-            int gc = linear_cell(kx, ky, kz, L);
-            double u1, u2;
-            if (!draw_axis_raw(d, ph, R, step, gc, u1, u2)) {
-                missing = true;
-                u1 = 0.5; u2 = 0.0;
-            }
-            // cpp/rotation_mpcd.cpp:19-25
-            double rho = 2 * u1 - 1;
-            double phi = 4.0 * asin(1.0) * u2;
-            double r1 = cos(phi) * sqrt(1 - rho * rho), r2 = sin(phi) * sqrt(1 - rho * rho), r3 = rho;
-            double ir1 = ph.ict * r1, ir2 = ph.ict * r2, ir3 = ph.ict * r3;
-            double a11 = ir1 * r1 + ph.ct, a12 = ir1 * r2 - ph.st * r3, a13 = ir1 * r3 + ph.st * r2;
-            double a21 = ir2 * r1 + ph.st * r3, a22 = ir2 * r2 + ph.ct, a23 = ir2 * r3 - ph.st * r1;
-            double a31 = ir3 * r1 - ph.st * r2, a32 = ir3 * r2 + ph.st * r1, a33 = ir3 * r3 + ph.ct;
-            // thermostat variance from the rotated velocities (:37-41)
-            double var = 0.0;
-            for (int i = b; i < e; i++) {
-                int gi = s_ord[i];
-                double dx = (double)s_v[gi] - ux, dy = (double)s_v[cap + gi] - uy, dz = (double)s_v[2 * cap + gi] - uz;
+        }
+    }
+    if (missing) atomicOr(d.status, BW_FLAG_DRAW_MISSING);
+    __syncthreads();  // the velocities in LDS are no longer needed: energy terms overwrite them
+    // per particle: rotated velocity relative to the mean, its squared norm (:37-41)
+#pragma unroll
+    for (int k = 0; k < SRD_PT; k++) {
+        int g = threadIdx.x + k * SRD_THREADS;
+        if (g < n) {
+            const int c = rkey[k];
+            if (s_start[c + 1] - s_start[c] > 1) {
+                double ux = s_u[3 * c], uy = s_u[3 * c + 1], uz = s_u[3 * c + 2];
+                double r1 = s_ax[3 * c], r2 = s_ax[3 * c + 1], r3 = s_ax[3 * c + 2];
+                double ir1 = ph.ict * r1, ir2 = ph.ict * r2, ir3 = ph.ict * r3;
+                double a11 = ir1 * r1 + ph.ct, a12 = ir1 * r2 - ph.st * r3, a13 = ir1 * r3 + ph.st * r2;
+                double a21 = ir2 * r1 + ph.st * r3, a22 = ir2 * r2 + ph.ct, a23 = ir2 * r3 - ph.st * r1;
+                double a31 = ir3 * r1 - ph.st * r2, a32 = ir3 * r2 + ph.st * r1, a33 = ir3 * r3 + ph.ct;
+                double dx = (double)rvx[k] - ux, dy = (double)rvy[k] - uy, dz = (double)rvz[k] - uz;
                 double v1x = ux + ((a11 * dx + a12 * dy) + a13 * dz);
                 double v1y = uy + ((a21 * dx + a22 * dy) + a23 * dz);
                 double v1z = uz + ((a31 * dx + a32 * dy) + a33 * dz);
                 double ex = v1x - ux, ey = v1y - uy, ez = v1z - uz;
-                var += (ex * ex + ey * ey) + ez * ez;
+                s_e2[g] = (ex * ex + ey * ey) + ez * ez;
             }
+        }
+    }
+    __syncthreads();
+    // per cell: variance in id order, thermostat factor
+    bool zero_var = false;
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        const int b = s_start[c], e = s_start[c + 1], N = e - b;
+        if (N > 1) {
+            double var = 0.0;
+            for (int i = b; i < e; i++) var += s_e2[s_ord[i]];
             double scale = sqrt(3.0 * (N - 1) * ph.kbt / (ph.mass_fl * var));
             if (!(var > 0)) {
                 zero_var = true;
                 scale = 1.0;
             }
-            s_u[3 * c] = ux; s_u[3 * c + 1] = uy; s_u[3 * c + 2] = uz;
-            s_ax[3 * c] = r1; s_ax[3 * c + 1] = r2; s_ax[3 * c + 2] = r3;
             s_sc[c] = scale;
         }
     }
     if (zero_var) atomicOr(d.status, BW_FLAG_ZERO_VARIANCE);
-    if (missing) atomicOr(d.status, BW_FLAG_DRAW_MISSING);
     __syncthreads();
-    // apply, count run() shells, write out (coalesced, gather order)
+    // apply; count run() shells; near test
+    unsigned nearbits = 0;
 #pragma unroll
     for (int k = 0; k < SRD_PT; k++) {
         int g = threadIdx.x + k * SRD_THREADS;
         if (g < n) {
             int c = rkey[k];
             int N = s_start[c + 1] - s_start[c];
-            Real vx = rvx[k], vy = rvy[k], vz = rvz[k];
             if (N > 1) {
                 double ux = s_u[3 * c], uy = s_u[3 * c + 1], uz = s_u[3 * c + 2];
                 double r1 = s_ax[3 * c], r2 = s_ax[3 * c + 1], r3 = s_ax[3 * c + 2];
@@ -383,37 +426,124 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
                 double a11 = ir1 * r1 + ph.ct, a12 = ir1 * r2 - ph.st * r3, a13 = ir1 * r3 + ph.st * r2;
                 double a21 = ir2 * r1 + ph.st * r3, a22 = ir2 * r2 + ph.ct, a23 = ir2 * r3 - ph.st * r1;
                 double a31 = ir3 * r1 - ph.st * r2, a32 = ir3 * r2 + ph.st * r1, a33 = ir3 * r3 + ph.ct;
-                double dx = (double)vx - ux, dy = (double)vy - uy, dz = (double)vz - uz;
+                double dx = (double)rvx[k] - ux, dy = (double)rvy[k] - uy, dz = (double)rvz[k] - uz;
                 double v1x = ux + ((a11 * dx + a12 * dy) + a13 * dz);
                 double v1y = uy + ((a21 * dx + a22 * dy) + a23 * dz);
                 double v1z = uz + ((a31 * dx + a32 * dy) + a33 * dz);
                 double sc = s_sc[c];
-                vx = (Real)(ux + (v1x - ux) * sc);
-                vy = (Real)(uy + (v1y - uy) * sc);
-                vz = (Real)(uz + (v1z - uz) * sc);
+                rvx[k] = (Real)(ux + (v1x - ux) * sc);
+                rvy[k] = (Real)(uy + (v1y - uy) * sc);
+                rvz[k] = (Real)(uz + (v1z - uz) * sc);
             }
+            bool near = !BLOCK;
             if (COUNT_SHELL) {
                 int ix, iy, iz;
                 unpack_cell(rcell[k], ix, iy, iz);
-                int ri = region_index(rg, ix, iy, iz, L);
-                unsigned m = ri >= 0 ? s_mask[ri] : 0xffffffffu;
-                double x = (double)ix + (double)rox[k], y = (double)iy + (double)roy[k], z = (double)iz + (double)roz[k];
+                const double x = (double)ix + (double)rox[k], y = (double)iy + (double)roy[k],
+                             z = (double)iz + (double)roz[k];
+                unsigned m = cell_mask(s_am[0], rg, ix, iy, iz, L);
                 for_candidates(tc, m, [&](int q) {
                     if (q < tc.n && shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
                 });
+                if (NEAR_TEST) {
+                    // conservative test in FP32, tile-local coordinates (2e-2 cell slack)
+                    const float Lf = (float)L, iLf = 1.0f / Lf;
+                    const float px = (float)pdiff(ix, ox0, L) + (float)rox[k];
+                    const float py = (float)pdiff(iy, oy0, L) + (float)roy[k];
+                    const float pz = (float)pdiff(iz, oz0, L) + (float)roz[k];
+                    const float vx = (float)rvx[k], vy = (float)rvy[k], vz = (float)rvz[k];
+                    const float tr = sqrtf((vx * vx + vy * vy) + vz * vz) * (float)ph.ndt;
+                    const bool inside01 = rox[k] >= Real(0) && rox[k] < Real(1) && roy[k] >= Real(0) &&
+                                          roy[k] < Real(1) && roz[k] >= Real(0) && roz[k] < Real(1);
+                    near = !(tr <= (float)ph.vcap_ndt) || !inside01;
+                    const float rs = sqrtf((float)ph.near_shell2) + 0.02f, rs2 = rs * rs;
+                    const float rc = (float)ph.near_base + tr + 0.02f, rc2 = rc * rc;
+                    auto d2 = [&](float cx, float cy, float cz) {
+                        float ax = px - cx, ay = py - cy, az = pz - cz;
+                        ax -= Lf * rintf(ax * iLf); ay -= Lf * rintf(ay * iLf); az -= Lf * rintf(az * iLf);
+                        return (ax * ax + ay * ay) + az * az;
+                    };
+                    if (!near)  // within reach of a run() kick
+                        for_candidates(tc, m, [&](int q) {
+                            if (q < tc.n && !near) near = d2(nc[0].rx[q], nc[0].ry[q], nc[0].rz[q]) <= rs2;
+                        });
+                    if (!near) {  // within reach of a collision during the step
+                        unsigned m2 = cell_mask(s_am[COUNT_SHELL ? 1 : 0], rg, ix, iy, iz, L);
+                        for_candidates(tc, m2, [&](int q) {
+                            if (q < tc.n && !near) near = d2(nc[0].cx[q], nc[0].cy[q], nc[0].cz[q]) <= rc2;
+                        });
+                    }
+                }
             }
-            int o = T * cap + g;
-            dst.cell[o] = rcell[k];
-            dst.ox[o] = rox[k]; dst.oy[o] = roy[k]; dst.oz[o] = roz[k];
-            dst.vx[o] = vx; dst.vy[o] = vy; dst.vz[o] = vz;
+            if (near) nearbits |= 1u << k;
+        }
+    }
+    // output slots: near particles first, each group in gather order
+    int nrk[SRD_PT];  // near particles of lower gather index
+    int n_near = n;
+    if (BLOCK) {
+        int base = 0;
+#pragma unroll
+        for (int k = 0; k < SRD_PT; k++) {
+            int g = threadIdx.x + k * SRD_THREADS;
+            int tot;
+            int idx = block_compact(g < n && (nearbits & (1u << k)), s_wtot, tot);
+            nrk[k] = base + idx;
+            base += tot;
+        }
+        n_near = base;
+    }
+    bool far_drift = false;
+#pragma unroll
+    for (int k = 0; k < SRD_PT; k++) {
+        int g = threadIdx.x + k * SRD_THREADS;
+        if (g < n) {
+            int slot_in_tile = g;
+            int cell = rcell[k];
+            Real ox = rox[k], oy = roy[k], oz = roz[k];
+            if (BLOCK) {
+                if (nearbits & (1u << k)) {
+                    slot_in_tile = nrk[k];
+                } else {
+                    slot_in_tile = n_near + g - nrk[k];
+                    // every substep of the step as the streaming kernel does it;
+                    // the offsets start in [0,1) and move by less than a cell
+                    int ix, iy, iz;
+                    unpack_cell(cell, ix, iy, iz);
+                    const Real dt = (Real)ph.dt;
+                    const Real dx = rvx[k] * dt, dy = rvy[k] * dt, dz = rvz[k] * dt;
+                    for (int l = 0; l < ph.n_sub; l++) {
+                        ox = ox + dx;
+                        oy = oy + dy;
+                        oz = oz + dz;
+                        step_cell(ix, ox, L);
+                        step_cell(iy, oy, L);
+                        step_cell(iz, oz, L);
+                    }
+                    cell = pack_cell(ix, iy, iz);
+                    if (!(near_interval(ix, ox0 - TILE_REACH, B0 + 2 * TILE_REACH, L) &&
+                          near_interval(iy, oy0 - TILE_REACH, B1 + 2 * TILE_REACH, L) &&
+                          near_interval(iz, oz0 - TILE_REACH, B2 + 2 * TILE_REACH, L)))
+                        far_drift = true;
+                }
+            }
+            int o = T * cap + slot_in_tile;
+            dst.cell[o] = cell;
+            dst.ox[o] = ox; dst.oy[o] = oy; dst.oz[o] = oz;
+            dst.vx[o] = rvx[k]; dst.vy[o] = rvy[k]; dst.vz[o] = rvz[k];
             dst.pid[o] = rpid[k];
             if (MAP) {
-                int kx = tg.tx * B0 + c % B0, ky = tg.ty * B1 + (c / B0) % B1, kz = tg.tz * B2 + c / (B0 * B1);
+                int c = rkey[k];
+                int kx = ox0 + c % B0, ky = oy0 + (c / B0) % B1, kz = oz0 + c / (B0 * B1);
                 d.shifted_cell[R.first_fluid + rpid[k]] = linear_cell(kx, ky, kz, L);
             }
         }
     }
-    if (threadIdx.x == 0) d.count_next[T] = n;
+    if (far_drift) atomicOr(d.status, BW_FLAG_DRIFT);
+    if (threadIdx.x == 0) {
+        d.count_next[T] = n;
+        d.near_cnt[T] = n_near;
+    }
     if (COUNT_SHELL) {
         __syncthreads();
         for (int q = threadIdx.x; q < tc.n; q += blockDim.x)
@@ -433,7 +563,7 @@ struct StreamOp {
     Fluid<Real> &f;
     const TileColl &tc;
     const Region &rg;
-    const unsigned *mdet, *mkick;
+    const AxisMask *mdet, *mkick;
     int L, r, lox, loy, loz, lnx, lny, lnz;
     bool drift, ch_over;
 
@@ -442,8 +572,7 @@ struct StreamOp {
         unpack_cell(c, ix, iy, iz);
         const double Ld = (double)L;
         if (KICK) {
-            int ri = region_index(rg, ix, iy, iz, L);
-            unsigned m = ri >= 0 ? mkick[ri] : 0xffffffffu;
+            unsigned m = cell_mask(*mkick, rg, ix, iy, iz, L);
             if (m || tc.n > MASK_BITS) {
                 double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
                 double wx = vx, wy = vy, wz = vz;
@@ -476,8 +605,7 @@ struct StreamOp {
                 drift = true;
         }
         if (DETECT) {
-            int ri = region_index(rg, ix, iy, iz, L);
-            unsigned m = ri >= 0 ? mdet[ri] : 0xffffffffu;
+            unsigned m = cell_mask(*mdet, rg, ix, iy, iz, L);
             if (m == 0 && tc.n <= MASK_BITS) return;
             double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
             int n_in = 0, gin = -1;
@@ -503,21 +631,23 @@ struct StreamOp {
     }
 };
 
+// near_only: the fused step's per-substep pass over the particles stored
+// first in each tile by the SRD kernel; otherwise every particle.
 template <class Real, bool KICK, bool STREAM, bool DETECT>
-__global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid<Real> f) {
+__global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid<Real> f, int near_only) {
     __shared__ TileColl tc;
-    __shared__ unsigned s_mask[(KICK ? 1 : 0) + (DETECT ? 1 : 0) + ((KICK || DETECT) ? 0 : 1)][MASK_MAX];
+    __shared__ AxisMask s_am[(KICK && DETECT) ? 2 : 1];
     const int T = blockIdx.x;
     Rep R;
     TileGeo tg = tile_geo(d, T, R);
     const int L = tg.L;
-    const int n = min(d.count[T], ph.cap);
+    const int n = min(near_only ? d.near_cnt[T] : d.count[T], ph.cap);
     Region rg = tile_region(R, tg.tx, tg.ty, tg.tz);
-    unsigned *mkick = s_mask[0], *mdet = s_mask[(KICK && DETECT) ? 1 : 0];
+    const AxisMask *mkick = &s_am[0], *mdet = &s_am[(KICK && DETECT) ? 1 : 0];
     if (KICK || DETECT) {
         load_tile_colloids(d, ph, R, tg.r, T, KICK, tc);
-        if (KICK) build_masks(tc, rg, L, true, mkick);
-        if (DETECT) build_masks(tc, rg, L, false, mdet);
+        if (KICK) build_axis_masks(tc, rg, L, true, 3, s_am[0]);
+        if (DETECT) build_axis_masks(tc, rg, L, false, 3, s_am[(KICK && DETECT) ? 1 : 0]);
     }
     StreamOp<Real, KICK, STREAM, DETECT> op{ph, d, f, tc, rg, mdet, mkick, L, tg.r,
                                             tg.tx * R.B[0] - TILE_REACH, tg.ty * R.B[1] - TILE_REACH,
@@ -859,7 +989,7 @@ __global__ __launch_bounds__(256) void k_colloid_update(Phys ph, Dev d, unsigned
         if (flags & PH_VELOCITY) colloid_velocity(ph, d, g);
         if (flags & PH_KICK) colloid_kick(ph, d, g);
         if (flags & PH_OLDF) colloid_oldf(d, g);
-        if (flags & PH_POS) colloid_pos(ph, d, R.L, r, g);
+        if (flags & PH_POS) colloid_pos(ph, d, R.L, r, g, (flags & PH_MARGIN) != 0);
     }
 }
 
@@ -868,7 +998,7 @@ template <class Real>
 __global__ __launch_bounds__(STREAM_THREADS) void k_run_count(Phys ph, Dev d, Fluid<Real> f) {
     __shared__ TileColl tc;
     __shared__ int s_shell[TCL_CAP];
-    __shared__ unsigned s_mask[MASK_MAX];
+    __shared__ AxisMask s_am;
     const int T = blockIdx.x;
     Rep R;
     TileGeo tg = tile_geo(d, T, R);
@@ -877,13 +1007,12 @@ __global__ __launch_bounds__(STREAM_THREADS) void k_run_count(Phys ph, Dev d, Fl
     for (int k = threadIdx.x; k < TCL_CAP; k += blockDim.x) s_shell[k] = 0;
     Region rg = tile_region(R, tg.tx, tg.ty, tg.tz);
     load_tile_colloids(d, ph, R, tg.r, T, true, tc);
-    build_masks(tc, rg, L, true, s_mask);
+    build_axis_masks(tc, rg, L, true, 3, s_am);
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int slot = T * ph.cap + i;
         int ix, iy, iz;
         unpack_cell(f.cell[slot], ix, iy, iz);
-        int ri = region_index(rg, ix, iy, iz, L);
-        unsigned m = ri >= 0 ? s_mask[ri] : 0xffffffffu;
+        unsigned m = cell_mask(s_am, rg, ix, iy, iz, L);
         double x = (double)ix + (double)f.ox[slot], y = (double)iy + (double)f.oy[slot],
                z = (double)iz + (double)f.oz[slot];
         for_candidates(tc, m, [&](int q) {
@@ -1057,46 +1186,50 @@ __global__ void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, lon
 size_t srd_lds_bytes(int ncell, int cap, bool fp64) {
     size_t rb = fp64 ? 8 : 4;
     return (size_t)ncell * 7 * 8 + 3 * (size_t)cap * rb + (size_t)cap * 4 + (size_t)(2 * ncell + 1) * 4 +
-           (size_t)cap * 4 + 16;
+           (size_t)cap * 2 + 16;
 }
-// static LDS of the SRD kernel's shell-counting variant (colloid list, masks)
-size_t srd_static_lds() { return sizeof(TileColl) + 4 * TCL_CAP + 4 * MASK_MAX + 512; }
+// static LDS of the SRD kernel's largest variant (colloid lists, masks)
+size_t srd_static_lds() { return sizeof(TileColl) + sizeof(NearColl) + 4 * TCL_CAP + 2 * sizeof(AxisMask) + 512; }
 
 template <class Real>
 void launch_depart(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream_t st) {
     LAUNCH(k_depart<Real>, ph.n_tiles, DEPART_THREADS, 0, st, ph, d, f);
 }
-template <class Real>
-void launch_srd(const Phys &ph, const Dev &d, const Fluid<Real> &src, const Fluid<Real> &dst, bool count_shell,
-                bool map, size_t lds, hipStream_t st) {
+template <class Real, bool C, bool M, bool B>
+static void srd_one(const Phys &ph, const Dev &d, const Fluid<Real> &src, const Fluid<Real> &dst, size_t lds,
+                    hipStream_t st) {
     static bool attr = false;
     if (!attr) {
         int bytes = 160 * 1024 - (int)srd_static_lds();
-        (void)hipFuncSetAttribute((const void *)k_srd<Real, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void *)k_srd<Real, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void *)k_srd<Real, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute((const void *)k_srd<Real, false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void *)k_srd<Real, C, M, B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         attr = true;
     }
-    if (count_shell) {
-        if (map) LAUNCH((k_srd<Real, true, true>), ph.n_tiles, SRD_THREADS, lds, st, ph, d, src, dst);
-        else LAUNCH((k_srd<Real, true, false>), ph.n_tiles, SRD_THREADS, lds, st, ph, d, src, dst);
-    } else {
-        if (map) LAUNCH((k_srd<Real, false, true>), ph.n_tiles, SRD_THREADS, lds, st, ph, d, src, dst);
-        else LAUNCH((k_srd<Real, false, false>), ph.n_tiles, SRD_THREADS, lds, st, ph, d, src, dst);
-    }
+    LAUNCH((k_srd<Real, C, M, B>), ph.n_tiles, SRD_THREADS, lds, st, ph, d, src, dst);
+}
+// count_shell: run() shell counts (colloids present); map: shifted-cell map of
+// every particle (the standalone rotation_mpcd); block: near/far split of the
+// fused step
+template <class Real>
+void launch_srd(const Phys &ph, const Dev &d, const Fluid<Real> &src, const Fluid<Real> &dst, bool count_shell,
+                bool map, bool block, size_t lds, hipStream_t st) {
+    if (map) srd_one<Real, false, true, false>(ph, d, src, dst, lds, st);
+    else if (count_shell && block) srd_one<Real, true, false, true>(ph, d, src, dst, lds, st);
+    else if (count_shell) srd_one<Real, true, false, false>(ph, d, src, dst, lds, st);
+    else if (block) srd_one<Real, false, false, true>(ph, d, src, dst, lds, st);
+    else srd_one<Real, false, false, false>(ph, d, src, dst, lds, st);
 }
 template <class Real>
 void launch_stream(const Phys &ph, const Dev &d, const Fluid<Real> &f, bool kick, bool stream, bool detect,
-                   hipStream_t st) {
+                   bool near_only, hipStream_t st) {
     int m = (kick ? 4 : 0) | (stream ? 2 : 0) | (detect ? 1 : 0);
+    int nf = near_only ? 1 : 0;
     switch (m) {
-        case 7: LAUNCH((k_stream<Real, true, true, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
-        case 6: LAUNCH((k_stream<Real, true, true, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
-        case 4: LAUNCH((k_stream<Real, true, false, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
-        case 3: LAUNCH((k_stream<Real, false, true, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
-        case 2: LAUNCH((k_stream<Real, false, true, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
-        case 1: LAUNCH((k_stream<Real, false, false, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f); break;
+        case 7: LAUNCH((k_stream<Real, true, true, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
+        case 6: LAUNCH((k_stream<Real, true, true, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
+        case 4: LAUNCH((k_stream<Real, true, false, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
+        case 3: LAUNCH((k_stream<Real, false, true, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
+        case 2: LAUNCH((k_stream<Real, false, true, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
+        case 1: LAUNCH((k_stream<Real, false, false, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
         default: break;
     }
 }
@@ -1153,8 +1286,8 @@ void launch_copy(const void *a, void *b, long long nbytes, hipStream_t st) {
 #define INST(Real)                                                                                                 \
     template void launch_depart<Real>(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);               \
     template void launch_srd<Real>(const Phys &, const Dev &, const Fluid<Real> &, const Fluid<Real> &, bool,    \
-                                   bool, size_t, hipStream_t);                                                     \
-    template void launch_stream<Real>(const Phys &, const Dev &, const Fluid<Real> &, bool, bool, bool,           \
+                                   bool, bool, size_t, hipStream_t);                                               \
+    template void launch_stream<Real>(const Phys &, const Dev &, const Fluid<Real> &, bool, bool, bool, bool,     \
                                       hipStream_t);                                                                \
     template void launch_collide<Real>(const Phys &, const Dev &, const Fluid<Real> &, int, const int2 *, int,    \
                                        hipStream_t);                                                               \

@@ -42,11 +42,17 @@ extern "C" {
 #define BW_FLAG_DRAW_MISMATCH 0x40u    /* injected reflection list differs from events    */
 #define BW_FLAG_ZERO_VARIANCE 0x80u    /* thermostat on a cell with zero relative KE      */
 #define BW_FLAG_NEIGH_OVERFLOW 0x100u  /* neigh_fl capacity exceeded                      */
+#define BW_FLAG_MARGIN 0x200u          /* a colloid moved beyond the blocked-streaming margin
+                                          within one step (use BW_STREAM_PER_SUBSTEP)    */
 
 enum bw_precision { BW_FP32 = 0, BW_FP64 = 1 };
 enum bw_vel_law { BW_VEL_REFERENCE = 0, BW_VEL_MAXWELL = 1 };
 enum bw_orient_law { BW_ORIENT_REFERENCE = 0, BW_ORIENT_SPHERE = 1 };
 enum bw_draw_source { BW_DRAWS_DEVICE = 0, BW_DRAWS_INJECTED = 1 };
+/* Fused-step streaming.  BLOCKED: particles that cannot reach a colloid within
+ * the step are streamed through all substeps by the rotation pass (same
+ * arithmetic, same results); PER_SUBSTEP: every particle every substep. */
+enum bw_streaming { BW_STREAM_BLOCKED = 0, BW_STREAM_PER_SUBSTEP = 1 };
 
 /* Run parameters.  Mirrors the reference's global defaults
  * (cpp/active_brownian.cpp:3-7) with every value exposed; see
@@ -71,7 +77,8 @@ typedef struct bw_params {
     int orient_law;      /* initial orientation law                                  */
     int draws;           /* BW_DRAWS_DEVICE (counter-based) or BW_DRAWS_INJECTED     */
     int tile[3];         /* tile shape in cells; 0 = choose                          */
-    int reserved[8];
+    int streaming;       /* bw_streaming of the fused step                           */
+    int reserved[7];
 } bw_params;
 
 typedef struct bw_ctx bw_ctx;

@@ -145,7 +145,9 @@ def test_c0like_fused_vs_reference():
 def test_overlapping_colloids_chain_particles():
     """Fluid inside two colloids: the serial per-colloid replay."""
     g, p = load("overlap")
-    s = B.System(**sys_params(p))
+    # the colloids here spin up far beyond any physical speed within the step,
+    # past the displacement bound of blocked streaming: stream every substep
+    s = B.System(**sys_params(p, streaming=B.BW_STREAM_PER_SUBSTEP))
     s.startup()
     o = O.Oracle(**p)
     o.startup()
@@ -195,6 +197,57 @@ def test_fused_step_equals_entry_point_sequence(precision):
     da, db = a.download_colloid(), b.download_colloid()
     for k in da:
         np.testing.assert_array_equal(da[k], db[k], err_msg=k)
+
+
+@pytest.mark.parametrize("precision", [B.BW_FP32, B.BW_FP64])
+def test_blocked_streaming_equals_per_substep(precision):
+    """Particles out of every colloid's reach streamed by the rotation pass
+    through all substeps == every particle streamed every substep (bit-exact)."""
+    prm = dict(L=16, density=10, n_colloid=8, sig_colloid=4.2, sigma=4.0, mass_colloid=50.0, n_sub=20,
+               v0=0.05, theta_deg=90.0, seed=2468, precision=precision, draws=B.BW_DRAWS_DEVICE,
+               vel_law=B.BW_VEL_MAXWELL, orient_law=B.BW_ORIENT_SPHERE)
+    a = B.System(streaming=B.BW_STREAM_BLOCKED, **prm)
+    b = B.System(streaming=B.BW_STREAM_PER_SUBSTEP, **prm)
+    a.startup(); b.startup()
+    a.step(3); b.step(3)
+    assert a.status() == 0 and b.status() == 0
+    pa, va, ca = a.download_fluid()
+    pb, vb, cb = b.download_fluid()
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(va, vb)
+    np.testing.assert_array_equal(ca, cb)
+    da, db = a.download_colloid(), b.download_colloid()
+    for k in da:
+        np.testing.assert_array_equal(da[k], db[k], err_msg=k)
+    # no colloids: the rotation pass does all of the streaming
+    prm0 = dict(prm, n_colloid=0)
+    a0 = B.System(streaming=B.BW_STREAM_BLOCKED, **prm0)
+    b0 = B.System(streaming=B.BW_STREAM_PER_SUBSTEP, **prm0)
+    a0.startup(); b0.startup()
+    a0.step(2); b0.step(2)
+    assert a0.status() == 0 and b0.status() == 0
+    pa, va, _ = a0.download_fluid()
+    pb, vb, _ = b0.download_fluid()
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(va, vb)
+
+
+def test_blocked_streaming_margin_violation_is_reported():
+    """A colloid driven (from the host) past the displacement bound within a
+    step raises BW_ERR_PHYSICS with the margin flag in blocked mode only."""
+    prm = dict(L=12, density=5, n_colloid=2, sig_colloid=2.5, sigma=2.0, mass_colloid=50.0, n_sub=10,
+               v0=0.0, theta_deg=90.0, seed=1357, precision=B.BW_FP64, draws=B.BW_DRAWS_DEVICE)
+    flags = {}
+    for mode in (B.BW_STREAM_BLOCKED, B.BW_STREAM_PER_SUBSTEP):
+        s = B.System(streaming=mode, **prm)
+        s.startup()
+        c = s.download_colloid()
+        c["vel"][0] = [6.0, 0.0, 0.0]  # 0.06 cells per substep: 0.25 passed within the step
+        s.upload_colloid(vel=c["vel"])
+        s.step(1)
+        flags[mode] = s.status(raise_on_error=False)
+    assert flags[B.BW_STREAM_BLOCKED] & 0x200
+    assert not flags[B.BW_STREAM_PER_SUBSTEP] & 0x200
 
 
 def test_fp32_per_step_vs_oracle():
