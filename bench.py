@@ -9,14 +9,20 @@ FP64 per-cell sums; Maxwell-Boltzmann velocities at kT = 1, orientations
 uniform on the sphere.  Synthetic initial state from a fixed seed.
 
 One line of JSON: fluid particle-steps/s of the fused, graph-replayed step
-(inputs resident in HBM), the HBM roofline of the dominant kernel (streaming)
-from HIP events, and the reference cpp/ program (compiled from its sources
-into oracle/_ref) timed on one host core on a bounded sample of the same state
+(inputs resident in HBM); the HBM roofline of the step's dominant kernel from
+HIP events on the launch stream (algorithmic bytes per launch / average launch
+time); and the reference cpp/ program (compiled from its sources into
+oracle/_ref) timed on one host core on a bounded sample of the same state
 (rotation + run() + the first substeps, extrapolated to one MPCD step).
+
+  python bench.py [--steps K --warmup W --L 128 --ncol 2400]
+  python bench.py --pmc-summary FETCH_DIR WRITE_DIR --out FILE   (rocprofv3 --pmc outputs -> bytes per launch)
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
 import sys
@@ -28,10 +34,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12        # MI355X HBM3E, bytes/s
-STREAM_BYTES = 44        # per particle per substep: R pos 16 + R vel 12 + W pos 16
-SRD_BYTES = 40           # per particle per step:    R pos 16 + R vel 12 + W vel 12
 METRIC = "fluid particle-steps/s & %HBM peak; ensemble+surrogate vs plain at matched MSD"
-PMC_FILE = os.path.join(ROOT, "profiles", "stream_kernel_c2_pmc.json")
+COPY_BYTES = 1 << 30     # calibration copy: reads and writes this many bytes
+# Algorithmic HBM bytes per particle and launch (FP32 storage: packed cell 4 B,
+# offsets 3x4 B, velocities 3x4 B, id 4 B):
+#   srd     read the state 32 + write it 32 (the tile re-sort moves every field)
+#   stream  read cell+offsets 16 + velocities 12, write cell+offsets 16 (near particles)
+#   depart  read cell+offsets 16, write the departure index 4
+ALG_BYTES = {"srd": 64, "stream": 44, "depart": 20}
+PMC_FILE = os.path.join(ROOT, "profiles", "srd_kernel_128cubed_pmc.json")
 
 
 def log(msg):
@@ -105,6 +116,56 @@ def cpu_worker(args):
     print(json.dumps(dict(step_s=step_s, rot_run_s=t_rot, substep_s=subs)))
 
 
+# ------------------------------------------------------------- PMC summary
+def _pmc_rows(d):
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def _per_kernel(rows, counter):
+    """Mean over dispatches of a per-dispatch counter (summed over its
+    dimensions), keyed by the kernel's base name (k_srd, k_stream, ...)."""
+    per = {}
+    for r in rows:
+        if r.get("Counter_Name") != counter:
+            continue
+        name = r.get("Kernel_Name", "")
+        base = next((k for k in ("k_srd", "k_stream", "k_depart", "k_collide", "k_colloid_update", "k_copy",
+                                 "k_md_candidates", "k_tile_colloids") if k in name), None)
+        if base is None:
+            continue
+        key = (base, r.get("Dispatch_Id") or r.get("Correlation_Id"))
+        per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+    out = {}
+    for (base, _), v in per.items():
+        out.setdefault(base, []).append(v)
+    return {k: (float(np.mean(v)), len(v)) for k, v in out.items()}
+
+
+def pmc_summary(fetch_dir, write_dir, out_file, command):
+    fetch = _per_kernel(_pmc_rows(fetch_dir), "FETCH_SIZE")
+    write = _per_kernel(_pmc_rows(write_dir), "WRITE_SIZE")
+    # FETCH_SIZE / WRITE_SIZE are KiB per dispatch; the plain copy of
+    # COPY_BYTES gives the factor from counter to bytes
+    f_fac = COPY_BYTES / (fetch["k_copy"][0] * 1024.0)
+    w_fac = COPY_BYTES / (write["k_copy"][0] * 1024.0)
+    res = {"counters": "FETCH_SIZE, WRITE_SIZE (KiB per dispatch), one rocprofv3 --pmc pass each",
+           "copy_kernel_bytes": COPY_BYTES, "fetch_factor": f_fac, "write_factor": w_fac,
+           "command": command, "kernels": {}}
+    for k in sorted(set(fetch) & set(write)):
+        fb = fetch[k][0] * 1024.0 * f_fac
+        wb = write[k][0] * 1024.0 * w_fac
+        res["kernels"][k] = {"fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb,
+                             "bytes_per_launch": fb + wb, "dispatches": fetch[k][1]}
+    with open(out_file, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(json.dumps(res, indent=1))
+
+
+# ------------------------------------------------------------------- bench
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,11 +176,19 @@ def main():
     ap.add_argument("--ncol", type=int, default=2400)
     ap.add_argument("--nsub", type=int, default=20)
     ap.add_argument("--seed", type=int, default=20240)
+    ap.add_argument("--streaming", choices=["blocked", "per-substep"], default="blocked")
+    ap.add_argument("--copy-reps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--out")
+    ap.add_argument("--command", default="")
     args = ap.parse_args()
     if args.cpu_worker:
         cpu_worker(args)
+        return
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.out, args.command)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; a multi-GPU figure is not produced "
@@ -129,8 +198,9 @@ def main():
     import brownian_amd as B
     prm = workload_params(args)
     t0 = time.time()
+    mode = B.BW_STREAM_BLOCKED if args.streaming == "blocked" else B.BW_STREAM_PER_SUBSTEP
     s = B.System(precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
-                 orient_law=B.BW_ORIENT_SPHERE, **prm)
+                 orient_law=B.BW_ORIENT_SPHERE, streaming=mode, **prm)
     rng = np.random.default_rng(args.seed)
     v0 = rng.uniform(0.0, 0.05, args.ncol)
     s.initialize()
@@ -152,30 +222,36 @@ def main():
     s.status()
     s.sync()
     log(f"{args.warmup} warm-up steps done")
-    ms = s.time_steps(args.steps)           # HIP events around the graph replays
+    ms = s.time_steps(args.steps)           # HIP events around the graph replays, on the launch stream
     s.status()
     log(f"{args.steps} timed steps: {ms / args.steps:.3f} ms/step")
     step_s = ms / 1e3 / args.steps
     value = nf / step_s
-    b_alg = STREAM_BYTES * args.nsub + SRD_BYTES
-    step_frac = nf * b_alg / step_s / HBM_PEAK
+    n_near = s.near_particles()
 
-    prof = s.profile_steps(args.profile_steps)  # HIP events around every launch
+    prof = s.profile_steps(args.profile_steps)  # HIP events around every launch, same stream
     s.status()
     log("per-kernel timing: " + ", ".join(f"{k} {v[0] / args.profile_steps:.3f} ms/step ({v[1]} launches)"
                                           for k, v in prof.items()))
-    st_ms, st_n = prof["stream"]
-    stream_avg_s = st_ms / 1e3 / max(st_n, 1)
-    achieved = nf * STREAM_BYTES / stream_avg_s
+    units = {"srd": nf, "depart": nf, "stream": n_near}
+    kern = {}
+    for k, (kms, kn) in prof.items():
+        e = {"ms_per_step": kms / args.profile_steps, "launches_per_step": kn / args.profile_steps,
+             "avg_launch_us": 1e3 * kms / max(kn, 1)}
+        if k in ALG_BYTES and kn:
+            b = ALG_BYTES[k] * units[k]
+            e.update(alg_bytes_per_launch=b, achieved_GBps=b / (kms / 1e3 / kn) / 1e9)
+        kern[k] = e
+    dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
+    rk = "srd"  # the HBM kernel that moves every particle of the blocked step
     traffic = None
     if os.path.exists(PMC_FILE):
         try:
-            traffic = json.load(open(PMC_FILE)).get("bytes_per_launch")
+            traffic = json.load(open(PMC_FILE))["kernels"]["k_srd"]["bytes_per_launch"]
         except Exception:
             traffic = None
-    copy_bytes = 1 << 30
-    copy_ms = s.debug_copy_ms(copy_bytes, reps=10)
-    copy_bw = 2 * copy_bytes / (copy_ms / 1e3)
+    copy_ms = s.debug_copy_ms(COPY_BYTES, reps=args.copy_reps)
+    copy_bw = 2 * COPY_BYTES / (copy_ms / 1e3)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -194,6 +270,7 @@ def main():
         elif r:
             cpu = {"value": None, "unit": "fluid particle-steps/s", "cores": 1, "kind": "reference",
                    "sample": "failed: " + str(r.get("error"))}
+    ach = kern[rk]["achieved_GBps"] * 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -209,15 +286,16 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.L}^3 cells x 10 fluid/cell = {nf:,} fluid particles, {args.ncol} colloids "
                                f"R=2 (WCA sigma 4.2, m 50), v0~U[0,0.05], SRD 90 deg, {args.nsub} substeps/step, "
-                               f"FP32 int-cell storage, FP64 cell sums",
+                               f"FP32 int-cell storage, FP64 cell sums, {args.streaming} streaming",
                    "L": args.L, "n_fluid": nf, "n_colloid": args.ncol, "n_sub": args.nsub,
-                   "tile": list(s.info[0]["tile"]), "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "k_stream (streaming + detection)",
-                     "kernel_avg_us": stream_avg_s * 1e6, "bytes_alg_per_launch": nf * STREAM_BYTES},
-        "step_hbm_frac": step_frac,
-        "step_bytes_alg_per_particle": b_alg,
-        "kernel_ms_per_step": {k: v[0] / args.profile_steps for k, v in prof.items()},
+                   "tile": list(s.info[0]["tile"]), "streaming": args.streaming, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK, "traffic": traffic,
+                     "kernel": "k_srd (gather, sort, rotate, stream far particles)",
+                     "kernel_avg_us": kern[rk]["avg_launch_us"], "alg_bytes_per_launch": kern[rk]["alg_bytes_per_launch"]},
+        "dominant_kernel_by_time": dom,
+        "kernels": kern,
+        "near_fraction": n_near / nf,
         "copy_kernel_GBps": copy_bw / 1e9,
         "cpu_baseline": cpu,
         "setup_s": t_setup,

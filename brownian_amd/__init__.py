@@ -304,6 +304,11 @@ class System:
         return dict(ke_colloid=o[0], ang_ke_colloid=o[1], potential=o[2], ke_fluid=o[3], momentum=o[4:7],
                     fluid_momentum=o[7:10], momentum_scale=o[10])
 
+    def near_particles(self, replica=0) -> int:
+        n = C.c_longlong(0)
+        self._chk(self.lib.bw_near_particles(self.h, replica, C.byref(n)))
+        return n.value
+
     def debug_copy_ms(self, nbytes, reps=10):
         ms = C.c_double(0)
         self._chk(self.lib.bw_debug_copy(self.h, C.c_longlong(nbytes), reps, C.byref(ms)))

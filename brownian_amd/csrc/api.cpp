@@ -1292,6 +1292,18 @@ BW_API int bw_host_initialize(const bw_params *p, double *pos_fl, double *vel_fl
     return BW_OK;
 }
 
+BW_API int bw_near_particles(bw_ctx *c, int r, long long *n_near) {
+    ReplicaHost *R = rep_of(c, r);
+    if (!R || !n_near) return BW_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::vector<int> cnt(R->n_tiles);
+    HIPCHK(c, hipMemcpy(cnt.data(), c->dv.near_cnt + R->first_tile, sizeof(int) * R->n_tiles, hipMemcpyDeviceToHost));
+    long long s = 0;
+    for (int v : cnt) s += v;
+    *n_near = s;
+    return BW_OK;
+}
+
 BW_API int bw_debug_copy(bw_ctx *c, long long nbytes, int reps, double *ms) {
     if (!c || nbytes <= 0 || reps < 1) return BW_ERR_ARG;
     void *a = nullptr, *b = nullptr;

@@ -181,23 +181,10 @@ __device__ __forceinline__ void normalise(int &i, Real &o, int L) {
     }
 }
 
-// normalise() for an offset in (-1, 2): the same operations, selected by
-// comparisons instead of floor (o - (-1) and o + 1 are the same IEEE add)
-template <class Real>
-__device__ __forceinline__ void step_cell(int &i, Real &o, int L) {
-    if (o < Real(0)) {
-        o = o + Real(1);
-        i -= 1;
-        if (o >= Real(1)) {
-            o = Real(0);
-            i += 1;
-        }
-        if (i < 0) i += L;
-    } else if (o >= Real(1)) {
-        o = o - Real(1);
-        i += 1;
-        if (i >= L) i -= L;
-    }
+// o in [0,1) (+0 included, -0 and NaN excluded) with one integer compare
+__device__ __forceinline__ bool in_unit(float o) { return __float_as_uint(o) < 0x3F800000u; }
+__device__ __forceinline__ bool in_unit(double o) {
+    return (unsigned long long)__double_as_longlong(o) < 0x3FF0000000000000ull;
 }
 
 // ------------------------------------------------------------ reflection

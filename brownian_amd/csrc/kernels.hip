@@ -213,6 +213,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     __shared__ NearColl nc[1];  // used by the near test only
     __shared__ int s_shell[COUNT_SHELL ? TCL_CAP : 1];
     __shared__ AxisMask s_am[COUNT_SHELL ? 2 : 1];  // [0] run shells, [1] near test
+    __shared__ int s_cc[SRD_PT][SRD_THREADS / 64];   // near particles per chunk and wave
     const int T = blockIdx.x;
     Rep R;
     TileGeo tg = tile_geo(d, T, R);
@@ -482,13 +483,25 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     int nrk[SRD_PT];  // near particles of lower gather index
     int n_near = n;
     if (BLOCK) {
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const unsigned long long lt = (1ull << lane) - 1ull;
+        unsigned long long bal[SRD_PT];
+#pragma unroll
+        for (int k = 0; k < SRD_PT; k++) {
+            bal[k] = __ballot((threadIdx.x + k * SRD_THREADS < n) && (nearbits & (1u << k)));
+            if (lane == 0) s_cc[k][w] = __popcll(bal[k]);
+        }
+        __syncthreads();
         int base = 0;
 #pragma unroll
         for (int k = 0; k < SRD_PT; k++) {
-            int g = threadIdx.x + k * SRD_THREADS;
-            int tot;
-            int idx = block_compact(g < n && (nearbits & (1u << k)), s_wtot, tot);
-            nrk[k] = base + idx;
+            int before = 0, tot = 0;
+            for (int v = 0; v < SRD_THREADS / 64; v++) {
+                int cnt = s_cc[k][v];
+                before += v < w ? cnt : 0;
+                tot += cnt;
+            }
+            nrk[k] = base + before + __popcll(bal[k] & lt);
             base += tot;
         }
         n_near = base;
@@ -506,8 +519,8 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
                     slot_in_tile = nrk[k];
                 } else {
                     slot_in_tile = n_near + g - nrk[k];
-                    // every substep of the step as the streaming kernel does it;
-                    // the offsets start in [0,1) and move by less than a cell
+                    // every substep of the step as the streaming kernel does it
+                    // (normalise() is the identity while an offset is in [0,1))
                     int ix, iy, iz;
                     unpack_cell(cell, ix, iy, iz);
                     const Real dt = (Real)ph.dt;
@@ -516,9 +529,11 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
                         ox = ox + dx;
                         oy = oy + dy;
                         oz = oz + dz;
-                        step_cell(ix, ox, L);
-                        step_cell(iy, oy, L);
-                        step_cell(iz, oz, L);
+                        if (!(in_unit(ox) && in_unit(oy) && in_unit(oz))) {
+                            normalise(ix, ox, L);
+                            normalise(iy, oy, L);
+                            normalise(iz, oz, L);
+                        }
                     }
                     cell = pack_cell(ix, iy, iz);
                     if (!(near_interval(ix, ox0 - TILE_REACH, B0 + 2 * TILE_REACH, L) &&

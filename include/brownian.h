@@ -210,6 +210,11 @@ BW_API int bw_host_initialize(const bw_params *p, double *pos_fl, double *vel_fl
 /* Calibration: copy n bytes device to device with a plain vector kernel. */
 BW_API int bw_debug_copy(bw_ctx *ctx, long long nbytes, int reps, double *ms);
 
+/* Diagnostics (no counterpart in the reference): particles of replica r that
+ * the last fused step's per-substep kernels touched ("near" a colloid under
+ * blocked streaming; all particles under per-substep streaming). */
+BW_API int bw_near_particles(bw_ctx *ctx, int replica, long long *n_near);
+
 #ifdef __cplusplus
 }
 #endif
