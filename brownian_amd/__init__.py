@@ -304,6 +304,15 @@ class System:
         return dict(ke_colloid=o[0], ang_ke_colloid=o[1], potential=o[2], ke_fluid=o[3], momentum=o[4:7],
                     fluid_momentum=o[7:10], momentum_scale=o[10])
 
+    def observables_reset(self):
+        self._chk(self.lib.bw_observables_reset(self.h))
+
+    def observables(self, replica=0) -> dict:
+        o = np.zeros(6)
+        self._chk(self.lib.bw_observables(self.h, replica, _ptr(o)))
+        return dict(lag_steps=int(o[0]), msd=o[1], orient_corr=o[2], t_fluid=o[3], t_colloid=o[4],
+                    t_rot=o[5])
+
     def near_particles(self, replica=0) -> int:
         n = C.c_longlong(0)
         self._chk(self.lib.bw_near_particles(self.h, replica, C.byref(n)))

@@ -55,6 +55,8 @@ struct ReplicaHost {
     std::vector<std::vector<double>> inj_v;
     double inj_shift[3];
     std::vector<double> inj_axes;
+    std::vector<double> ra0;  // orientations at the observables' time origin
+    long long obs_step0 = 0;
 };
 
 struct FluidAlloc {
@@ -74,6 +76,7 @@ struct bw_ctx {
     Phys ph{};
     Dev dv{};
     hipStream_t st = nullptr;
+    long long steps_done = 0;      // MPCD steps taken (fused or through rotation_mpcd)
     int cur = 0;                   // fluid buffer holding the state
     int cb = 0;                    // colloid position/cell buffer holding the state
     int substep = 0;
@@ -366,6 +369,7 @@ int do_steps(bw_ctx *c, int n) {
         HIPCHK(c, hipGraphLaunch(c->graphs[c->cur][c->cb][fin], c->st));
         c->cb = cb_after(c, c->cb, fin != 0);
         c->cur ^= 1;
+        c->steps_done++;
     }
     c->pending_prologue = true;
     c->substep = 0;
@@ -383,6 +387,7 @@ int do_profile(bw_ctx *c, int n, double *ms, int *launches) {
         c->pending_prologue = false;
         c->cb = enqueue_step<Real>(c, c->cur, c->cb, s == n - 1, &prof);
         c->cur ^= 1;
+        c->steps_done++;
     }
     prof.mark(c->st, BW_KERNEL_OTHER);
     c->pending_prologue = true;
@@ -837,6 +842,7 @@ BW_API int initialize(bw_ctx *c) {
     c->cb = 0;
     c->pending_prologue = true;
     c->substep = 0;
+    c->steps_done = 0;
     return BW_OK;
 }
 
@@ -947,6 +953,7 @@ BW_API int rotation_mpcd(bw_ctx *c) {
     }
     c->cur ^= 1;
     c->substep = 0;
+    c->steps_done++;
     return BW_OK;
 }
 
@@ -1289,6 +1296,60 @@ BW_API int bw_host_initialize(const bw_params *p, double *pos_fl, double *vel_fl
     if (omega) memcpy(omega, cw.data(), n);
     if (ra) memcpy(ra, cr.data(), n);
     if (rng_out36) R.rng.get(rng_out36);
+    return BW_OK;
+}
+
+BW_API int bw_observables_reset(bw_ctx *c) {
+    if (!c) return BW_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    size_t NC = std::max(c->nc_total, 1);
+    HIPCHK(c, hipMemsetAsync(c->dv.cdisp, 0, sizeof(double) * 3 * NC, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (auto &R : c->reps) {
+        R.ra0.assign(3 * (size_t)R.nc, 0.0);
+        if (R.nc)
+            HIPCHK(c, hipMemcpy(R.ra0.data(), c->dv.cra + 3 * (size_t)R.first_colloid, sizeof(double) * 3 * R.nc,
+                                hipMemcpyDeviceToHost));
+        R.obs_step0 = c->steps_done;
+    }
+    return BW_OK;
+}
+
+BW_API int bw_observables(bw_ctx *c, int r, double *out) {
+    ReplicaHost *R = rep_of(c, r);
+    if (!R || !out) return BW_ERR_ARG;
+    double a[11];
+    int rc = bw_audit(c, r, a);  // synchronises; FP64 fluid sums
+    if (rc) return rc;
+    const double m = R->p.mass_fl, N = (double)R->nf;
+    const double p2 = a[7] * a[7] + a[8] * a[8] + a[9] * a[9];  // |m sum v|^2
+    out[0] = (double)(c->steps_done - R->obs_step0);
+    out[3] = R->nf > 1 ? (2.0 * a[3] - p2 / (N * m)) / (3.0 * (N - 1.0)) : 0.0;
+    out[1] = out[2] = out[4] = out[5] = 0.0;
+    if (R->nc) {
+        size_t n3 = 3 * (size_t)R->nc, o = 3 * (size_t)R->first_colloid;
+        std::vector<double> d(n3), ra(n3), v(n3), w(n3);
+        HIPCHK(c, hipMemcpy(d.data(), c->dv.cdisp + o, sizeof(double) * n3, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(ra.data(), c->dv.cra + o, sizeof(double) * n3, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(v.data(), c->dv.cvel + o, sizeof(double) * n3, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(w.data(), c->dv.comega + o, sizeof(double) * n3, hipMemcpyDeviceToHost));
+        if (R->ra0.size() != n3) R->ra0 = ra;  // no reset yet: the origin is now
+        double msd = 0, oc = 0, tv = 0, tw = 0;
+        for (int i = 0; i < R->nc; i++) {
+            double dd = 0, rr = 0, vv = 0, ww = 0;
+            for (int k = 0; k < 3; k++) {
+                dd += d[3 * i + k] * d[3 * i + k];
+                rr += ra[3 * i + k] * R->ra0[3 * i + k];
+                vv += v[3 * i + k] * v[3 * i + k];
+                ww += w[3 * i + k] * w[3 * i + k];
+            }
+            msd += dd; oc += rr; tv += vv; tw += ww;
+        }
+        out[1] = msd / R->nc;
+        out[2] = oc / R->nc;
+        out[4] = R->p.mass_colloid * tv / (3.0 * R->nc);
+        out[5] = c->ph.I_colloid * tw / (3.0 * R->nc);
+    }
     return BW_OK;
 }
 

@@ -210,6 +210,20 @@ BW_API int bw_host_initialize(const bw_params *p, double *pos_fl, double *vel_fl
 /* Calibration: copy n bytes device to device with a plain vector kernel. */
 BW_API int bw_debug_copy(bw_ctx *ctx, long long nbytes, int reps, double *ms);
 
+/* ----------------------------------------------------------- observables */
+/* No counterpart in the reference (it computes energies and prints nothing,
+ * cpp/active_brownian.cpp:39-55).  bw_observables_reset sets the time origin
+ * of every replica: colloid displacements restart from zero and the current
+ * orientations become ra(0).  bw_observables then returns for replica r:
+ *   out[0] MPCD steps since the reset
+ *   out[1] colloid mean squared displacement <|x(t)-x(0)|^2> (unwrapped)
+ *   out[2] orientation autocorrelation <ra(t).ra(0)>
+ *   out[3] fluid temperature m (sum|v|^2 - |sum v|^2/N) / (3 (N-1))
+ *   out[4] colloid translational temperature m_c <|v|^2> / 3
+ *   out[5] colloid rotational temperature I <|w|^2> / 3                    */
+BW_API int bw_observables_reset(bw_ctx *ctx);
+BW_API int bw_observables(bw_ctx *ctx, int replica, double *out6);
+
 /* Diagnostics (no counterpart in the reference): particles of replica r that
  * the last fused step's per-substep kernels touched ("near" a colloid under
  * blocked streaming; all particles under per-substep streaming). */

@@ -1,0 +1,98 @@
+"""GPU: observables (colloid MSD, orientation autocorrelation, temperatures),
+the SRD thermostat's per-cell closed forms, and the FP32 error report."""
+import numpy as np
+import pytest
+
+import brownian_amd as B
+from brownian_amd.error_report import report
+
+pytestmark = pytest.mark.gpu
+
+
+def test_msd_and_orientation_closed_forms():
+    """One free colloid (no force, no collisions called): x(t) = x0 + v t and
+    ra rotates about z by w dt per update (cpp/update.cpp:34-54)."""
+    prm = dict(L=12, density=3, n_colloid=1, sig_colloid=2.5, sigma=2.0, mass_colloid=50.0, n_sub=10,
+               precision=B.BW_FP64)
+    s = B.System(**prm)
+    s.startup()
+    z = np.zeros((1, 3))
+    v, w = 0.5, 2.0
+    s.upload_colloid(pos=np.array([[6.0, 6.0, 6.0]]), vel=np.array([[v, 0.0, 0.0]]),
+                     omega=np.array([[0.0, 0.0, w]]), f=z, old_f=z, ra=np.array([[1.0, 0.0, 0.0]]))
+    s.observables_reset()
+    dt = 0.1 / prm["n_sub"]
+    k = 7
+    for _ in range(k):
+        s.update_activity_direction()
+        s.update_pos_md()
+    o = s.observables()
+    assert o["lag_steps"] == 0
+    assert o["msd"] == pytest.approx((k * v * dt) ** 2, rel=1e-12)
+    assert o["orient_corr"] == pytest.approx(np.cos(k * w * dt), rel=1e-12)
+    assert o["t_colloid"] == pytest.approx(50.0 * v * v / 3, rel=1e-14)
+    I = 0.1 * 50.0 * 2.0 ** 2
+    assert o["t_rot"] == pytest.approx(I * w * w / 3, rel=1e-14)
+    c = s.download_colloid()
+    np.testing.assert_allclose(c["pos"][0], [6.0 + k * v * dt, 6.0, 6.0], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("precision", [B.BW_FP64, B.BW_FP32])
+def test_thermostat_per_cell_closed_form(precision):
+    """After rotation_mpcd every shifted cell with N > 1 keeps its momentum and
+    has relative kinetic energy 3 (N-1) kT / 2 exactly (cpp/rotation_mpcd.cpp:37-49)."""
+    prm = dict(L=10, density=5, n_colloid=2, sig_colloid=2.5, sigma=2.0, mass_colloid=50.0, n_sub=10,
+               theta_deg=90.0, precision=precision, vel_law=B.BW_VEL_MAXWELL)
+    s = B.System(**prm)
+    s.startup()
+    s.step(2)
+    s.status()
+    _, v0, _ = s.download_fluid()
+    s.rotation_mpcd()
+    assert s.status() == 0
+    _, v1, _ = s.download_fluid()
+    cells = s.shifted_cells()
+    ncell = prm["L"] ** 3
+    cnt = np.bincount(cells, minlength=ncell)
+    tol = 1e-12 if precision == B.BW_FP64 else 2e-6
+    for comp in range(3):
+        p0 = np.bincount(cells, weights=v0[:, comp], minlength=ncell)
+        p1 = np.bincount(cells, weights=v1[:, comp], minlength=ncell)
+        assert (np.abs(p1 - p0) / np.maximum(cnt, 1)).max() < tol
+    mean = np.stack([np.bincount(cells, weights=v1[:, k], minlength=ncell) for k in range(3)], 1)
+    mean /= np.maximum(cnt, 1)[:, None]
+    rel = ((v1 - mean[cells]) ** 2).sum(1)
+    e = np.bincount(cells, weights=rel, minlength=ncell)
+    m = cnt > 1
+    assert np.abs(e[m] / (3.0 * (cnt[m] - 1)) - 1).max() < 10 * tol
+
+
+def test_fluid_temperature_observable_matches_state():
+    prm = dict(L=12, density=10, n_colloid=3, sig_colloid=4.2, sigma=4.0, mass_colloid=50.0, n_sub=20,
+               theta_deg=90.0, v0=0.02, precision=B.BW_FP32, vel_law=B.BW_VEL_MAXWELL,
+               orient_law=B.BW_ORIENT_SPHERE)
+    s = B.System(**prm)
+    s.startup()
+    s.observables_reset()
+    s.step(5)
+    o = s.observables()
+    assert o["lag_steps"] == 5
+    _, v, _ = s.download_fluid()
+    N = len(v)
+    t = (np.sum(v * v) - np.sum(np.sum(v, 0) ** 2) / N) / (3 * (N - 1))
+    assert o["t_fluid"] == pytest.approx(t, rel=1e-10)
+    assert 0.9 < o["t_fluid"] < 1.1       # the thermostat holds kT = 1
+    assert -1.0 <= o["orient_corr"] <= 1.0 and o["msd"] > 0
+
+
+def test_fp32_error_report_within_north_star_tolerance():
+    """FP32 storage vs an FP64 run from the same state: per-step errors far
+    below 1e-5 relative, apart from discrete collision decisions within a
+    rounding distance of a colloid surface (counted by the report)."""
+    r = report(L=16, ncol=6, steps=2)
+    w = r["per_step_worst"]
+    assert w["fluid_vel_rel_q999999"] < 1e-5
+    assert w["fluid_pos_abs_q999999"] < 1e-5 * 16
+    assert w["fluid_n_vel_rel_over_1e-5"] <= 4
+    for k in ("colloid_pos_rel", "colloid_vel_rel", "colloid_omega_rel", "colloid_ra_rel"):
+        assert w[k] < 1e-5, k
