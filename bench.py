@@ -165,6 +165,49 @@ def pmc_summary(fetch_dir, write_dir, out_file, command):
     print(json.dumps(res, indent=1))
 
 
+# ---------------------------------------------------------------- ensemble
+def ensemble_params(n_rep, seed0=1000):
+    """Batched ensemble: box edge uniform in {16,24,32,40,48} at 10 particles
+    per cell, colloid count uniform in 1..cap(L) with the 4 % volume-fraction
+    cap (16->4, 24->16, 32->39, >=40->64), v0 uniform in [0, 0.1], seeds
+    seed0 + r."""
+    rng = np.random.default_rng(seed0)
+    cap = {16: 4, 24: 16, 32: 39}
+    reps = []
+    for r in range(n_rep):
+        L = int(rng.choice([16, 24, 32, 40, 48]))
+        nc = int(rng.integers(1, cap.get(L, 64) + 1))
+        reps.append(dict(L=L, density=10, n_colloid=nc, n_sub=20, ndt=0.1, kbt=1.0, mass_fl=1.0,
+                         mass_colloid=50.0, sig_colloid=4.2, eps=1.0, sigma=4.0, theta_deg=90.0,
+                         v0=float(rng.uniform(0.0, 0.1)), seed=seed0 + r))
+    return reps
+
+
+def run_ensemble(B, n_rep, steps, warmup):
+    reps = ensemble_params(n_rep)
+    t0 = time.time()
+    s = B.System(replicas=[dict(r, precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
+                                orient_law=B.BW_ORIENT_SPHERE) for r in reps])
+    s.startup()
+    s.sync()
+    t_setup = time.time() - t0
+    nf = sum(i["nf"] for i in s.info)
+    nc = sum(i["nc"] for i in s.info)
+    log(f"ensemble: {n_rep} replicas, {nf:,} fluid particles, {nc} colloids, start-up {t_setup:.1f} s")
+    s.step(warmup)
+    s.status()
+    ms = s.time_steps(steps)
+    s.status()
+    near = sum(s.near_particles(r) for r in range(n_rep))
+    step_s = ms / 1e3 / steps
+    log(f"ensemble: {steps} timed steps: {ms / steps:.3f} ms/step")
+    return {"workload": f"{n_rep} replicas, L~U{{16,24,32,40,48}} x 10/cell, colloids U{{1..cap(L)}} (4 % cap), "
+                        f"v0~U[0,0.1], seeds 1000+r, 20 substeps, FP32, one set of launches per step",
+            "replicas": n_rep, "n_fluid": nf, "n_colloid": nc, "steps": steps, "ms_per_step": ms / steps,
+            "fluid_particle_steps_per_s": nf / step_s, "colloid_steps_per_s": nc / step_s,
+            "near_fraction": near / nf, "setup_s": t_setup}
+
+
 # ------------------------------------------------------------------- bench
 def main():
     ap = argparse.ArgumentParser()
@@ -178,6 +221,9 @@ def main():
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--streaming", choices=["blocked", "per-substep"], default="blocked")
     ap.add_argument("--copy-reps", type=int, default=10)
+    ap.add_argument("--ensemble", type=int, default=256, help="replicas of the batched run (0 = skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=50)
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
@@ -199,10 +245,18 @@ def main():
     prm = workload_params(args)
     t0 = time.time()
     mode = B.BW_STREAM_BLOCKED if args.streaming == "blocked" else B.BW_STREAM_PER_SUBSTEP
-    s = B.System(precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
-                 orient_law=B.BW_ORIENT_SPHERE, streaming=mode, **prm)
+    sprm = dict(precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
+                orient_law=B.BW_ORIENT_SPHERE, streaming=mode, **prm)
     rng = np.random.default_rng(args.seed)
     v0 = rng.uniform(0.0, 0.05, args.ncol)
+    tune = None
+    if args.autotune:
+        from brownian_amd.autotune import autotune
+        log("autotuning the tile shape (a few steps per candidate)")
+        tune = autotune(sprm, steps=3, warmup=1, v0=v0, log=log)
+        sprm["tile"] = tune["best"]
+        log(f"tile {tune['best']} chosen")
+    s = B.System(**sprm)
     s.initialize()
     s.initialize_colloid()
     log("colloids placed")
@@ -217,7 +271,8 @@ def main():
     t_setup = time.time() - t0
     nf = s.info[0]["nf"]
 
-    log(f"start-up done in {t_setup:.1f} s; {nf:,} fluid particles, tile {s.info[0]['tile']}")
+    tile_used = list(s.info[0]["tile"])
+    log(f"start-up done in {t_setup:.1f} s; {nf:,} fluid particles, tile {tuple(tile_used)}")
     s.step(args.warmup)
     s.status()
     s.sync()
@@ -270,6 +325,13 @@ def main():
         elif r:
             cpu = {"value": None, "unit": "fluid particle-steps/s", "cores": 1, "kind": "reference",
                    "sample": "failed: " + str(r.get("error"))}
+    ens = None
+    if args.ensemble > 0:
+        del s
+        try:
+            ens = run_ensemble(B, args.ensemble, args.ensemble_steps, 5)
+        except Exception as e:  # reported, the single-system figure stands
+            ens = {"error": repr(e)}
     ach = kern[rk]["achieved_GBps"] * 1e9
     out = {
         "metric": METRIC,
@@ -288,7 +350,7 @@ def main():
                                f"R=2 (WCA sigma 4.2, m 50), v0~U[0,0.05], SRD 90 deg, {args.nsub} substeps/step, "
                                f"FP32 int-cell storage, FP64 cell sums, {args.streaming} streaming",
                    "L": args.L, "n_fluid": nf, "n_colloid": args.ncol, "n_sub": args.nsub,
-                   "tile": list(s.info[0]["tile"]), "streaming": args.streaming, "parallelism": "single GPU"},
+                   "tile": tile_used, "streaming": args.streaming, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": ach / HBM_PEAK, "traffic": traffic,
                      "kernel": "k_srd (gather, sort, rotate, stream far particles)",
@@ -296,6 +358,12 @@ def main():
         "dominant_kernel_by_time": dom,
         "kernels": kern,
         "near_fraction": n_near / nf,
+        "step_bytes_per_particle": {"plain_per_substep": 44 * args.nsub + 40, "fused_declared": 84,
+                                    "plain_equivalent_hbm_frac": (44 * args.nsub + 40) * value / HBM_PEAK,
+                                    "fused_hbm_frac": 84 * value / HBM_PEAK},
+        "autotune": None if tune is None else {"best": list(tune["best"]),
+                                                "ms_per_step": {str(k): v for k, v in tune["ms_per_step"].items()}},
+        "ensemble": ens,
         "copy_kernel_GBps": copy_bw / 1e9,
         "cpu_baseline": cpu,
         "setup_s": t_setup,

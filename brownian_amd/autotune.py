@@ -1,0 +1,87 @@
+"""Tile-shape autotuner.
+
+The fused step's tile shape (cells per tile along x, y, z) sets the work of
+one workgroup in the rotation pass, the departure lists and the per-substep
+kernels, and with it the LDS footprint and the occupancy.  Results do not
+depend on it (the tile re-sort, the per-cell sums in id order and the event
+ranking are shape-independent), so the tuner may pick any candidate by
+speed alone.  It times a few fused steps of the caller's system for each
+candidate, from one common start-up state, and returns the fastest.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import BW_FP64, System
+
+MAX_CAP = 4096
+AXIS_MAX = 64
+TILE_REACH = 3
+
+
+def tile_slots(ncell: int, density: int) -> int:
+    """Slots per tile as the library sizes them (mean + 15 % + 64, to 64)."""
+    need = (ncell * density * 23 + 19) // 20 + 64
+    return (need + 63) // 64 * 64
+
+
+def candidates(L: int, density: int, fp64: bool = False) -> list[tuple[int, int, int]]:
+    """Tile shapes the library accepts for a box of edge L: each edge divides
+    L and is at least 4, at most 512 cells, the slots and the SRD pass's LDS
+    (per-cell FP64 sums, per-slot staging) fit, the region fits the masks."""
+    divs = [b for b in range(4, min(L, 16) + 1) if L % b == 0] or [L]
+    rb = 8 if fp64 else 4
+    out = []
+    for bx in divs:
+        for by in divs:
+            for bz in divs:
+                if by > bx or bz > by:
+                    continue
+                n = bx * by * bz
+                cap = tile_slots(n, density)
+                if n < 64 or n > 512 or cap > MAX_CAP:
+                    continue
+                if any(min(b + 2 * TILE_REACH, L) > AXIS_MAX for b in (bx, by, bz)):
+                    continue
+                lds = n * 56 + 3 * cap * rb + cap * 4 + (2 * n + 1) * 4 + cap * 2
+                if lds > 160 * 1024 - 16 * 1024:
+                    continue
+                out.append((bx, by, bz))
+    return out
+
+
+def autotune(params: dict, steps: int = 4, warmup: int = 2, shapes=None, v0=None, log=None) -> dict:
+    """Time `steps` fused steps for each candidate tile shape of one system
+    (params as for System) from one common start-up state; returns
+    {"best": shape, "ms_per_step": {shape: ms}}."""
+    prm = {k: v for k, v in params.items() if k != "tile"}
+    base = System(**prm)
+    base.startup()
+    pos, vel, _ = base.download_fluid()
+    col = base.download_colloid()
+    base.close()
+    fp64 = prm.get("precision", 0) == BW_FP64
+    shapes = shapes or candidates(prm["L"], prm.get("density", 10), fp64)
+    times = {}
+    for shape in shapes:
+        t0 = time.time()
+        s = System(tile=shape, **prm)
+        s.initialize()
+        s.upload_fluid(pos, vel)
+        s.upload_colloid(**col)
+        s.neighbour_list_md()
+        s.compute_force_md()
+        if v0 is not None:
+            s.set_v0(v0)
+        s.step(warmup)
+        s.status()
+        ms = s.time_steps(steps) / steps
+        s.status()
+        s.close()
+        times[shape] = ms
+        if log:
+            log(f"tile {shape}: {ms:.3f} ms/step ({time.time() - t0:.1f} s)")
+    best = min(times, key=times.get)
+    return {"best": best, "ms_per_step": times}

@@ -96,3 +96,49 @@ def test_fp32_error_report_within_north_star_tolerance():
     assert w["fluid_n_vel_rel_over_1e-5"] <= 4
     for k in ("colloid_pos_rel", "colloid_vel_rel", "colloid_omega_rel", "colloid_ra_rel"):
         assert w[k] < 1e-5, k
+
+
+def test_autotuner_picks_a_candidate_with_identical_results():
+    """The autotuner times every accepted tile shape; whichever it picks, the
+    trajectory is bit-identical to the default shape's."""
+    from brownian_amd.autotune import autotune, candidates
+    prm = dict(L=16, density=10, n_colloid=4, sig_colloid=4.2, sigma=4.0, mass_colloid=50.0, n_sub=20,
+               theta_deg=90.0, v0=0.03, precision=B.BW_FP32, vel_law=B.BW_VEL_MAXWELL,
+               orient_law=B.BW_ORIENT_SPHERE)
+    res = autotune(prm, steps=2, warmup=1)
+    assert set(res["ms_per_step"]) == set(candidates(16, 10))
+    assert res["best"] in res["ms_per_step"]
+    a = B.System(**prm)
+    b = B.System(tile=res["best"], **prm)
+    a.startup(); b.startup()
+    a.step(3); b.step(3)
+    assert a.status() == 0 and b.status() == 0
+    pa, va, _ = a.download_fluid()
+    pb, vb, _ = b.download_fluid()
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(va, vb)
+
+
+def test_small_ensemble_of_mixed_boxes_equals_single_runs():
+    """Replicas of different edge, colloid count and v0 stepped together give
+    each replica's single-system result bit for bit."""
+    reps = [dict(L=L, density=10, n_colloid=nc, n_sub=20, sig_colloid=4.2, sigma=4.0, mass_colloid=50.0,
+                 theta_deg=90.0, v0=v0, seed=1000 + i, precision=B.BW_FP32, vel_law=B.BW_VEL_MAXWELL,
+                 orient_law=B.BW_ORIENT_SPHERE)
+            for i, (L, nc, v0) in enumerate([(16, 3, 0.05), (24, 10, 0.02), (32, 20, 0.08), (16, 1, 0.0)])]
+    e = B.System(replicas=reps)
+    e.startup()
+    e.step(3)
+    assert e.status() == 0
+    for r, p in enumerate(reps):
+        s = B.System(**p)
+        s.startup()
+        s.step(3)
+        assert s.status() == 0
+        pe, ve, _ = e.download_fluid(r)
+        ps, vs, _ = s.download_fluid()
+        np.testing.assert_array_equal(pe, ps)
+        np.testing.assert_array_equal(ve, vs)
+        ce, cs = e.download_colloid(r), s.download_colloid()
+        for k in ce:
+            np.testing.assert_array_equal(ce[k], cs[k], err_msg=f"replica {r} {k}")
