@@ -33,6 +33,7 @@ ERRORS = {-1: "BW_ERR_ARG", -2: "BW_ERR_HIP", -3: "BW_ERR_CAPACITY", -4: "BW_ERR
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 
 
 class BrownianError(RuntimeError):
@@ -313,6 +314,29 @@ class System:
         return dict(lag_steps=int(o[0]), msd=o[1], orient_corr=o[2], t_fluid=o[3], t_colloid=o[4],
                     t_rot=o[5])
 
+    def set_moments(self, on=True):
+        self._chk(self.lib.bw_set_moments(self.h, int(bool(on))))
+
+    def surrogate_attach(self, mlp, lo=None, hi=None, t_lo=None, t_hi=None):
+        """Drive colloids by the network where every feature lies in [lo, hi]
+        (impulses clamped to [t_lo, t_hi]); mlp=None detaches."""
+        if mlp is None:
+            self._chk(self.lib.bw_surrogate_attach(self.h, None, None, None, None, None))
+            self._mlp = None
+            return
+        a = [np.ascontiguousarray(v, np.float32) for v in (lo, hi, t_lo, t_hi)]
+        self._chk(self.lib.bw_surrogate_attach(self.h, mlp.h, *[v.ctypes.data_as(_fp) for v in a]))
+        self._mlp = mlp  # keeps the network alive while attached
+
+    def surrogate_data(self, replica=0):
+        """features at the last step's start, its collision impulse, colloid modes"""
+        nc = self.info[replica]["nc"]
+        f = np.zeros((nc, 144), np.float32)
+        imp = np.zeros((nc, 6))
+        mode = np.zeros(nc, np.int32)
+        self._chk(self.lib.bw_get_surrogate_data(self.h, replica, f.ctypes.data_as(_fp), _ptr(imp), _ptr(mode, _ip)))
+        return f, imp, mode
+
     def near_particles(self, replica=0) -> int:
         n = C.c_longlong(0)
         self._chk(self.lib.bw_near_particles(self.h, replica, C.byref(n)))
@@ -322,3 +346,85 @@ class System:
         ms = C.c_double(0)
         self._chk(self.lib.bw_debug_copy(self.h, C.c_longlong(nbytes), reps, C.byref(ms)))
         return ms.value
+
+
+class MLP:
+    """The surrogate network (144 -> 3 x 256 SiLU -> 12) on the GPU."""
+
+    N_IN, N_OUT, N_T = 144, 12, 6
+
+    def __init__(self, seed=1):
+        L = lib()
+        L.bw_mlp_last_error.restype = C.c_char_p
+        L.bw_mlp_n_params.restype = C.c_longlong
+        L.bw_mlp_train.argtypes = [C.c_void_p, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, _dp]
+        self.lib = L
+        h = C.c_void_p()
+        rc = L.bw_mlp_create(int(seed), C.byref(h))
+        if rc:
+            msg = L.bw_mlp_last_error(h).decode() if h.value else ""
+            if h.value:
+                L.bw_mlp_destroy(h)
+            raise BrownianError(rc, "bw_mlp_create: " + msg)
+        self.h = h
+        self.n_params = int(L.bw_mlp_n_params())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bw_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise BrownianError(rc, self.lib.bw_mlp_last_error(self.h).decode())
+        return rc
+
+    @staticmethod
+    def _f32(a, cols):
+        a = np.ascontiguousarray(a, np.float32)
+        assert a.ndim == 2 and a.shape[1] == cols
+        return a
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        self._chk(self.lib.bw_mlp_get_params(self.h, p.ctypes.data_as(_fp)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        self._chk(self.lib.bw_mlp_set_params(self.h, p.ctypes.data_as(_fp)))
+
+    def set_normalisation(self, x_mean, x_std, t_mean, t_std):
+        a = [np.ascontiguousarray(v, np.float32) for v in (x_mean, x_std, t_mean, t_std)]
+        self._chk(self.lib.bw_mlp_set_normalisation(self.h, *[v.ctypes.data_as(_fp) for v in a]))
+        self.norm = a
+
+    def forward(self, x) -> np.ndarray:
+        x = self._f32(x, self.N_IN)
+        y = np.zeros((len(x), self.N_OUT), np.float32)
+        self._chk(self.lib.bw_mlp_forward(self.h, x.ctypes.data_as(_fp), len(x), y.ctypes.data_as(_fp)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = self._f32(x, self.N_IN)
+        t = self._f32(t, self.N_T)
+        loss = C.c_double(0)
+        g = np.zeros(self.n_params, np.float32)
+        self._chk(self.lib.bw_mlp_loss_grad(self.h, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp), len(x),
+                                            C.byref(loss), g.ctypes.data_as(_fp)))
+        return loss.value, g
+
+    def train(self, x, t, epochs=10, batch=4096, lr=1e-3, seed=7) -> np.ndarray:
+        x = self._f32(x, self.N_IN)
+        t = self._f32(t, self.N_T)
+        out = np.zeros(epochs)
+        self._chk(self.lib.bw_mlp_train(self.h, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp), len(x), int(epochs),
+                                        int(batch), float(lr), int(seed), out.ctypes.data_as(_dp)))
+        return out

@@ -34,7 +34,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(BIN, exist_ok=True)
     objs = []
-    for src, kind in [("kernels.hip", "hip"), ("api.cpp", "hip"), ("host_init.cpp", "host")]:
+    for src, kind in [("kernels.hip", "hip"), ("mlp.hip", "hip"), ("api.cpp", "hip"), ("host_init.cpp", "host")]:
         out = os.path.join(OBJ, src + ".o")
         if kind == "hip":
             _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(SRC, src), "-o", out], verbose)

@@ -12,6 +12,7 @@
 #include "../../include/brownian.h"
 #include "bw_common.h"
 #include "host_init.h"
+#include "mlp.h"
 
 namespace bw {
 // kernel launchers (kernels.hip)
@@ -37,6 +38,10 @@ void launch_colloid_update(const Phys &, const Dev &, unsigned, const int2 *, in
 void launch_md_candidates(const Phys &, const Dev &, const int2 *, int, hipStream_t);
 void launch_tile_colloids(const Phys &, const Dev &, hipStream_t);
 void launch_copy(const void *, void *, long long, hipStream_t);
+void launch_features(const Phys &, const Dev &, const int *, hipStream_t);
+template <class Real> void launch_mark_phantoms(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
+hipError_t prepare_kernels();
+void launch_sur_sample(const Phys &, const Dev &, const int *, hipStream_t);
 }  // namespace bw
 
 using namespace bw;
@@ -97,6 +102,9 @@ struct bw_ctx {
     double *stage_pos = nullptr, *stage_vel = nullptr;
     int *stage_cell = nullptr;
     bool inj_dirty = false;
+    bool moments_ok = false;        // every replica has an even edge and even tile edges
+    bw_mlp *mlp = nullptr;          // attached surrogate network
+    MlpBuf mbuf;                    // its activations for the step's inference
     hipGraphExec_t graphs[2][2][2] = {};  // [cur][cb][final]
 };
 
@@ -301,6 +309,15 @@ int enqueue_step(bw_ctx *c, int cur, int cb, bool final, Prof *prof) {
     Fluid<Real> &src = fluid_of<Real>(c, cur), &dst = fluid_of<Real>(c, 1 - cur);
     Dev d0 = dev_at(c, cur, cb);
     if (prof) prof->mark(st, BW_KERNEL_OTHER);
+    if (col && ph.moments) {
+        // surrogate inputs from the last rotation pass (its shift is still current)
+        launch_features(ph, d0, c->colloid_rep, st);
+        if (ph.surrogate) {
+            mlp_forward_device(c->mlp, c->mbuf, d0.feat, c->nc_total, d0.sur_out, st);
+            launch_sur_sample(ph, d0, c->colloid_rep, st);
+        }
+        if (prof) prof->mark(st, BW_KERNEL_COLLOID);
+    }
     launch_step_begin(ph, d0, st);
     if (col) {
         launch_md_candidates(ph, d0, c->wgt, c->nwg, st);
@@ -522,6 +539,12 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
     c->nc_total = nc;
     int maxcell = 0;
     for (auto &R : c->reps) maxcell = std::max(maxcell, R.B[0] * R.B[1] * R.B[2]);
+    c->moments_ok = true;  // coarse cells of 2x2x2 need even edges everywhere
+    for (auto &R : c->reps)
+        if (R.L % 2 || R.B[0] % 2 || R.B[1] % 2 || R.B[2] % 2) c->moments_ok = false;
+    if (c->moments_ok)
+        for (auto &R : c->reps)
+            if (R.first_cell % 8) c->moments_ok = false;
     c->srd_lds = srd_lds_bytes(maxcell, cap, c->fp64);
     if (c->srd_lds > 160 * 1024 - srd_static_lds()) {
         delete c;
@@ -531,6 +554,11 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
     hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
+        return BW_ERR_HIP;
+    }
+    if ((e = prepare_kernels()) != hipSuccess) {
+        c->err = std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
+        *out = c;
         return BW_ERR_HIP;
     }
     // device memory
@@ -556,6 +584,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         for (int k = 0; k < 3; k++) { q.B[k] = R.B[k]; q.nt[k] = R.nt[k]; }
         q.first_tile = R.first_tile; q.n_tiles = R.n_tiles;
         q.first_cell = R.first_cell; q.first_fluid = R.first_fluid;
+        q.first_coarse = R.first_cell / 8;
         unsigned long long s = (unsigned long long)(long long)R.p.seed;
         // key from the seed only, so a replica draws the same numbers alone or in an ensemble
         q.seed_lo = (uint32_t)(s * 0x9E3779B97F4A7C15ull >> 32) ^ 0x5bd1e995u;
@@ -581,6 +610,9 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         (rc = dalloc(c, &d.md_stale, n_rep)) || (rc = dalloc(c, &d.tcl_stale, n_rep)) ||
         (rc = dalloc(c, &d.ovf_cnt, n_rep)) || (rc = dalloc(c, &d.ovf_list, (size_t)n_rep * OVF_CAP)) ||
         (rc = dalloc(c, &d.ev_total, n_rep)) ||
+        (rc = dalloc(c, &d.cmom, c->moments_ok ? (size_t)cells / 8 * 5 : 1)) || (rc = dalloc(c, &d.mom_valid, 1)) ||
+        (rc = dalloc(c, &d.cmode, NC)) || (rc = dalloc(c, &d.cmode_prev, NC)) || (rc = dalloc(c, &d.csur, 6 * NC)) || (rc = dalloc(c, &d.cimp, 6 * NC)) ||
+        (rc = dalloc(c, &d.feat, NC * N_FEAT)) || (rc = dalloc(c, &d.sur_out, NC * 12)) ||
         (rc = dalloc(c, &d.cvel, 3 * NC)) || (rc = dalloc(c, &d.comega, 3 * NC)) ||
         (rc = dalloc(c, &d.cf, 3 * NC)) || (rc = dalloc(c, &d.coldf, 3 * NC)) || (rc = dalloc(c, &d.cra, 3 * NC)) ||
         (rc = dalloc(c, &d.cdisp, 3 * NC)) || (rc = dalloc(c, &d.cpos_run, 3 * NC)) ||
@@ -786,7 +818,9 @@ BW_API int bw_ctx_destroy(bw_ctx *c) {
     if (c->st) hipStreamSynchronize(c->st);
     destroy_graphs(c);
     for (void *q : c->allocs) hipFree(q);
-    const void *inj[] = {c->dv.inj_shift, c->dv.inj_axes, c->dv.inj_off, c->dv.inj_pid, c->dv.inj_val};
+    mlp_buf_free(c->mbuf);
+    const void *inj[] = {c->dv.inj_shift, c->dv.inj_axes, c->dv.inj_off, c->dv.inj_pid, c->dv.inj_val,
+                         c->dv.sur_lo, c->dv.sur_hi};
     for (const void *q : inj)
         if (q) hipFree((void *)q);
     if (c->st) hipStreamDestroy(c->st);
@@ -838,6 +872,10 @@ BW_API int initialize(bw_ctx *c) {
     int *per_rep[] = {c->dv.chain_cnt, c->dv.cont_cnt, c->dv.ovf_cnt, c->dv.ev_total, c->dv.md_stale, c->dv.tcl_stale};
     for (int *a : per_rep) HIPCHK(c, hipMemsetAsync(a, 0, sizeof(int) * c->n_rep, c->st));
     HIPCHK(c, hipMemsetAsync(c->dv.ticket, 0, sizeof(unsigned) * c->n_rep, c->st));
+    HIPCHK(c, hipMemsetAsync(c->dv.cimp, 0, sizeof(double) * 6 * NC, c->st));
+    HIPCHK(c, hipMemsetAsync(c->dv.cmode, 0, sizeof(int) * NC, c->st));
+    HIPCHK(c, hipMemsetAsync(c->dv.cmode_prev, 0, sizeof(int) * NC, c->st));
+    HIPCHK(c, hipMemsetAsync(c->dv.mom_valid, 0, sizeof(int), c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->cb = 0;
     c->pending_prologue = true;
@@ -1350,6 +1388,77 @@ BW_API int bw_observables(bw_ctx *c, int r, double *out) {
         out[4] = R->p.mass_colloid * tv / (3.0 * R->nc);
         out[5] = c->ph.I_colloid * tw / (3.0 * R->nc);
     }
+    return BW_OK;
+}
+
+BW_API int bw_set_moments(bw_ctx *c, int on) {
+    if (!c) return BW_ERR_ARG;
+    if (on && !c->moments_ok) return fail(c, BW_ERR_ARG, "coarse moments need even box and tile edges");
+    if (!on && c->ph.surrogate) return fail(c, BW_ERR_STATE, "the attached surrogate needs the moments");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->ph.moments = on ? 1 : 0;
+    HIPCHK(c, hipMemsetAsync(c->dv.mom_valid, 0, sizeof(int), c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    destroy_graphs(c);  // Phys is part of the captured launches
+    return BW_OK;
+}
+
+BW_API int bw_surrogate_attach(bw_ctx *c, bw_mlp *m, const float *lo, const float *hi, const float *t_lo,
+                               const float *t_hi) {
+    if (!c) return BW_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    destroy_graphs(c);
+    if (!m) {
+        c->mlp = nullptr;
+        // fluid inside the colloids that were phantoms collides with none
+        // until it has left them
+        if (c->ph.surrogate) {
+            if (c->fp64) launch_mark_phantoms<double>(c->ph, dev_for(c, c->cur), c->fl[c->cur].f64, c->st);
+            else launch_mark_phantoms<float>(c->ph, dev_for(c, c->cur), c->fl[c->cur].f32, c->st);
+        }
+        c->ph.surrogate = 0;
+        HIPCHK(c, hipMemsetAsync(c->dv.cmode, 0, sizeof(int) * std::max(c->nc_total, 1), c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        return BW_OK;
+    }
+    if (!lo || !hi || !t_lo || !t_hi) return BW_ERR_ARG;
+    if (!c->moments_ok) return fail(c, BW_ERR_ARG, "the surrogate needs even box and tile edges");
+    if (c->nc_total == 0) return fail(c, BW_ERR_ARG, "no colloids");
+    if (mlp_buf_alloc(c->mbuf, c->nc_total)) return fail(c, BW_ERR_HIP, "hipMalloc (surrogate activations)");
+    for (int k = 0; k < 2; k++) {
+        const float **dst = k ? &c->dv.sur_hi : &c->dv.sur_lo;
+        if (!*dst) {
+            void *q = nullptr;
+            HIPCHK(c, hipMalloc(&q, sizeof(float) * (N_FEAT + 6)));
+            *dst = (const float *)q;
+        }
+        HIPCHK(c, hipMemcpy((void *)*dst, k ? hi : lo, sizeof(float) * N_FEAT, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy((void *)(*dst + N_FEAT), k ? t_hi : t_lo, sizeof(float) * 6, hipMemcpyHostToDevice));
+    }
+    if (!c->ph.moments) {
+        c->ph.moments = 1;
+        HIPCHK(c, hipMemsetAsync(c->dv.mom_valid, 0, sizeof(int), c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    c->mlp = m;
+    c->ph.surrogate = 1;
+    return BW_OK;
+}
+
+// per colloid of replica r after a fused step: the features it started with
+// (n_colloid x 144), its collision impulse over the step (n_colloid x 6:
+// momentum and angular momentum in units of the fluid mass) and its mode
+// (0 explicit, 1 surrogate)
+BW_API int bw_get_surrogate_data(bw_ctx *c, int r, float *feat, double *imp, int *mode) {
+    ReplicaHost *R = rep_of(c, r);
+    if (!R) return BW_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (!R->nc) return BW_OK;
+    size_t g0 = (size_t)R->first_colloid;
+    if (feat)
+        HIPCHK(c, hipMemcpy(feat, c->dv.feat + g0 * N_FEAT, sizeof(float) * N_FEAT * R->nc, hipMemcpyDeviceToHost));
+    if (imp) HIPCHK(c, hipMemcpy(imp, c->dv.cimp + g0 * 6, sizeof(double) * 6 * R->nc, hipMemcpyDeviceToHost));
+    if (mode) HIPCHK(c, hipMemcpy(mode, c->dv.cmode + g0, sizeof(int) * R->nc, hipMemcpyDeviceToHost));
     return BW_OK;
 }
 

@@ -34,6 +34,8 @@ constexpr int NDIR = 27;
 constexpr int TILE_REACH = 3;       // cells a particle may stand outside its tile box
 constexpr double MD_SKIN = 2.0;     // candidate-list skin for the colloid pair list
 constexpr double STREAM_MARGIN = 0.25;  // colloid displacement bound within a step (blocked streaming)
+constexpr int N_FEAT = 144;         // surrogate features per colloid
+constexpr int COARSE_MAX = 64;      // coarse cells of one tile
 
 // colloid phase flags (applied in this order)
 enum : unsigned {
@@ -67,6 +69,8 @@ struct Phys {
     // position at the step's start
     double margin, near_base, near_shell2, vcap_ndt;
     int near_reach;                   // per-axis cell reach of the near test
+    int moments;                      // SRD pass writes coarse-cell moments (surrogate inputs)
+    int surrogate;                    // colloids may be driven by the surrogate impulses
     int n_sub;
     int cap;                          // slots per tile
     int n_rep, n_tiles, n_colloid_total;
@@ -80,6 +84,7 @@ struct Rep {
     int first_tile, n_tiles;
     long long first_cell;   // offset into per-cell arrays
     long long first_fluid;  // offset into id-indexed per-particle arrays
+    long long first_coarse; // offset into coarse-cell arrays (2x2x2 cells of the shifted grid)
     uint32_t seed_lo, seed_hi;
 };
 
@@ -129,6 +134,17 @@ struct Dev {
     int neigh_cap;
     int *shifted_cell;         // id-indexed
     double *audit_partial;     // [n_tiles * 8]
+    // surrogate closure
+    float *cmom;               // [coarse cells * 5]: count, sum v (3), sum |v|^2 after the last SRD pass
+    int *mom_valid;            // the moments describe the current fluid
+    int *cmode;                // per colloid: 0 explicit collisions, 1 surrogate impulse
+    int *cmode_prev;           // per colloid: the mode of the previous step
+    double *csur;              // per colloid: surrogate impulse per substep (6, cdv/cdw units)
+    double *cimp;              // per colloid: collision impulse accumulated since the features (6)
+    float *feat;               // per colloid: surrogate features (144)
+    float *sur_out;            // per colloid: impulse mean (6) and log-variance (6)
+    const float *sur_lo, *sur_hi;  // feature range of the training data (144 each), then
+                                   // impulse range (6 each) at offset N_FEAT
 };
 
 }  // namespace bw

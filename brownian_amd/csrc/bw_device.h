@@ -28,7 +28,7 @@ __device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
     return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
 }
 
-enum : uint32_t { KIND_SHIFT = 1, KIND_AXIS = 2, KIND_REFL = 3 };
+enum : uint32_t { KIND_SHIFT = 1, KIND_AXIS = 2, KIND_REFL = 3, KIND_SUR = 5 };
 
 __device__ __forceinline__ uint2 rep_key(const Rep &R) { return make_uint2(R.seed_lo, R.seed_hi); }
 
@@ -108,10 +108,14 @@ __device__ __forceinline__ double img1(double a, double L) { return a - L * roun
 __device__ __forceinline__ double mod1(double a, double L) { return a - L * round((a - L / 2) / L); }
 
 constexpr int CELL_MASK = (1 << CELL_BITS) - 1;
+// bit 30 of the packed cell: the particle is inside a colloid that was a
+// surrogate-driven phantom when it got there; it collides with no colloid
+// until a detection finds it inside none
+constexpr int PHANTOM = 1 << 30;
 __device__ __forceinline__ void unpack_cell(int c, int &ix, int &iy, int &iz) {
     ix = c & CELL_MASK;
     iy = (c >> CELL_BITS) & CELL_MASK;
-    iz = c >> (2 * CELL_BITS);
+    iz = (c >> (2 * CELL_BITS)) & CELL_MASK;
 }
 __device__ __forceinline__ int pack_cell(int ix, int iy, int iz) {
     return ix | (iy << CELL_BITS) | (iz << (2 * CELL_BITS));
@@ -275,7 +279,7 @@ __device__ __forceinline__ bool near_interval(int c, int lo, int len, int L) {
 // replica's lists stale and every workgroup then rebuilds its own list.
 constexpr int TILE_LIST_REACH = TILE_REACH + 4;
 
-__device__ inline void build_tile_list(const Dev &d, const Rep &R, int T) {
+__device__ inline void build_tile_list(const Phys &ph, const Dev &d, const Rep &R, int T) {
     __shared__ int s_wtot[16];
     __shared__ int s_n;
     const int lt = T - R.first_tile;
@@ -289,7 +293,7 @@ __device__ inline void build_tile_list(const Dev &d, const Rep &R, int T) {
         int i = base + threadIdx.x;
         int g = R.first_colloid + i;
         bool p = false;
-        if (i < R.nc) {
+        if (i < R.nc && !(ph.surrogate && d.cmode[g])) {  // surrogate colloids touch no fluid
             p = near_interval(d.ccell[3 * g], lox, lnx, R.L) && near_interval(d.ccell[3 * g + 1], loy, lny, R.L) &&
                 near_interval(d.ccell[3 * g + 2], loz, lnz, R.L);
             p = p || (near_interval(d.ccell_run[3 * g], lox, lnx, R.L) &&
@@ -330,7 +334,7 @@ struct NearColl {
 
 __device__ inline void load_tile_colloids(const Dev &d, const Phys &ph, const Rep &R, int r, int T, bool kick_data,
                                           TileColl &tc) {
-    if (d.tcl_stale[r]) build_tile_list(d, R, T);
+    if (d.tcl_stale[r]) build_tile_list(ph, d, R, T);
     const int n = min(d.tcl_cnt[T], TCL_CAP);
     for (int slot = threadIdx.x; slot < n; slot += blockDim.x) {
         int g = d.tcl_list[T * TCL_CAP + slot];
@@ -441,10 +445,18 @@ __device__ __forceinline__ void for_candidates(const TileColl &tc, unsigned m, F
 
 // cpp/fluid_colloid_collision.cpp:46-47 (the sums dv, dw were formed in the
 // reference's event order)
+// A surrogate-driven colloid receives its sampled impulse per substep
+// instead (it has no collision events); the event sums are also accumulated
+// for the surrogate's training pairs.
 __device__ inline void colloid_apply(const Phys &ph, const Dev &d, int g) {
+    const bool sur = ph.surrogate && d.cmode[g];
     for (int k = 0; k < 3; k++) {
-        d.cvel[3 * g + k] = d.cvel[3 * g + k] + d.cdv[3 * g + k] * ph.mass_fl / ph.mass_colloid;
-        d.comega[3 * g + k] = d.comega[3 * g + k] + d.cdw[3 * g + k] * ph.mass_fl / ph.I_colloid;
+        const double dv = sur ? d.csur[6 * g + k] : d.cdv[3 * g + k];
+        const double dw = sur ? d.csur[6 * g + 3 + k] : d.cdw[3 * g + k];
+        d.cvel[3 * g + k] = d.cvel[3 * g + k] + dv * ph.mass_fl / ph.mass_colloid;
+        d.comega[3 * g + k] = d.comega[3 * g + k] + dw * ph.mass_fl / ph.I_colloid;
+        d.cimp[6 * g + k] += d.cdv[3 * g + k];
+        d.cimp[6 * g + 3 + k] += d.cdw[3 * g + k];
         d.cdv[3 * g + k] = 0;
         d.cdw[3 * g + k] = 0;
     }

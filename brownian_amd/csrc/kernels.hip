@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_md_candidates(Phys ph, Dev d, const int
 __global__ __launch_bounds__(256) void k_tile_colloids(Phys ph, Dev d) {
     const int T = blockIdx.x;
     const int r = d.tile_rep[T];
-    build_tile_list(d, d.rep[r], T);
+    build_tile_list(ph, d, d.rep[r], T);
     if (threadIdx.x == 0) d.tcl_stale[r] = 0;
 }
 
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     for (int w = threadIdx.x; w < 4 * ncell; w += blockDim.x) {
         const int comp = w / ncell, c = w - comp * ncell;
         const int b = s_start[c], e = s_start[c + 1], N = e - b;
-        if (N > 1) {
+        if (N > 1 || (N == 1 && comp < 3 && ph.moments)) {  // one-particle means: moments only
             if (comp < 3) {
                 const Real *sv = s_v + comp * cap;
                 double u = 0;
@@ -411,6 +411,30 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
         }
     }
     if (zero_var) atomicOr(d.status, BW_FLAG_ZERO_VARIANCE);
+    if (ph.moments) {
+        // coarse-cell moments (2x2x2 shifted cells) for the surrogate, from the
+        // per-cell counts and means: momentum N u, and kinetic energy N |u|^2
+        // plus the thermostat's relative energy 3 (N - 1) kT / m, summed in a
+        // fixed order (even tile edges, checked at creation)
+        const int C0 = B0 / 2, C1 = B1 / 2, C2 = B2 / 2, Lc = L / 2;
+        for (int q = threadIdx.x; q < C0 * C1 * C2; q += blockDim.x) {
+            const int qx = q % C0, qy = (q / C0) % C1, qz = q / (C0 * C1);
+            double n = 0, px = 0, py = 0, pz = 0, en = 0;
+            for (int o = 0; o < 8; o++) {
+                const int c = (2 * qx + (o & 1)) + B0 * ((2 * qy + ((o >> 1) & 1)) + B1 * (2 * qz + (o >> 2)));
+                const int N = s_start[c + 1] - s_start[c];
+                if (N < 1) continue;
+                const double ux = s_u[3 * c], uy = s_u[3 * c + 1], uz = s_u[3 * c + 2];
+                n += N;
+                px += N * ux; py += N * uy; pz += N * uz;
+                en += N * ((ux * ux + uy * uy) + uz * uz) + (N > 1 ? 3.0 * (N - 1) * ph.kbt / ph.mass_fl : 0.0);
+            }
+            const long long gq = R.first_coarse + (ox0 / 2 + qx) + (long long)Lc * ((oy0 / 2 + qy) + (long long)Lc * (oz0 / 2 + qz));
+            float *mo = d.cmom + 5 * gq;
+            mo[0] = (float)n; mo[1] = (float)px; mo[2] = (float)py; mo[3] = (float)pz; mo[4] = (float)en;
+        }
+        if (threadIdx.x == 0 && T == 0) *d.mom_valid = 1;
+    }
     __syncthreads();
     // apply; count run() shells; near test
     unsigned nearbits = 0;
@@ -446,6 +470,14 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
                 for_candidates(tc, m, [&](int q) {
                     if (q < tc.n && shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
                 });
+                if (ph.surrogate) {  // inside a colloid that was a phantom in the last step
+                    const double Ld = (double)L;
+                    for_candidates(tc, m, [&](int q) {
+                        if (q >= tc.n || !d.cmode_prev[tc.g[q]]) return;
+                        double ax = img1(x - tc.px[q], Ld), ay = img1(y - tc.py[q], Ld), az = img1(z - tc.pz[q], Ld);
+                        if ((ax * ax + ay * ay) + az * az <= ph.lim_coll) rcell[k] |= PHANTOM;
+                    });
+                }
                 if (NEAR_TEST) {
                     // conservative test in FP32, tile-local coordinates (2e-2 cell slack)
                     const float Lf = (float)L, iLf = 1.0f / Lf;
@@ -535,7 +567,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
                             normalise(iz, oz, L);
                         }
                     }
-                    cell = pack_cell(ix, iy, iz);
+                    cell = pack_cell(ix, iy, iz) | (rcell[k] & PHANTOM);
                     if (!(near_interval(ix, ox0 - TILE_REACH, B0 + 2 * TILE_REACH, L) &&
                           near_interval(iy, oy0 - TILE_REACH, B1 + 2 * TILE_REACH, L) &&
                           near_interval(iz, oz0 - TILE_REACH, B2 + 2 * TILE_REACH, L)))
@@ -585,6 +617,7 @@ struct StreamOp {
     __device__ __forceinline__ void operator()(int slot, int c, Real ox, Real oy, Real oz, Real vx, Real vy, Real vz) {
         int ix, iy, iz;
         unpack_cell(c, ix, iy, iz);
+        const int phantom = c & PHANTOM;
         const double Ld = (double)L;
         if (KICK) {
             unsigned m = cell_mask(*mkick, rg, ix, iy, iz, L);
@@ -614,14 +647,17 @@ struct StreamOp {
             normalise(ix, ox, L);
             normalise(iy, oy, L);
             normalise(iz, oz, L);
-            f.cell[slot] = pack_cell(ix, iy, iz);
+            f.cell[slot] = pack_cell(ix, iy, iz) | phantom;
             f.ox[slot] = ox; f.oy[slot] = oy; f.oz[slot] = oz;
             if (!(near_interval(ix, lox, lnx, L) && near_interval(iy, loy, lny, L) && near_interval(iz, loz, lnz, L)))
                 drift = true;
         }
         if (DETECT) {
             unsigned m = cell_mask(*mdet, rg, ix, iy, iz, L);
-            if (m == 0 && tc.n <= MASK_BITS) return;
+            if (m == 0 && tc.n <= MASK_BITS) {
+                if (phantom) f.cell[slot] = pack_cell(ix, iy, iz);
+                return;
+            }
             double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
             int n_in = 0, gin = -1;
             for_candidates(tc, m, [&](int k) {
@@ -632,6 +668,10 @@ struct StreamOp {
                     gin = tc.g[k];
                 }
             });
+            if (phantom) {  // no events until it is inside no colloid
+                if (n_in == 0) f.cell[slot] = pack_cell(ix, iy, iz);
+                return;
+            }
             if (n_in > 0) {
                 int e = n_in == 1 ? atomicAdd(&d.ev_cnt[gin], 1) : EV_CAP;
                 if (e < EV_CAP) {
@@ -804,7 +844,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             const double ax = (double)ix + bx, ay = (double)iy + by, az = (double)iz + bz;
             for (int m = 0; m < d.md_cnt[g]; m++) {
                 int j = d.md_list[g * MD_CAP + m];
-                if (j <= g) continue;
+                if (j <= g || (ph.surrogate && d.cmode[j])) continue;
                 if (!in_stencil(ix, iy, iz, d.ccell[3 * j], d.ccell[3 * j + 1], d.ccell[3 * j + 2], L)) continue;
                 double rx = img1(d.cpos[3 * j] - ax, Ld), ry = img1(d.cpos[3 * j + 1] - ay, Ld),
                        rz = img1(d.cpos[3 * j + 2] - az, Ld);
@@ -898,7 +938,8 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             __syncthreads();
             for (int base = from; base < gend; base += blockDim.x) {
                 int gg = base + threadIdx.x;
-                bool p = gg < gend && in_stencil(cx, cy, cz, d.ccell[3 * gg], d.ccell[3 * gg + 1], d.ccell[3 * gg + 2], L);
+                bool p = gg < gend && !(ph.surrogate && d.cmode[gg]) &&
+                         in_stencil(cx, cy, cz, d.ccell[3 * gg], d.ccell[3 * gg + 1], d.ccell[3 * gg + 2], L);
                 int tot;
                 int idx = block_compact(p, s_wt, tot);
                 if (p && s_cand_n + idx < TCL_CAP) s_cand[s_cand_n + idx] = gg;
@@ -1188,6 +1229,136 @@ __global__ void k_gather(Phys ph, Dev d, Fluid<Real> f, double *pos, double *vel
     }
 }
 
+// ---------------------------------------------------------- surrogate
+// Features of colloid g at the start of a step, from the coarse moments of
+// the last SRD pass (its shift is still in d.shift):
+//   27 coarse cells around the colloid, x fastest: density n/(8 density),
+//   mean velocity relative to the colloid (3), temperature (135 values);
+//   then the colloid's velocity (3), angular velocity (3), orientation (3).
+// The colloid is driven by the surrogate in this step when surrogate mode is
+// on, the moments are valid and every feature lies in the training range;
+// otherwise it collides explicitly.  Resets the impulse accumulator.
+__global__ void k_features(Phys ph, Dev d, const int *colloid_rep) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ph.n_colloid_total) return;
+    const int r = colloid_rep[g];
+    const Rep R = d.rep[r];
+    const int L = R.L, Lc = L / 2;
+    const double Ld = (double)L;
+    const bool valid = *d.mom_valid != 0;
+    float *fo = d.feat + (long long)g * N_FEAT;
+    bool in = ph.surrogate && valid;
+    auto put = [&](int k, double v) {
+        const float x = (float)v;
+        fo[k] = x;
+        if (ph.surrogate && !(x >= d.sur_lo[k] && x <= d.sur_hi[k])) in = false;
+    };
+    int c3[3];
+    for (int k = 0; k < 3; k++) c3[k] = coord_cell(mod1(d.cpos[3 * g + k] + d.shift[3 * r + k], Ld), L) / 2;
+    const double vcx = d.cvel[3 * g], vcy = d.cvel[3 * g + 1], vcz = d.cvel[3 * g + 2];
+    const double rho0 = 8.0 * (double)R.nf / ((double)L * L * L);
+    for (int o = 0; o < 27; o++) {
+        const int qx = wrapi(c3[0] + o % 3 - 1, Lc), qy = wrapi(c3[1] + (o / 3) % 3 - 1, Lc),
+                  qz = wrapi(c3[2] + o / 9 - 1, Lc);
+        const float *mo = d.cmom + 5 * (R.first_coarse + qx + (long long)Lc * (qy + (long long)Lc * qz));
+        const double n = valid ? mo[0] : 0.0;
+        double ux = 0, uy = 0, uz = 0, t = 0;
+        if (n >= 1) {
+            ux = mo[1] / n; uy = mo[2] / n; uz = mo[3] / n;
+            if (n >= 2) t = ph.mass_fl * (mo[4] - n * ((ux * ux + uy * uy) + uz * uz)) / (3.0 * (n - 1));
+            ux -= vcx; uy -= vcy; uz -= vcz;
+        }
+        put(5 * o, n / rho0);
+        put(5 * o + 1, ux); put(5 * o + 2, uy); put(5 * o + 3, uz);
+        put(5 * o + 4, t);
+    }
+    for (int k = 0; k < 3; k++) {
+        put(135 + k, d.cvel[3 * g + k]);
+        put(138 + k, d.comega[3 * g + k]);
+        put(141 + k, d.cra[3 * g + k]);
+    }
+    d.cmode_prev[g] = d.cmode[g];
+    d.cmode[g] = in ? 1 : 0;
+    for (int k = 0; k < 6; k++) d.cimp[6 * g + k] = 0.0;
+}
+
+// impulse of a surrogate-driven colloid over the coming step: mean + sd * xi,
+// xi standard normal (Box-Muller on Philox draws keyed by step and colloid),
+// clamped to the training range of the impulses, applied in n_sub equal parts
+__global__ void k_sur_sample(Phys ph, Dev d, const int *colloid_rep) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ph.n_colloid_total) return;
+    const Rep R = d.rep[colloid_rep[g]];
+    const long long step = *d.step + 1;
+    const uint32_t jl = (uint32_t)(g - R.first_colloid);
+    double xi[6];
+    for (int h = 0; h < 3; h++) {
+        uint4 a = philox(make_uint4(KIND_SUR | ((uint32_t)h << 8), (uint32_t)step, jl, (uint32_t)(step >> 32)), rep_key(R));
+        double u1 = u01(a.x, a.y), u2 = u01(a.z, a.w);
+        double rr = sqrt(-2.0 * log(1.0 - u1)), ph2 = 4.0 * asin(1.0) * u2;
+        xi[2 * h] = rr * cos(ph2);
+        xi[2 * h + 1] = rr * sin(ph2);
+    }
+    for (int k = 0; k < 6; k++) {
+        double imp = 0.0;
+        if (d.cmode[g]) {
+            const double mu = d.sur_out[(long long)g * 12 + k], lv = d.sur_out[(long long)g * 12 + 6 + k];
+            // never more than the explicit collisions gave in training
+            const double v = fmin(fmax(mu + exp(0.5 * lv) * xi[k], (double)d.sur_lo[N_FEAT + k]),
+                                  (double)d.sur_hi[N_FEAT + k]);
+            imp = v / ph.n_sub;
+        }
+        d.csur[6 * g + k] = imp;
+    }
+}
+
+// When the surrogate is detached: every particle inside a colloid that is a
+// phantom now is marked, per tile (the tile's phantom colloids are listed as
+// for the tile colloid lists)
+template <class Real>
+__global__ __launch_bounds__(256) void k_mark_phantoms(Phys ph, Dev d, Fluid<Real> f) {
+    __shared__ int s_g[TCL_CAP];
+    __shared__ int s_n, s_wtot[16];
+    const int T = blockIdx.x;
+    Rep R;
+    TileGeo tg = tile_geo(d, T, R);
+    const int L = tg.L;
+    const double Ld = (double)L;
+    const int reach = TILE_LIST_REACH;
+    const int lox = tg.tx * R.B[0] - reach, loy = tg.ty * R.B[1] - reach, loz = tg.tz * R.B[2] - reach;
+    const int lnx = R.B[0] + 2 * reach, lny = R.B[1] + 2 * reach, lnz = R.B[2] + 2 * reach;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (int base = 0; base < R.nc; base += blockDim.x) {
+        const int i = base + threadIdx.x, g = R.first_colloid + i;
+        const bool p = i < R.nc && d.cmode[g] && near_interval(d.ccell[3 * g], lox, lnx, L) &&
+                       near_interval(d.ccell[3 * g + 1], loy, lny, L) && near_interval(d.ccell[3 * g + 2], loz, lnz, L);
+        int tot;
+        const int idx = block_compact(p, s_wtot, tot);
+        if (p && s_n + idx < TCL_CAP) s_g[s_n + idx] = g;
+        __syncthreads();
+        if (threadIdx.x == 0) s_n = min(s_n + tot, TCL_CAP);
+        __syncthreads();
+    }
+    const int n = min(d.count[T], ph.cap), ng = s_n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int slot = T * ph.cap + i;
+        const int c = f.cell[slot];
+        int ix, iy, iz;
+        unpack_cell(c, ix, iy, iz);
+        const double x = (double)ix + (double)f.ox[slot], y = (double)iy + (double)f.oy[slot],
+                     z = (double)iz + (double)f.oz[slot];
+        for (int q = 0; q < ng; q++) {
+            const double *pc = d.cpos + 3 * s_g[q];
+            const double ax = img1(x - pc[0], Ld), ay = img1(y - pc[1], Ld), az = img1(z - pc[2], Ld);
+            if ((ax * ax + ay * ay) + az * az <= ph.lim_coll) {
+                f.cell[slot] = c | PHANTOM;
+                break;
+            }
+        }
+    }
+}
+
 // plain copy for HBM counter calibration
 __global__ void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1213,14 +1384,12 @@ void launch_depart(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream
 template <class Real, bool C, bool M, bool B>
 static void srd_one(const Phys &ph, const Dev &d, const Fluid<Real> &src, const Fluid<Real> &dst, size_t lds,
                     hipStream_t st) {
-    static bool attr = false;
-    if (!attr) {
-        int bytes = 160 * 1024 - (int)srd_static_lds();
-        (void)hipFuncSetAttribute((const void *)k_srd<Real, C, M, B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        attr = true;
-    }
     LAUNCH((k_srd<Real, C, M, B>), ph.n_tiles, SRD_THREADS, lds, st, ph, d, src, dst);
 }
+// Dynamic LDS up to the CU's 160 KB is granted at launch on gfx950 without an
+// opt-in attribute; the context checks the SRD pass's need against
+// 160 KB minus its static LDS when it is created.
+hipError_t prepare_kernels() { return hipSuccess; }
 // count_shell: run() shell counts (colloids present); map: shifted-cell map of
 // every particle (the standalone rotation_mpcd); block: near/far split of the
 // fused step
@@ -1293,6 +1462,18 @@ void launch_md_candidates(const Phys &ph, const Dev &d, const int2 *wgt, int nwg
 void launch_tile_colloids(const Phys &ph, const Dev &d, hipStream_t st) {
     LAUNCH(k_tile_colloids, ph.n_tiles, 256, 0, st, ph, d);
 }
+void launch_features(const Phys &ph, const Dev &d, const int *colloid_rep, hipStream_t st) {
+    LAUNCH(k_features, (ph.n_colloid_total + 63) / 64, 64, 0, st, ph, d, colloid_rep);
+}
+void launch_sur_sample(const Phys &ph, const Dev &d, const int *colloid_rep, hipStream_t st) {
+    LAUNCH(k_sur_sample, (ph.n_colloid_total + 63) / 64, 64, 0, st, ph, d, colloid_rep);
+}
+template <class Real>
+void launch_mark_phantoms(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream_t st) {
+    LAUNCH(k_mark_phantoms<Real>, ph.n_tiles, 256, 0, st, ph, d, f);
+}
+template void launch_mark_phantoms<float>(const Phys &, const Dev &, const Fluid<float> &, hipStream_t);
+template void launch_mark_phantoms<double>(const Phys &, const Dev &, const Fluid<double> &, hipStream_t);
 void launch_copy(const void *a, void *b, long long nbytes, hipStream_t st) {
     long long n = nbytes / 16;
     LAUNCH(k_copy, 2048, 256, 0, st, (const float4 *)a, (float4 *)b, n);

@@ -224,6 +224,47 @@ BW_API int bw_debug_copy(bw_ctx *ctx, long long nbytes, int reps, double *ms);
 BW_API int bw_observables_reset(bw_ctx *ctx);
 BW_API int bw_observables(bw_ctx *ctx, int replica, double *out6);
 
+/* ------------------------------------------------------------ surrogate */
+/* No counterpart in the reference.  A learned closure for the colloid-fluid
+ * collisions (cpp/fluid_colloid_collision.cpp:26-49): a network predicts each
+ * colloid's impulse over one MPCD step (mean and log-variance of momentum and
+ * angular momentum transfer, 6 each) from 144 features (27 coarse-cell fluid
+ * moments around the colloid and its velocity, angular velocity, orientation).
+ *
+ * Network: 144 -> 256 -> 256 -> 256 -> 12, SiLU; BF16 matrix-core products with
+ * FP32 accumulation; FP32 master weights; Gaussian NLL; Adam. */
+typedef struct bw_mlp bw_mlp;
+BW_API int bw_mlp_create(int seed, bw_mlp **out);
+BW_API int bw_mlp_destroy(bw_mlp *m);
+BW_API const char *bw_mlp_last_error(bw_mlp *m);
+/* parameters, logical layout: W1 (144x256, row-major), b1, W2 (256x256), b2,
+ * W3, b3, W4 (256x12), b4 */
+BW_API long long bw_mlp_n_params(void);
+BW_API int bw_mlp_get_params(bw_mlp *m, float *out);
+BW_API int bw_mlp_set_params(bw_mlp *m, const float *in);
+/* standardisation of features (144) and targets (6) */
+BW_API int bw_mlp_set_normalisation(bw_mlp *m, const float *x_mean, const float *x_std, const float *t_mean,
+                                    const float *t_std);
+/* standardised outputs (n x 12) for raw features x (n x 144) */
+BW_API int bw_mlp_forward(bw_mlp *m, const float *x, int n, float *y);
+/* mean loss over n samples (targets t: n x 6 raw) and its parameter gradient */
+BW_API int bw_mlp_loss_grad(bw_mlp *m, const float *x, const float *t, int n, double *loss, float *grad);
+/* Adam over shuffled mini-batches; loss_out[epoch] = mean loss of the epoch */
+BW_API int bw_mlp_train(bw_mlp *m, const float *x, const float *t, int n, int epochs, int batch, float lr,
+                        int seed, double *loss_out);
+/* the rotation pass also writes coarse-cell moments (needs even edges) */
+BW_API int bw_set_moments(bw_ctx *ctx, int on);
+/* drive colloids by the network's sampled impulses wherever every feature lies
+ * in [lo, hi] (144 each); elsewhere, and in the first step, the explicit
+ * collisions are kept.  Sampled impulses are clamped to [t_lo, t_hi] (6 each,
+ * the training range).  NULL detaches.  The network must stay alive and must
+ * not be trained while attached. */
+BW_API int bw_surrogate_attach(bw_ctx *ctx, bw_mlp *m, const float *lo, const float *hi, const float *t_lo,
+                               const float *t_hi);
+/* after a fused step: features at its start (n_colloid x 144), collision
+ * impulse over it (n_colloid x 6) and mode (0 explicit, 1 surrogate) */
+BW_API int bw_get_surrogate_data(bw_ctx *ctx, int replica, float *feat, double *imp, int *mode);
+
 /* Diagnostics (no counterpart in the reference): particles of replica r that
  * the last fused step's per-substep kernels touched ("near" a colloid under
  * blocked streaming; all particles under per-substep streaming). */

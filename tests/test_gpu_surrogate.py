@@ -1,0 +1,126 @@
+"""GPU: the surrogate network (BF16 matrix-core forward and backward) against
+its CPU reference, its training, and the surrogate closure inside the step."""
+import numpy as np
+import pytest
+
+import brownian_amd as B
+from brownian_amd import surrogate as S
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(rng, n=100):
+    net = B.MLP(seed=5)
+    layers = [(rng.normal(0, 1.0 / np.sqrt(k), (k, m)), rng.normal(0, 0.1, m)) for k, m in M.SIZES]
+    p = M.pack(layers).astype(np.float32)
+    net.set_params(p)
+    xm, xs = rng.normal(0, 0.1, 144).astype(np.float32), rng.uniform(0.5, 2, 144).astype(np.float32)
+    tm, ts = rng.normal(0, 0.1, 6).astype(np.float32), rng.uniform(0.5, 2, 6).astype(np.float32)
+    net.set_normalisation(xm, xs, tm, ts)
+    x = rng.normal(0, 1, (n, 144)).astype(np.float32)
+    t = rng.normal(0, 1, (n, 6)).astype(np.float32)
+    return net, p, x, t, xm, xs, tm, ts
+
+
+def test_parameters_round_trip():
+    net = B.MLP(seed=3)
+    p = net.params()
+    assert p.shape == (M.pack([(np.zeros((k, n)), np.zeros(n)) for k, n in M.SIZES]).size,)
+    q = np.random.default_rng(0).normal(0, 1, p.size).astype(np.float32)
+    net.set_params(q)
+    np.testing.assert_array_equal(net.params(), q)
+
+
+def test_forward_matches_cpu_reference():
+    rng = np.random.default_rng(11)
+    net, p, x, _, xm, xs, _, _ = _net(rng, n=77)
+    y = net.forward(x)
+    ref, _ = M.forward(p, x, xm, xs, emulate_bf16=True)
+    assert np.abs(y - ref).max() <= 2e-3 * np.abs(ref).max()
+
+
+def test_gradient_matches_cpu_reference():
+    """GPU gradient (BF16 products, FP32 accumulation) against the FP64
+    reference with the same BF16 roundings; the reference itself is pinned by
+    finite differences on the CPU."""
+    rng = np.random.default_rng(12)
+    net, p, x, t, xm, xs, tm, ts = _net(rng, n=96)
+    loss, g = net.loss_grad(x, t)
+    rl, rg = M.loss_grad(p, x, t, xm, xs, tm, ts, emulate_bf16=True)
+    assert loss == pytest.approx(rl, rel=1e-4)
+    o = 0
+    for k, n in M.SIZES:
+        for sz in (k * n, n):
+            a, b = g[o:o + sz], rg[o:o + sz]
+            assert np.linalg.norm(a - b) <= 1e-2 * np.linalg.norm(b) + 1e-7, (o, sz)
+            o += sz
+
+
+def test_training_learns_a_noisy_linear_law():
+    rng = np.random.default_rng(13)
+    n = 20000
+    x = rng.normal(0, 1, (n, 144)).astype(np.float32)
+    A = rng.normal(0, 0.3, (144, 6))
+    noise = np.array([0.1, 0.2, 0.3, 0.1, 0.2, 0.3])
+    t = (x @ A + noise * rng.normal(0, 1, (n, 6))).astype(np.float32)
+    net, info = S.fit(x, t, epochs=25, batch=1024, lr=2e-3, seed=3)
+    assert info["val_loss_after"] < info["val_loss_before"] - 2.0
+    y = net.forward(x[:2000])
+    tm, tsd = t.mean(0), t.std(0)
+    mu = tm + tsd * y[:, :6]
+    r2 = 1 - ((mu - t[:2000]) ** 2).sum(0) / ((t[:2000] - tm) ** 2).sum(0)
+    assert r2.min() > 0.9
+
+
+PRM = dict(L=16, density=10, n_colloid=8, sig_colloid=4.2, sigma=4.0, mass_colloid=50.0, n_sub=20,
+           theta_deg=90.0, v0=0.03, seed=4242, precision=B.BW_FP32, vel_law=B.BW_VEL_MAXWELL,
+           orient_law=B.BW_ORIENT_SPHERE)
+
+
+def test_out_of_range_surrogate_is_the_explicit_step():
+    """Attached with an empty training range every colloid keeps its explicit
+    collisions: the trajectory is bit-identical to the plain run."""
+    a = B.System(**PRM)
+    b = B.System(**PRM)
+    a.startup(); b.startup()
+    net = B.MLP(seed=1)
+    lo = np.full(144, np.inf, np.float32)
+    hi = np.full(144, -np.inf, np.float32)
+    b.surrogate_attach(net, lo, hi, np.zeros(6), np.zeros(6))
+    a.step(4); b.step(4)
+    assert a.status() == 0 and b.status() == 0
+    _, _, mode = b.surrogate_data()
+    assert not mode.any()
+    pa, va, _ = a.download_fluid()
+    pb, vb, _ = b.download_fluid()
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(va, vb)
+    ca, cb = a.download_colloid(), b.download_colloid()
+    for k in ca:
+        np.testing.assert_array_equal(ca[k], cb[k], err_msg=k)
+
+
+def test_surrogate_drives_colloids_inside_the_training_range():
+    s = B.System(**PRM)
+    s.startup()
+    X, T = S.collect(s, 100)
+    assert X.shape == (8 * 100, 144) and np.isfinite(X).all() and np.abs(T).sum() > 0
+    # a few hundred pairs: this checks the closure's plumbing, not the model
+    net, info = S.fit(X, T, epochs=10, batch=64, lr=1e-3, seed=2)
+    assert info["train_loss"][-1] < info["train_loss"][0]
+    plain_near = s.near_particles()
+    S.attach(s, net, info)
+    s.step(6)
+    assert s.status() == 0
+    _, imp, mode = s.surrogate_data()
+    assert mode.sum() > 0                       # inside the range: driven by the network
+    assert np.all(imp[mode == 1] == 0)          # no collision events for those colloids
+    c = s.download_colloid()
+    assert np.isfinite(c["vel"]).all() and np.isfinite(c["omega"]).all()
+    assert s.near_particles() < plain_near      # their fluid is streamed by the rotation pass
+    s.surrogate_attach(None)
+    s.step(2)
+    assert s.status() == 0
+    _, _, mode = s.surrogate_data()
+    assert not mode.any()
