@@ -208,6 +208,53 @@ def run_ensemble(B, n_rep, steps, warmup):
             "near_fraction": near / nf, "setup_s": t_setup}
 
 
+def run_surrogate(B, n_rep, collect_steps, epochs, steps):
+    """Plain vs surrogate-driven ensemble from one start: colloid-steps/s and
+    the pooled colloid mean squared displacement after `steps` steps.  The
+    network is trained on pairs from an ensemble with disjoint seeds."""
+    from brownian_amd import surrogate as S
+    common = dict(precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
+                  orient_law=B.BW_ORIENT_SPHERE)
+    t0 = time.time()
+    tr = B.System(replicas=[dict(r, **common) for r in ensemble_params(n_rep, seed0=5000)])
+    tr.startup()
+    X, T = S.collect(tr, collect_steps)
+    del tr
+    log(f"surrogate: {len(X):,} training pairs collected ({time.time() - t0:.1f} s)")
+    t1 = time.time()
+    net, info = S.fit(X, T, epochs=epochs, batch=4096, lr=1e-3, seed=1)
+    log(f"surrogate: trained {epochs} epochs in {time.time() - t1:.1f} s, held-out NLL "
+        f"{info['val_loss_before']:.3f} -> {info['val_loss_after']:.3f}")
+    reps = [dict(r, **common) for r in ensemble_params(n_rep)]
+    out = {"training_pairs": int(len(X)), "epochs": epochs, "val_nll_before": info["val_loss_before"],
+           "val_nll_after": info["val_loss_after"], "train_nll": info["train_loss"]}
+    res = {}
+    for name in ("plain", "surrogate"):
+        s = B.System(replicas=reps)
+        s.startup()
+        if name == "surrogate":
+            S.attach(s, net, info)
+        s.step(3)
+        s.status()
+        s.observables_reset()
+        ms = s.time_steps(steps)
+        s.status()
+        nc = np.array([i["nc"] for i in s.info])
+        msd = np.array([s.observables(r)["msd"] for r in range(n_rep)])
+        frac = None
+        if name == "surrogate":
+            frac = float(np.mean(np.concatenate([s.surrogate_data(r)[2] for r in range(n_rep)])))
+        res[name] = {"ms_per_step": ms / steps, "colloid_steps_per_s": nc.sum() / (ms / 1e3 / steps),
+                     "msd": float((nc * msd).sum() / nc.sum()), "surrogate_fraction": frac}
+        log(f"surrogate: {name} {ms / steps:.3f} ms/step, pooled MSD {res[name]['msd']:.4f}")
+        del s
+    out.update(res)
+    out["speedup_colloid_steps"] = res["surrogate"]["colloid_steps_per_s"] / res["plain"]["colloid_steps_per_s"]
+    out["msd_ratio"] = res["surrogate"]["msd"] / res["plain"]["msd"]
+    out["msd_lag_steps"] = steps
+    return out
+
+
 # ------------------------------------------------------------------- bench
 def main():
     ap = argparse.ArgumentParser()
@@ -224,6 +271,10 @@ def main():
     ap.add_argument("--ensemble", type=int, default=256, help="replicas of the batched run (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=50)
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True)
+    ap.add_argument("--surrogate", action=argparse.BooleanOptionalAction, default=True)
+    ap.add_argument("--surrogate-collect", type=int, default=40)
+    ap.add_argument("--surrogate-epochs", type=int, default=8)
+    ap.add_argument("--surrogate-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
@@ -332,6 +383,12 @@ def main():
             ens = run_ensemble(B, args.ensemble, args.ensemble_steps, 5)
         except Exception as e:  # reported, the single-system figure stands
             ens = {"error": repr(e)}
+        if args.surrogate:
+            try:
+                ens["surrogate"] = run_surrogate(B, args.ensemble, args.surrogate_collect, args.surrogate_epochs,
+                                                 args.surrogate_steps)
+            except Exception as e:
+                ens["surrogate"] = {"error": repr(e)}
     ach = kern[rk]["achieved_GBps"] * 1e9
     out = {
         "metric": METRIC,

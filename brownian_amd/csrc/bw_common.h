@@ -112,6 +112,7 @@ struct Dev {
     int *crep, *ccell_run, *ccell_md, *ccnt;
     int *md_stale, *tcl_stale;  // per replica: pair candidates / tile lists outdated
     int *tcl_cnt, *tcl_list;  // per tile: colloids that can touch it this step
+    int *tsl_cnt, *tsl_list;  // per tile: surrogate-driven colloids near it this step
     int *md_cnt, *md_list, *mdc_cnt, *mdc_list;
     double *cpot;        // [n_rep] potential
     int *ev_cnt, *ev_slot, *chain_cnt, *chain_slot;

@@ -279,9 +279,12 @@ __device__ __forceinline__ bool near_interval(int c, int lo, int len, int L) {
 // replica's lists stale and every workgroup then rebuilds its own list.
 constexpr int TILE_LIST_REACH = TILE_REACH + 4;
 
-__device__ inline void build_tile_list(const Phys &ph, const Dev &d, const Rep &R, int T) {
+// sur = false: the colloids that collide explicitly (tcl_*); sur = true: the
+// surrogate-driven ones, whose fluid the rotation pass keeps out (tsl_*)
+__device__ inline void build_tile_list(const Phys &ph, const Dev &d, const Rep &R, int T, bool sur = false) {
     __shared__ int s_wtot[16];
     __shared__ int s_n;
+    int *list = sur ? d.tsl_list : d.tcl_list;
     const int lt = T - R.first_tile;
     const int tx = lt % R.nt[0], ty = (lt / R.nt[0]) % R.nt[1], tz = lt / (R.nt[0] * R.nt[1]);
     const int reach = TILE_LIST_REACH;
@@ -293,7 +296,7 @@ __device__ inline void build_tile_list(const Phys &ph, const Dev &d, const Rep &
         int i = base + threadIdx.x;
         int g = R.first_colloid + i;
         bool p = false;
-        if (i < R.nc && !(ph.surrogate && d.cmode[g])) {  // surrogate colloids touch no fluid
+        if (i < R.nc && (ph.surrogate && d.cmode[g]) == sur) {
             p = near_interval(d.ccell[3 * g], lox, lnx, R.L) && near_interval(d.ccell[3 * g + 1], loy, lny, R.L) &&
                 near_interval(d.ccell[3 * g + 2], loz, lnz, R.L);
             p = p || (near_interval(d.ccell_run[3 * g], lox, lnx, R.L) &&
@@ -303,14 +306,14 @@ __device__ inline void build_tile_list(const Phys &ph, const Dev &d, const Rep &
         int tot;
         int idx = block_compact(p, s_wtot, tot);
         if (p) {
-            if (s_n + idx < TCL_CAP) d.tcl_list[T * TCL_CAP + s_n + idx] = g;
+            if (s_n + idx < TCL_CAP) list[T * TCL_CAP + s_n + idx] = g;
             else atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
         }
         __syncthreads();
         if (threadIdx.x == 0) s_n = min(s_n + tot, TCL_CAP);
         __syncthreads();
     }
-    if (threadIdx.x == 0) d.tcl_cnt[T] = s_n;
+    if (threadIdx.x == 0) (sur ? d.tsl_cnt : d.tcl_cnt)[T] = s_n;
     __syncthreads();
 }
 

@@ -84,6 +84,7 @@ __global__ __launch_bounds__(256) void k_tile_colloids(Phys ph, Dev d) {
     const int T = blockIdx.x;
     const int r = d.tile_rep[T];
     build_tile_list(ph, d, d.rep[r], T);
+    if (ph.surrogate) build_tile_list(ph, d, d.rep[r], T, true);
     if (threadIdx.x == 0) d.tcl_stale[r] = 0;
 }
 
@@ -214,6 +215,8 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     __shared__ int s_shell[COUNT_SHELL ? TCL_CAP : 1];
     __shared__ AxisMask s_am[COUNT_SHELL ? 2 : 1];  // [0] run shells, [1] near test
     __shared__ int s_cc[SRD_PT][SRD_THREADS / 64];   // near particles per chunk and wave
+    __shared__ double s_sp[COUNT_SHELL ? TCL_CAP : 1][6];  // surrogate colloids: position, velocity
+    __shared__ int s_sn;
     const int T = blockIdx.x;
     Rep R;
     TileGeo tg = tile_geo(d, T, R);
@@ -259,6 +262,17 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
         s_pre[27] = acc;
     }
     Region rg = tile_region(R, tg.tx, tg.ty, tg.tz);
+    if (COUNT_SHELL && ph.surrogate) {
+        const int ns = min(d.tsl_cnt[T], TCL_CAP);
+        for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+            const int g = d.tsl_list[T * TCL_CAP + q];
+            for (int k = 0; k < 3; k++) {
+                s_sp[q][k] = d.cpos[3 * g + k];
+                s_sp[q][3 + k] = d.cvel[3 * g + k];
+            }
+        }
+        if (threadIdx.x == 0) s_sn = ns;
+    }
     if (COUNT_SHELL) {
         load_tile_colloids(d, ph, R, tg.r, T, true, tc);
         build_axis_masks(tc, rg, L, true, 3, s_am[0]);
@@ -460,7 +474,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
                 rvy[k] = (Real)(uy + (v1y - uy) * sc);
                 rvz[k] = (Real)(uz + (v1z - uz) * sc);
             }
-            bool near = !BLOCK;
+            bool near = !BLOCK, moved = false;
             if (COUNT_SHELL) {
                 int ix, iy, iz;
                 unpack_cell(rcell[k], ix, iy, iz);
@@ -470,6 +484,33 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
                 for_candidates(tc, m, [&](int q) {
                     if (q < tc.n && shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
                 });
+                if (ph.surrogate) {
+                    // fluid that entered a surrogate-driven colloid during the last
+                    // step is mirrored back out through its surface, its velocity
+                    // reversed relative to the colloid's (no impulse on the colloid)
+                    const double Ld = (double)L, R2 = 2.0 * sqrt(ph.lim_coll);  // colloid diameter
+                    for (int q = 0; q < s_sn; q++) {
+                        double ax = img1(x - s_sp[q][0], Ld), ay = img1(y - s_sp[q][1], Ld), az = img1(z - s_sp[q][2], Ld);
+                        double r2 = (ax * ax + ay * ay) + az * az;
+                        if (r2 >= ph.lim_coll) continue;
+                        double rr = sqrt(r2);
+                        double sc = rr > 1e-12 ? (R2 - rr) / rr : 0.0;  // radius rr -> 2R - rr
+                        double nx = s_sp[q][0] + (rr > 1e-12 ? ax * sc : 0.5 * R2 + 1e-6);
+                        double ny = s_sp[q][1] + ay * sc, nz = s_sp[q][2] + az * sc;
+                        nx = mod1(nx, Ld); ny = mod1(ny, Ld); nz = mod1(nz, Ld);
+                        int jx = coord_cell(nx, L), jy = coord_cell(ny, L), jz = coord_cell(nz, L);
+                        rcell[k] = pack_cell(jx, jy, jz) | (rcell[k] & PHANTOM);
+                        rox[k] = (Real)(nx - jx); roy[k] = (Real)(ny - jy); roz[k] = (Real)(nz - jz);
+                        if (!(rox[k] < Real(1))) rox[k] = Real(0);
+                        if (!(roy[k] < Real(1))) roy[k] = Real(0);
+                        if (!(roz[k] < Real(1))) roz[k] = Real(0);
+                        rvx[k] = (Real)(2.0 * s_sp[q][3] - (double)rvx[k]);
+                        rvy[k] = (Real)(2.0 * s_sp[q][4] - (double)rvy[k]);
+                        rvz[k] = (Real)(2.0 * s_sp[q][5] - (double)rvz[k]);
+                        moved = true;  // streamed per substep (its near test saw the old position)
+                        break;
+                    }
+                }
                 if (ph.surrogate) {  // inside a colloid that was a phantom in the last step
                     const double Ld = (double)L;
                     for_candidates(tc, m, [&](int q) {
@@ -508,7 +549,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
                     }
                 }
             }
-            if (near) nearbits |= 1u << k;
+            if (near || moved) nearbits |= 1u << k;
         }
     }
     // output slots: near particles first, each group in gather order
@@ -660,16 +701,20 @@ struct StreamOp {
             }
             double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
             int n_in = 0, gin = -1;
+            bool inside_any = false;  // by distance alone (the stencil skips a colloid's own cell)
             for_candidates(tc, m, [&](int k) {
-                if (k >= tc.n || !in_stencil(ix, iy, iz, tc.cx[k], tc.cy[k], tc.cz[k], L)) return;
+                if (k >= tc.n) return;
                 double rx = img1(tc.px[k] - x, Ld), ry = img1(tc.py[k] - y, Ld), rz = img1(tc.pz[k] - z, Ld);
                 if ((rx * rx + ry * ry) + rz * rz <= ph.lim_coll) {
-                    n_in++;
-                    gin = tc.g[k];
+                    inside_any = true;
+                    if (in_stencil(ix, iy, iz, tc.cx[k], tc.cy[k], tc.cz[k], L)) {
+                        n_in++;
+                        gin = tc.g[k];
+                    }
                 }
             });
             if (phantom) {  // no events until it is inside no colloid
-                if (n_in == 0) f.cell[slot] = pack_cell(ix, iy, iz);
+                if (!inside_any) f.cell[slot] = pack_cell(ix, iy, iz);
                 return;
             }
             if (n_in > 0) {
