@@ -30,7 +30,9 @@ def tile_slots(ncell: int, density: int) -> int:
 def candidates(L: int, density: int, fp64: bool = False) -> list[tuple[int, int, int]]:
     """Tile shapes the library accepts for a box of edge L: each edge divides
     L and is at least 4, at most 512 cells, the slots and the SRD pass's LDS
-    (per-cell FP64 sums, per-slot staging) fit, the region fits the masks."""
+    (per-cell FP64 sums, per-slot staging) fit, the region fits the masks.
+    The SRD workgroup follows the slots (4 per thread: 256, 512 or 1024
+    threads), so smaller tiles let several workgroups share a CU."""
     divs = [b for b in range(4, min(L, 16) + 1) if L % b == 0] or [L]
     rb = 8 if fp64 else 4
     out = []
@@ -45,7 +47,7 @@ def candidates(L: int, density: int, fp64: bool = False) -> list[tuple[int, int,
                     continue
                 if any(min(b + 2 * TILE_REACH, L) > AXIS_MAX for b in (bx, by, bz)):
                     continue
-                lds = n * 56 + 3 * cap * rb + cap * 4 + (2 * n + 1) * 4 + cap * 2
+                lds = max(n * 56, cap * 4) + 3 * cap * rb + (2 * n + 1) * 4 + cap * 2
                 if lds > 160 * 1024 - 16 * 1024:
                     continue
                 out.append((bx, by, bz))

@@ -25,6 +25,8 @@ void launch_srd(const Phys &, const Dev &, const Fluid<Real> &, const Fluid<Real
 template <class Real>
 void launch_stream(const Phys &, const Dev &, const Fluid<Real> &, bool, bool, bool, bool, hipStream_t);
 template <class Real>
+void launch_stream_near(const Phys &, const Dev &, const Fluid<Real> &, bool, hipStream_t);
+template <class Real>
 void launch_collide(const Phys &, const Dev &, const Fluid<Real> &, int, const int2 *, int, hipStream_t);
 template <class Real> void launch_run_count(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
 template <class Real>
@@ -40,7 +42,7 @@ void launch_tile_colloids(const Phys &, const Dev &, hipStream_t);
 void launch_copy(const void *, void *, long long, hipStream_t);
 void launch_features(const Phys &, const Dev &, const int *, hipStream_t);
 template <class Real> void launch_mark_phantoms(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
-hipError_t prepare_kernels();
+hipError_t prepare_kernels(Phys &ph, bool fp64);
 void launch_sur_sample(const Phys &, const Dev &, const int *, hipStream_t);
 }  // namespace bw
 
@@ -331,7 +333,8 @@ int enqueue_step(bw_ctx *c, int cur, int cb, bool final, Prof *prof) {
     for (int l = 0; l < ph.n_sub; l++) {
         Dev d1 = dev_at(c, 1 - cur, cb);
         // blocked without colloids: the rotation pass did all the streaming
-        if (col || !c->block) launch_stream<Real>(ph, d1, dst, col && l == 0, true, col, true, st);
+        if (col && c->block) launch_stream_near<Real>(ph, d1, dst, l == 0, st);  // near chunks only
+        else if (col || !c->block) launch_stream<Real>(ph, d1, dst, col && l == 0, true, col, true, st);
         if (prof) prof->mark(st, BW_KERNEL_STREAM);
         if (!col) continue;
         launch_collide<Real>(ph, d1, dst, l, c->wgt, c->nwg, st);
@@ -531,6 +534,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         return BW_ERR_ARG;
     }
     ph.cap = cap;
+    ph.srd_threads = cap <= 4 * 256 ? 256 : cap <= 4 * 512 ? 512 : 1024;
     ph.n_tiles = (int)tiles;
     ph.n_colloid_total = nc;
     c->total_slots = tiles * cap;
@@ -556,8 +560,8 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         delete c;
         return BW_ERR_HIP;
     }
-    if ((e = prepare_kernels()) != hipSuccess) {
-        c->err = std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
+    if ((e = prepare_kernels(c->ph, c->fp64)) != hipSuccess) {
+        c->err = std::string("kernel occupancy query: ") + hipGetErrorString(e);
         *out = c;
         return BW_ERR_HIP;
     }
@@ -574,7 +578,11 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         if ((rc = dalloc(c, &c->counts[b], tiles))) { *out = c; return rc; }
     }
     Dev &d = c->dv;
-    if ((rc = dalloc(c, &d.near_cnt, tiles))) { *out = c; return rc; }
+    if ((rc = dalloc(c, &d.near_cnt, tiles)) || (rc = dalloc(c, &d.chunk_cnt, 1)) ||
+        (rc = dalloc(c, &d.chunk_tile, (size_t)tiles * ((cap + 63) / 64)))) {
+        *out = c;
+        return rc;
+    }
     std::vector<Rep> hrep(n_rep);
     std::vector<int> trep(tiles), crep(std::max(nc, 1));
     for (int r = 0; r < n_rep; r++) {
@@ -629,7 +637,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         (rc = dalloc(c, &d.cont_cnt, n_rep)) || (rc = dalloc(c, &d.cont_slot, (size_t)n_rep * CONT_CAP)) ||
         (rc = dalloc(c, &d.cont_from, (size_t)n_rep * CONT_CAP)) ||
         (rc = dalloc(c, &d.cont_dump, (size_t)n_rep * CONT_CAP * 3)) ||
-        (rc = dalloc(c, &d.ticket, n_rep)) || (rc = dalloc(c, &d.status, 1)) || (rc = dalloc(c, &d.step, 1)) ||
+        (rc = dalloc(c, &d.status, 1)) || (rc = dalloc(c, &d.step, 1)) ||
         (rc = dalloc(c, &d.shift, 3 * n_rep)) || (rc = dalloc(c, &d.audit_partial, (size_t)tiles * 8))) {
         *out = c;
         return rc;
@@ -872,7 +880,6 @@ BW_API int initialize(bw_ctx *c) {
     HIPCHK(c, hipMemsetAsync(c->dv.ev_cnt, 0, sizeof(int) * NC, c->st));
     int *per_rep[] = {c->dv.chain_cnt, c->dv.cont_cnt, c->dv.ovf_cnt, c->dv.ev_total, c->dv.md_stale, c->dv.tcl_stale};
     for (int *a : per_rep) HIPCHK(c, hipMemsetAsync(a, 0, sizeof(int) * c->n_rep, c->st));
-    HIPCHK(c, hipMemsetAsync(c->dv.ticket, 0, sizeof(unsigned) * c->n_rep, c->st));
     HIPCHK(c, hipMemsetAsync(c->dv.cimp, 0, sizeof(double) * 6 * NC, c->st));
     HIPCHK(c, hipMemsetAsync(c->dv.cmode, 0, sizeof(int) * NC, c->st));
     HIPCHK(c, hipMemsetAsync(c->dv.cmode_prev, 0, sizeof(int) * NC, c->st));

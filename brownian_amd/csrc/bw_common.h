@@ -24,9 +24,9 @@ constexpr int CELL_BITS = 10;       // packed cell coordinates: L <= 1023
 constexpr int MD_CAP = 128;         // full MD neighbour list per colloid
 constexpr int MDC_CAP = 256;        // MD candidates (cutoff + skin) per colloid
 constexpr int MAX_CAP = 4096;       // slots per tile upper bound
-constexpr int SRD_THREADS = 1024;
-constexpr int SRD_PT = MAX_CAP / SRD_THREADS;  // particles per thread in the SRD kernel
+constexpr int SRD_PT = 4;           // slots per thread in the SRD kernel (256, 512 or 1024 threads)
 constexpr int STREAM_THREADS = 256;
+constexpr int NEAR_BLOCKS = 2048;    // near-particle pass: at most this many workgroups of 4 waves
 constexpr int DEPART_THREADS = 256;
 constexpr int COLLIDE_THREADS = 256; // 4 colloids (one wave each) per workgroup
 constexpr int OVF_CAP = 64;         // colloids with an overflowed event bucket, per replica
@@ -73,6 +73,8 @@ struct Phys {
     int surrogate;                    // colloids may be driven by the surrogate impulses
     int n_sub;
     int cap;                          // slots per tile
+    int srd_threads;                  // SRD workgroup size (4 slots per thread)
+    int near_blocks;                  // grid of the near-particle pass
     int n_rep, n_tiles, n_colloid_total;
     int draws;                        // BW_DRAWS_*
 };
@@ -102,6 +104,8 @@ struct Dev {
     int *count;          // [n_tiles] particles in each tile (current buffer)
     int *count_next;     // [n_tiles] for the buffer being written
     int *near_cnt;       // [n_tiles] particles stored first that the substeps touch
+    int *chunk_cnt;      // [1] chunks of near particles this step
+    int *chunk_tile;     // [n_tiles * cap/64] chunk -> tile << 6 | chunk index in the tile
     int *dep_off;        // [n_tiles * 28]
     int *dep_idx;        // [n_tiles * cap]
     // colloids (global index g), FP64, 3 per colloid
@@ -119,7 +123,6 @@ struct Dev {
     int *cont_cnt, *cont_slot, *cont_from;  // continuations of the colloid loop
     double *cont_dump;
     int *ovf_cnt, *ovf_list, *ev_total;     // per replica: overflowed buckets, events seen
-    unsigned *ticket;    // [n_rep]
     unsigned *status;
     long long *step;     // MPCD step counter (device generator key)
     double *shift;       // [n_rep*3] shift of the current step

@@ -1,11 +1,26 @@
 // Device helpers of libbrownian (gfx950).  Included by the kernel sources only.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include "../../include/brownian.h"
 #include "bw_common.h"
 
 namespace bw {
+
+// A wave-uniform load through the scalar data cache (read-only use: the data
+// must not change while the kernel runs).  Many waves reading one line at a
+// kernel's start then share the scalar cache instead of each sending a vector
+// request to L2.
+template <class T>
+__device__ __forceinline__ T ld_const(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    typedef const __attribute__((address_space(4))) unsigned cword;
+    T v;
+    unsigned *o = (unsigned *)&v;
+    for (int i = 0; i < (int)(sizeof(T) / 4); i++) o[i] = ((cword *)p)[i];
+    return v;
+}
 
 // ------------------------------------------------------------------ generator
 // Counter-based generator for the device draw source: Philox4x32-10
@@ -102,8 +117,16 @@ __device__ inline bool draw_reflection(const Dev &d, const Phys &ph, const Rep &
 }
 
 // -------------------------------------------------------------- geometry
-// img(a,b) = a - b*round(a/b) with round half away from zero (cpp/parameters.hpp:74,76)
-__device__ __forceinline__ double img1(double a, double L) { return a - L * round(a / L); }
+// img(a,b) = a - b*round(a/b) with round half away from zero (cpp/parameters.hpp:74,76).
+// Without the division where round(a/L) is certain: 0 for |a| < 0.4999 L
+// (a + 0.0 turns -0 into +0, as a - L*(-0.0) does), +-1 for 0.5001 L < |a| <
+// 1.4999 L (L*(+-1) = +-L exactly).  Bit-identical to the formula.
+__device__ __forceinline__ double img1(double a, double L) {
+    const double h = fabs(a);
+    if (h < 0.4999 * L) return a + 0.0;
+    if (h > 0.5001 * L && h < 1.4999 * L) return a - copysign(L, a);
+    return a - L * round(a / L);
+}
 // mod(a,b) = a - b*round((a - b/2)/b) (cpp/parameters.hpp:75)
 __device__ __forceinline__ double mod1(double a, double L) { return a - L * round((a - L / 2) / L); }
 
@@ -133,9 +156,15 @@ __device__ __forceinline__ int pdiff(int a, int b, int L) {
     if (d < -(L / 2)) d += L;
     return d;
 }
+// a mod n in [0, n); one conditional step covers a in [-n, 2n)
 __device__ __forceinline__ int wrapi(int a, int n) {
-    a %= n;
-    return a < 0 ? a + n : a;
+    if (a < 0) a += n;
+    else if (a >= n) a -= n;
+    if ((unsigned)a >= (unsigned)n) {
+        a %= n;
+        if (a < 0) a += n;
+    }
+    return a;
 }
 // cell (ix,iy,iz) belongs to the 7^3-1 stencil of cell (cx,cy,cz)
 // (cpp/neighbour_list.cpp:12-25); requires L >= 7
@@ -264,10 +293,7 @@ __device__ inline int block_exclusive_scan(int *s, int n, int *s_wtot) {
 // ------------------------------------------------ colloids listed per tile
 __device__ __forceinline__ bool near_interval(int c, int lo, int len, int L) {
     if (len >= L) return true;
-    int d = c - lo;
-    d %= L;
-    if (d < 0) d += L;
-    return d < len;
+    return wrapi(c - lo, L) < len;
 }
 
 // Colloids that can touch a particle of a tile during one MPCD step: a
@@ -281,28 +307,40 @@ constexpr int TILE_LIST_REACH = TILE_REACH + 4;
 
 // sur = false: the colloids that collide explicitly (tcl_*); sur = true: the
 // surrogate-driven ones, whose fluid the rotation pass keeps out (tsl_*)
+// cell window of tile T within which a colloid's current or run() cell puts
+// it on the tile's list
+__device__ __forceinline__ void tile_list_window(const Rep &R, int T, int *lo, int *ln) {
+    const int lt = T - R.first_tile;
+    const int t[3] = {lt % R.nt[0], (lt / R.nt[0]) % R.nt[1], lt / (R.nt[0] * R.nt[1])};
+    for (int k = 0; k < 3; k++) {
+        lo[k] = t[k] * R.B[k] - TILE_LIST_REACH;
+        ln[k] = R.B[k] + 2 * TILE_LIST_REACH;
+    }
+}
+__device__ __forceinline__ bool tile_list_member(const Phys &ph, const Dev &d, const Rep &R, int g, const int *lo,
+                                                 const int *ln, bool sur) {
+    if ((ph.surrogate && d.cmode[g]) != sur) return false;
+    bool p = near_interval(d.ccell[3 * g], lo[0], ln[0], R.L) && near_interval(d.ccell[3 * g + 1], lo[1], ln[1], R.L) &&
+             near_interval(d.ccell[3 * g + 2], lo[2], ln[2], R.L);
+    return p || (near_interval(d.ccell_run[3 * g], lo[0], ln[0], R.L) &&
+                 near_interval(d.ccell_run[3 * g + 1], lo[1], ln[1], R.L) &&
+                 near_interval(d.ccell_run[3 * g + 2], lo[2], ln[2], R.L));
+}
+
+// sur = false: the colloids that collide explicitly (tcl_*); sur = true: the
+// surrogate-driven ones, whose fluid the rotation pass keeps out (tsl_*)
 __device__ inline void build_tile_list(const Phys &ph, const Dev &d, const Rep &R, int T, bool sur = false) {
     __shared__ int s_wtot[16];
     __shared__ int s_n;
     int *list = sur ? d.tsl_list : d.tcl_list;
-    const int lt = T - R.first_tile;
-    const int tx = lt % R.nt[0], ty = (lt / R.nt[0]) % R.nt[1], tz = lt / (R.nt[0] * R.nt[1]);
-    const int reach = TILE_LIST_REACH;
-    const int lox = tx * R.B[0] - reach, loy = ty * R.B[1] - reach, loz = tz * R.B[2] - reach;
-    const int lnx = R.B[0] + 2 * reach, lny = R.B[1] + 2 * reach, lnz = R.B[2] + 2 * reach;
+    int lo[3], ln[3];
+    tile_list_window(R, T, lo, ln);
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     for (int base = 0; base < R.nc; base += blockDim.x) {
         int i = base + threadIdx.x;
         int g = R.first_colloid + i;
-        bool p = false;
-        if (i < R.nc && (ph.surrogate && d.cmode[g]) == sur) {
-            p = near_interval(d.ccell[3 * g], lox, lnx, R.L) && near_interval(d.ccell[3 * g + 1], loy, lny, R.L) &&
-                near_interval(d.ccell[3 * g + 2], loz, lnz, R.L);
-            p = p || (near_interval(d.ccell_run[3 * g], lox, lnx, R.L) &&
-                      near_interval(d.ccell_run[3 * g + 1], loy, lny, R.L) &&
-                      near_interval(d.ccell_run[3 * g + 2], loz, lnz, R.L));
-        }
+        bool p = i < R.nc && tile_list_member(ph, d, R, g, lo, ln, sur);
         int tot;
         int idx = block_compact(p, s_wtot, tot);
         if (p) {
@@ -317,11 +355,51 @@ __device__ inline void build_tile_list(const Phys &ph, const Dev &d, const Rep &
     __syncthreads();
 }
 
-struct TileColl {
+// LDS written by some lanes of a wave is visible to the wave's other lanes
+// afterwards
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// build_tile_list (explicit colloids) by one wave; the list also goes to
+// lds_g.  Returns the list length.
+__device__ inline int build_tile_list_wave(const Phys &ph, const Dev &d, const Rep &R, int T, int *lds_g) {
+    const int lane = threadIdx.x & 63;
+    int lo[3], ln[3];
+    tile_list_window(R, T, lo, ln);
+    int n = 0;
+    for (int base = 0; base < R.nc; base += 64) {
+        const int i = base + lane, g = R.first_colloid + i;
+        const bool p = i < R.nc && tile_list_member(ph, d, R, g, lo, ln, false);
+        const unsigned long long b = __ballot(p);
+        const int rank = __popcll(b & ((1ull << lane) - 1ull));
+        if (p) {
+            if (n + rank < TCL_CAP) {
+                d.tcl_list[T * TCL_CAP + n + rank] = g;
+                lds_g[n + rank] = g;
+            } else {
+                atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
+            }
+        }
+        n += __popcll(b);
+    }
+    n = min(n, TCL_CAP);
+    if (lane == 0) d.tcl_cnt[T] = n;
+    wave_sync();
+    return n;
+}
+
+// colloids listed for a tile: id, position, cell (every per-tile pass) ...
+struct TileCore {
     int n;
     int g[TCL_CAP];
     double px[TCL_CAP], py[TCL_CAP], pz[TCL_CAP];
     int cx[TCL_CAP], cy[TCL_CAP], cz[TCL_CAP];
+};
+// ... and their run() data (the passes that count shells or kick)
+struct TileColl : TileCore {
     double rx[TCL_CAP], ry[TCL_CAP], rz[TCL_CAP];    // position at run() time
     int qx[TCL_CAP], qy[TCL_CAP], qz[TCL_CAP];       // its cell
     double vx[TCL_CAP], vy[TCL_CAP], vz[TCL_CAP];    // velocity after the kick
@@ -335,15 +413,13 @@ struct NearColl {
     float rx[TCL_CAP], ry[TCL_CAP], rz[TCL_CAP];
 };
 
-__device__ inline void load_tile_colloids(const Dev &d, const Phys &ph, const Rep &R, int r, int T, bool kick_data,
-                                          TileColl &tc) {
-    if (d.tcl_stale[r]) build_tile_list(ph, d, R, T);
-    const int n = min(d.tcl_cnt[T], TCL_CAP);
-    for (int slot = threadIdx.x; slot < n; slot += blockDim.x) {
-        int g = d.tcl_list[T * TCL_CAP + slot];
-        tc.g[slot] = g;
-        tc.px[slot] = d.cpos[3 * g]; tc.py[slot] = d.cpos[3 * g + 1]; tc.pz[slot] = d.cpos[3 * g + 2];
-        tc.cx[slot] = d.ccell[3 * g]; tc.cy[slot] = d.ccell[3 * g + 1]; tc.cz[slot] = d.ccell[3 * g + 2];
+// listed colloid g into slot `slot` of the tile's LDS list
+template <class TC>
+__device__ __forceinline__ void load_tile_slot(const Dev &d, const Phys &ph, int g, int slot, bool kick_data, TC &tc) {
+    tc.g[slot] = g;
+    tc.px[slot] = d.cpos[3 * g]; tc.py[slot] = d.cpos[3 * g + 1]; tc.pz[slot] = d.cpos[3 * g + 2];
+    tc.cx[slot] = d.ccell[3 * g]; tc.cy[slot] = d.ccell[3 * g + 1]; tc.cz[slot] = d.ccell[3 * g + 2];
+    if constexpr (!std::is_same<TileCore, TC>::value) {
         if (kick_data) {
             tc.rx[slot] = d.cpos_run[3 * g]; tc.ry[slot] = d.cpos_run[3 * g + 1]; tc.rz[slot] = d.cpos_run[3 * g + 2];
             tc.qx[slot] = d.ccell_run[3 * g]; tc.qy[slot] = d.ccell_run[3 * g + 1]; tc.qz[slot] = d.ccell_run[3 * g + 2];
@@ -357,6 +433,14 @@ __device__ inline void load_tile_colloids(const Dev &d, const Phys &ph, const Re
             tc.kz[slot] = (d.cra[3 * g + 2] * v0) * fac;
         }
     }
+}
+template <class TC>
+__device__ inline void load_tile_colloids(const Dev &d, const Phys &ph, const Rep &R, int r, int T, bool kick_data,
+                                          TC &tc) {
+    if (d.tcl_stale[r]) build_tile_list(ph, d, R, T);
+    const int n = min(d.tcl_cnt[T], TCL_CAP);
+    for (int slot = threadIdx.x; slot < n; slot += blockDim.x)
+        load_tile_slot(d, ph, d.tcl_list[T * TCL_CAP + slot], slot, kick_data, tc);
     if (threadIdx.x == 0) tc.n = n;
     __syncthreads();
 }
@@ -402,20 +486,44 @@ __device__ __forceinline__ unsigned cell_mask(const AxisMask &am, const Region &
     return am.m[0][ux] & am.m[1][uy] & am.m[2][uz];
 }
 // run_cells: the colloids' cells at run() time, else their current cells
-__device__ inline void build_axis_masks(const TileColl &tc, const Region &g, int L, bool run_cells, int reach,
-                                        AxisMask &am) {
+// masks of the cells (cx, cy, cz)[k] (current or run-time cells of the listed
+// colloids)
+__device__ inline void build_axis_masks(int n, const int *cx, const int *cy, const int *cz, const Region &g, int L,
+                                        int reach, AxisMask &am) {
     for (int i = threadIdx.x; i < 3 * AXIS_MAX; i += blockDim.x) (&am.m[0][0])[i] = 0u;
     __syncthreads();
-    const int nk = min(tc.n, MASK_BITS), span = 2 * reach + 1;
-    for (int w = threadIdx.x; w < nk * 3 * span; w += blockDim.x) {
-        int k = w / (3 * span), rem = w - k * 3 * span;
-        int a = rem / span, dd = rem - a * span - reach;
-        int c = a == 0 ? (run_cells ? tc.qx[k] : tc.cx[k]) : a == 1 ? (run_cells ? tc.qy[k] : tc.cy[k])
-                                                                      : (run_cells ? tc.qz[k] : tc.cz[k]);
-        int u = axis_index(g, a, wrapi(c + dd, L), L);
-        if (u >= 0) atomicOr(&am.m[a][u], 1u << k);
+    const int nk = min(n, MASK_BITS);
+    for (int w = threadIdx.x; w < nk * 3; w += blockDim.x) {
+        const int k = w / 3, a = w - 3 * k;
+        const int c = a == 0 ? cx[k] : a == 1 ? cy[k] : cz[k];
+        for (int dd = -reach; dd <= reach; dd++) {
+            int u = axis_index(g, a, wrapi(c + dd, L), L);
+            if (u >= 0) atomicOr(&am.m[a][u], 1u << k);
+        }
     }
     __syncthreads();
+}
+// build_axis_masks by one wave
+__device__ inline void build_axis_masks_wave(int n, const int *cx, const int *cy, const int *cz, const Region &g,
+                                             int L, int reach, AxisMask &am) {
+    const int lane = threadIdx.x & 63;
+    for (int i = lane; i < 3 * AXIS_MAX; i += 64) (&am.m[0][0])[i] = 0u;
+    wave_sync();
+    const int nk = min(n, MASK_BITS);
+    for (int w = lane; w < nk * 3; w += 64) {
+        const int k = w / 3, a = w - 3 * k;
+        const int c = a == 0 ? cx[k] : a == 1 ? cy[k] : cz[k];
+        for (int dd = -reach; dd <= reach; dd++) {
+            int u = axis_index(g, a, wrapi(c + dd, L), L);
+            if (u >= 0) atomicOr(&am.m[a][u], 1u << k);
+        }
+    }
+    wave_sync();
+}
+__device__ inline void build_axis_masks(const TileColl &tc, const Region &g, int L, bool run_cells, int reach,
+                                        AxisMask &am) {
+    if (run_cells) build_axis_masks(tc.n, tc.qx, tc.qy, tc.qz, g, L, reach, am);
+    else build_axis_masks(tc.n, tc.cx, tc.cy, tc.cz, g, L, reach, am);
 }
 
 // run() rear-shell test of listed colloid k for a particle at binned cell
@@ -432,7 +540,7 @@ __device__ __forceinline__ bool shell_test(const Phys &ph, const TileColl &tc, i
 // candidate colloids of a particle, ascending: mask bits, then the unmasked
 // tail k >= MASK_BITS (every use applies the exact stencil test)
 template <class F>
-__device__ __forceinline__ void for_candidates(const TileColl &tc, unsigned m, F &&f) {
+__device__ __forceinline__ void for_candidates(const TileCore &tc, unsigned m, F &&f) {
     while (m) {
         int k = __ffs(m) - 1;
         m &= m - 1;
@@ -446,36 +554,6 @@ __device__ __forceinline__ void for_candidates(const TileColl &tc, unsigned m, F
 // the one-workgroup-per-replica kernel (individual entry points) and the
 // wave-per-colloid kernel of the fused step; same operations in the same order.
 
-// cpp/fluid_colloid_collision.cpp:46-47 (the sums dv, dw were formed in the
-// reference's event order)
-// A surrogate-driven colloid receives its sampled impulse per substep
-// instead (it has no collision events); the event sums are also accumulated
-// for the surrogate's training pairs.
-__device__ inline void colloid_apply(const Phys &ph, const Dev &d, int g) {
-    const bool sur = ph.surrogate && d.cmode[g];
-    for (int k = 0; k < 3; k++) {
-        const double dv = sur ? d.csur[6 * g + k] : d.cdv[3 * g + k];
-        const double dw = sur ? d.csur[6 * g + 3 + k] : d.cdw[3 * g + k];
-        d.cvel[3 * g + k] = d.cvel[3 * g + k] + dv * ph.mass_fl / ph.mass_colloid;
-        d.comega[3 * g + k] = d.comega[3 * g + k] + dw * ph.mass_fl / ph.I_colloid;
-        d.cimp[6 * g + k] += d.cdv[3 * g + k];
-        d.cimp[6 * g + 3 + k] += d.cdw[3 * g + k];
-        d.cdv[3 * g + k] = 0;
-        d.cdw[3 * g + k] = 0;
-    }
-}
-// cpp/update.cpp:34-46
-__device__ inline void colloid_activity(const Phys &ph, const Dev &d, int g) {
-    double bx = d.comega[3 * g] * ph.dt, by = d.comega[3 * g + 1] * ph.dt, bz = d.comega[3 * g + 2] * ph.dt;
-    double sbx = sin(bx), sby = sin(by), sbz = sin(bz), cbx = cos(bx), cby = cos(by), cbz = cos(bz);
-    double m1x = cby * cbz, m1y = -cby * sbz, m1z = sby;
-    double m2x = sbx * sby * cbz + cbx * sbz, m2y = -sbx * sby * sbz + cbx * cbz, m2z = -sbx * cby;
-    double m3x = -cbx * sby * cbz + sbx * sbz, m3y = cbx * sby * sbz + sbx * cbz, m3z = cbx * cby;
-    double a = d.cra[3 * g], b = d.cra[3 * g + 1], c = d.cra[3 * g + 2];
-    d.cra[3 * g] = (m1x * a + m1y * b) + m1z * c;
-    d.cra[3 * g + 1] = (m2x * a + m2y * b) + m2z * c;
-    d.cra[3 * g + 2] = (m3x * a + m3y * b) + m3z * c;
-}
 // cpp/update.cpp:8-13
 __device__ __forceinline__ double powi(double x, int r) {
     double ans = 1;
@@ -506,7 +584,7 @@ __device__ __forceinline__ bool force_term(const Phys &ph, const Dev &d, double 
                : pair_force(ph, L, d.cpos + 3 * g, d.cpos + 3 * j, c[0], c[1], c[2]);
 }
 // cpp/update.cpp:14-32 for one colloid, serially in list order
-__device__ inline void colloid_force_serial(const Phys &ph, const Dev &d, double L, int g) {
+__device__ inline void colloid_force_serial(const Phys &ph, const Dev &d, double L, int g, double *f) {
     double f0 = 0, f1 = 0, f2 = 0;
     for (int m = 0; m < d.md_cnt[g]; m++) {
         double c[3];
@@ -516,49 +594,105 @@ __device__ inline void colloid_force_serial(const Phys &ph, const Dev &d, double
             else { f0 = f0 + c[0]; f1 = f1 + c[1]; f2 = f2 + c[2]; }
         }
     }
-    d.cf[3 * g] = f0; d.cf[3 * g + 1] = f1; d.cf[3 * g + 2] = f2;
+    f[0] = f0; f[1] = f1; f[2] = f2;
 }
-// cpp/update.cpp:62-67
-__device__ inline void colloid_velocity(const Phys &ph, const Dev &d, int g) {
-    double dtb2 = ph.dt / (ph.mass_colloid * 2);
-    for (int k = 0; k < 3; k++) d.cvel[3 * g + k] = d.cvel[3 * g + k] + (d.coldf[3 * g + k] + d.cf[3 * g + k]) * dtb2;
-}
-// cpp/run.cpp:14 (colloid part of run) + the run-time snapshot
-__device__ inline void colloid_kick(const Phys &ph, const Dev &d, int g) {
-    double v0 = d.cv0[g];
+// The per-colloid phases of one substep, in PH_* order, by one thread:
+// every input is loaded up front, the phases run in registers, and the
+// results are stored at the end (each phase feeds the next without a round
+// trip through memory).  fn is the new WCA force when PH_FORCE is set.
+//  PH_APPLY     cpp/fluid_colloid_collision.cpp:46-47 (the sums dv, dw were
+//               formed in the reference's event order); a surrogate-driven
+//               colloid receives its sampled impulse instead, and the event
+//               sums are accumulated for the surrogate's training pairs
+//  PH_ACTIVITY  cpp/update.cpp:34-46
+//  PH_VELOCITY  cpp/update.cpp:62-67
+//  PH_KICK      cpp/run.cpp:14 (colloid part of run) + the run-time snapshot
+//  PH_OLDF      cpp/active_brownian.cpp:28
+//  PH_POS       cpp/update.cpp:48-54, into the other position buffer; marks
+//               the replica's pair candidates / tile lists stale if the
+//               colloid left their margins
+__device__ inline void colloid_update(const Phys &ph, const Dev &d, int L, int r, int g, unsigned flags,
+                                      const double *fn) {
+    double v[3], w[3], ra[3], f[3], of[3], p0[3], dv[3] = {0, 0, 0}, dw[3] = {0, 0, 0};
     for (int k = 0; k < 3; k++) {
-        d.cvel[3 * g + k] = d.cvel[3 * g + k] + d.cra[3 * g + k] * v0;
-        d.cpos_run[3 * g + k] = d.cpos[3 * g + k];
-        d.ccell_run[3 * g + k] = d.ccell[3 * g + k];
+        v[k] = d.cvel[3 * g + k];
+        w[k] = d.comega[3 * g + k];
+        ra[k] = d.cra[3 * g + k];
+        f[k] = (flags & PH_FORCE) ? fn[k] : d.cf[3 * g + k];
+        of[k] = d.coldf[3 * g + k];
+        p0[k] = d.cpos[3 * g + k];
     }
-    d.ccnt[g] = 0;
-}
-// cpp/active_brownian.cpp:28
-__device__ inline void colloid_oldf(const Dev &d, int g) {
-    for (int k = 0; k < 3; k++) d.coldf[3 * g + k] = d.cf[3 * g + k];
-}
-// cpp/update.cpp:48-54, into the other position buffer; marks the replica's
-// pair candidates / tile lists stale if the colloid left their margins
-__device__ inline void colloid_pos(const Phys &ph, const Dev &d, int L, int r, int g, bool margin) {
-    double dt2 = ph.dt * ph.dt, ddt = 0.5 * dt2 / ph.mass_colloid;
-    bool far_tl = false;
-    double m2 = 0;
+    if (flags & PH_APPLY) {
+        const bool sur = ph.surrogate && d.cmode[g];
+        for (int k = 0; k < 3; k++) {
+            dv[k] = d.cdv[3 * g + k];
+            dw[k] = d.cdw[3 * g + k];
+            const double iv = sur ? d.csur[6 * g + k] : dv[k];
+            const double iw = sur ? d.csur[6 * g + 3 + k] : dw[k];
+            v[k] = v[k] + iv * ph.mass_fl / ph.mass_colloid;
+            w[k] = w[k] + iw * ph.mass_fl / ph.I_colloid;
+        }
+    }
+    if (flags & PH_ACTIVITY) {
+        double bx = w[0] * ph.dt, by = w[1] * ph.dt, bz = w[2] * ph.dt;
+        double sbx = sin(bx), sby = sin(by), sbz = sin(bz), cbx = cos(bx), cby = cos(by), cbz = cos(bz);
+        double m1x = cby * cbz, m1y = -cby * sbz, m1z = sby;
+        double m2x = sbx * sby * cbz + cbx * sbz, m2y = -sbx * sby * sbz + cbx * cbz, m2z = -sbx * cby;
+        double m3x = -cbx * sby * cbz + sbx * sbz, m3y = cbx * sby * sbz + sbx * cbz, m3z = cbx * cby;
+        double a = ra[0], b = ra[1], c = ra[2];
+        ra[0] = (m1x * a + m1y * b) + m1z * c;
+        ra[1] = (m2x * a + m2y * b) + m2z * c;
+        ra[2] = (m3x * a + m3y * b) + m3z * c;
+    }
+    if (flags & PH_VELOCITY) {
+        double dtb2 = ph.dt / (ph.mass_colloid * 2);
+        for (int k = 0; k < 3; k++) v[k] = v[k] + (of[k] + f[k]) * dtb2;
+    }
+    if (flags & PH_KICK) {
+        double v0 = d.cv0[g];
+        for (int k = 0; k < 3; k++) {
+            v[k] = v[k] + ra[k] * v0;
+            d.cpos_run[3 * g + k] = p0[k];
+            d.ccell_run[3 * g + k] = d.ccell[3 * g + k];
+        }
+        d.ccnt[g] = 0;
+    }
+    if (flags & PH_OLDF)
+        for (int k = 0; k < 3; k++) of[k] = f[k];
+    if (flags & PH_POS) {
+        double dt2 = ph.dt * ph.dt, ddt = 0.5 * dt2 / ph.mass_colloid;
+        bool far_tl = false;
+        double m2 = 0;
+        for (int k = 0; k < 3; k++) {
+            double dx = v[k] * ph.dt + f[k] * ddt;
+            double p = p0[k] + dx;
+            d.cdisp[3 * g + k] += dx;
+            p = mod1(p, (double)L);
+            d.cpos_nxt[3 * g + k] = p;
+            int cc = coord_cell(p, L);
+            d.ccell_nxt[3 * g + k] = cc;
+            int dm = pdiff(cc, d.ccell_md[3 * g + k], L);
+            if (dm > 1 || dm < -1) far_tl = true;
+            double md = img1(p - d.cpos_md[3 * g + k], (double)L);
+            m2 += md * md;
+        }
+        if (m2 > 0.25 * MD_SKIN * MD_SKIN) atomicOr(&d.md_stale[r], 1);
+        if (far_tl) atomicOr(&d.tcl_stale[r], 1);
+        if ((flags & PH_MARGIN) && m2 > ph.margin * ph.margin) atomicOr(d.status, BW_FLAG_MARGIN);
+    }
     for (int k = 0; k < 3; k++) {
-        double dx = d.cvel[3 * g + k] * ph.dt + d.cf[3 * g + k] * ddt;
-        double p = d.cpos[3 * g + k] + dx;
-        d.cdisp[3 * g + k] += dx;
-        p = mod1(p, (double)L);
-        d.cpos_nxt[3 * g + k] = p;
-        int cc = coord_cell(p, L);
-        d.ccell_nxt[3 * g + k] = cc;
-        int dm = pdiff(cc, d.ccell_md[3 * g + k], L);
-        if (dm > 1 || dm < -1) far_tl = true;
-        double md = img1(p - d.cpos_md[3 * g + k], (double)L);
-        m2 += md * md;
+        if (flags & PH_APPLY) {
+            d.cimp[6 * g + k] += dv[k];
+            d.cimp[6 * g + 3 + k] += dw[k];
+            d.cdv[3 * g + k] = 0;
+            d.cdw[3 * g + k] = 0;
+        }
+        if (flags & (PH_APPLY | PH_VELOCITY | PH_KICK)) d.cvel[3 * g + k] = v[k];
+        if (flags & PH_APPLY) d.comega[3 * g + k] = w[k];
+        if (flags & PH_ACTIVITY) d.cra[3 * g + k] = ra[k];
+        if (flags & PH_FORCE) d.cf[3 * g + k] = f[k];
+        if (flags & PH_OLDF) d.coldf[3 * g + k] = of[k];
     }
-    if (m2 > 0.25 * MD_SKIN * MD_SKIN) atomicOr(&d.md_stale[r], 1);
-    if (far_tl) atomicOr(&d.tcl_stale[r], 1);
-    if (margin && m2 > ph.margin * ph.margin) atomicOr(d.status, BW_FLAG_MARGIN);
 }
 // pair test of cpp/neighbour_list.cpp:33-34 (symmetric: img(-a) = -img(a))
 __device__ __forceinline__ bool md_pair(const Phys &ph, const Dev &d, double L, int g, int j) {
@@ -584,30 +718,6 @@ __device__ inline void colloid_mdlist_serial(const Phys &ph, const Dev &d, const
     if (w > MD_CAP) { atomicOr(d.status, BW_FLAG_LIST_OVERFLOW); w = MD_CAP; }
     d.md_cnt[g] = w;
 }
-// the same list by one wave (ballot compaction keeps the ascending order)
-__device__ inline void colloid_mdlist_wave(const Phys &ph, const Dev &d, const Rep &R, int r, int g) {
-    const int lane = threadIdx.x & 63;
-    const bool full = d.md_stale[r] != 0;
-    const int ncand = full ? R.nc : d.mdc_cnt[g];
-    int w = 0;
-    for (int base = 0; base < ncand; base += 64) {
-        int m = base + lane;
-        int j = -1;
-        bool p = false;
-        if (m < ncand) {
-            j = full ? R.first_colloid + m : d.mdc_list[g * MDC_CAP + m];
-            p = j != g && md_pair(ph, d, (double)R.L, g, j);
-        }
-        unsigned long long b = __ballot(p);
-        int rank = __popcll(b & ((1ull << lane) - 1ull));
-        if (p && w + rank < MD_CAP) d.md_list[g * MD_CAP + w + rank] = j;
-        w += __popcll(b);
-    }
-    if (lane == 0) {
-        if (w > MD_CAP) { atomicOr(d.status, BW_FLAG_LIST_OVERFLOW); w = MD_CAP; }
-        d.md_cnt[g] = w;
-    }
-}
 
 // All colloid work of one replica by one workgroup, phases in PH_* order.
 __device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R, int r, unsigned flags) {
@@ -615,13 +725,9 @@ __device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R
     const double L = (double)R.L;
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
         int g = g0 + i;
-        if (flags & PH_APPLY) colloid_apply(ph, d, g);
-        if (flags & PH_ACTIVITY) colloid_activity(ph, d, g);
-        if (flags & PH_FORCE) colloid_force_serial(ph, d, L, g);
-        if (flags & PH_VELOCITY) colloid_velocity(ph, d, g);
-        if (flags & PH_KICK) colloid_kick(ph, d, g);
-        if (flags & PH_OLDF) colloid_oldf(d, g);
-        if (flags & PH_POS) colloid_pos(ph, d, R.L, r, g, (flags & PH_MARGIN) != 0);
+        double fn[3] = {0, 0, 0};
+        if (flags & PH_FORCE) colloid_force_serial(ph, d, L, g, fn);
+        colloid_update(ph, d, R.L, r, g, flags, fn);
     }
     __syncthreads();
     if (flags & PH_MDLIST)

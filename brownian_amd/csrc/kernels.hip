@@ -36,7 +36,10 @@ __global__ void k_step_begin(Phys ph, Dev d) {
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) *d.step = s;
+    if (threadIdx.x == 0) {
+        *d.step = s;
+        *d.chunk_cnt = 0;
+    }
 }
 
 // --------------------------------------------------------- colloid phases
@@ -204,8 +207,15 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
 // shell -- is streamed through all n_sub substeps here, in registers, with
 // the arithmetic of the streaming kernel.  The others ("near") are stored
 // first in the tile, and only they are touched by the per-substep kernels.
-template <class Real, bool COUNT_SHELL, bool MAP, bool BLOCK>
-__global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real> src, Fluid<Real> dst) {
+// bytes of the SRD pass's per-cell region (7 doubles per cell), which also
+// holds the cap ids of the sort
+__host__ __device__ inline size_t srd_cell_region(int ncell, int cap) {
+    const size_t a = (size_t)ncell * 56, b = ((size_t)cap * 4 + 7) / 8 * 8;
+    return a > b ? a : b;
+}
+
+template <class Real, bool COUNT_SHELL, bool MAP, bool BLOCK, int TH>
+__global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Fluid<Real> dst) {
     constexpr bool NEAR_TEST = COUNT_SHELL && BLOCK;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_pre[28], s_src[27], s_base[27];
@@ -214,7 +224,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     __shared__ NearColl nc[1];  // used by the near test only
     __shared__ int s_shell[COUNT_SHELL ? TCL_CAP : 1];
     __shared__ AxisMask s_am[COUNT_SHELL ? 2 : 1];  // [0] run shells, [1] near test
-    __shared__ int s_cc[SRD_PT][SRD_THREADS / 64];   // near particles per chunk and wave
+    __shared__ int s_cc[SRD_PT][TH / 64];   // near particles per chunk and wave
     __shared__ double s_sp[COUNT_SHELL ? TCL_CAP : 1][6];  // surrogate colloids: position, velocity
     __shared__ int s_sn;
     const int T = blockIdx.x;
@@ -225,14 +235,15 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     const int B0 = R.B[0], B1 = R.B[1], B2 = R.B[2];
     const int ncell = B0 * B1 * B2;
     const int ox0 = tg.tx * B0, oy0 = tg.ty * B1, oz0 = tg.tz * B2;  // tile origin (cells)
-    // LDS carve-up
+    // LDS carve-up.  The id list s_pk is dead once the ranks are known, before
+    // the per-cell means are formed, so it shares the per-cell region.
     double *s_u = (double *)smem;                       // 3 * ncell
     double *s_ax = s_u + 3 * ncell;                     // 3 * ncell  (rotation axis)
     double *s_sc = s_ax + 3 * ncell;                    // ncell      (thermostat factor)
-    Real *s_v = (Real *)(s_sc + ncell);                 // 3 * cap; later the per-particle energy terms
+    int *s_pk = (int *)smem;                            // cap: ids in cell-sorted (unranked) order
+    Real *s_v = (Real *)(smem + srd_cell_region(ncell, cap));  // 3 * cap; later the per-particle energy terms
     double *s_e2 = (double *)s_v;
-    int *s_pk = (int *)(s_v + 3 * cap);                 // cap: ids in cell-sorted (unranked) order
-    int *s_start = s_pk + cap;                          // ncell + 1
+    int *s_start = (int *)(s_v + 3 * cap);              // ncell + 1
     int *s_fill = s_start + ncell + 1;                  // ncell
     unsigned short *s_ord = (unsigned short *)(s_fill + ncell);  // cap: cell-sorted, id-ranked
 
@@ -301,7 +312,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     bool drift = false;
 #pragma unroll
     for (int k = 0; k < SRD_PT; k++) {
-        int g = threadIdx.x + k * SRD_THREADS;
+        int g = threadIdx.x + k * TH;
         rkey[k] = -1;
         if (g < n) {
             int lo = 0, hi = 26;  // the source list holding gather index g
@@ -340,14 +351,14 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < SRD_PT; k++) {
-        int g = threadIdx.x + k * SRD_THREADS;
+        int g = threadIdx.x + k * TH;
         if (g < n) s_pk[atomicAdd(&s_fill[rkey[k]], 1)] = rpid[k];
     }
     __syncthreads();
     // rank of each particle in its cell by id: the cell's canonical order
 #pragma unroll
     for (int k = 0; k < SRD_PT; k++) {
-        int g = threadIdx.x + k * SRD_THREADS;
+        int g = threadIdx.x + k * TH;
         if (g < n) {
             const int b = s_start[rkey[k]], e = s_start[rkey[k] + 1];
             int rank = 0;
@@ -389,7 +400,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     // per particle: rotated velocity relative to the mean, its squared norm (:37-41)
 #pragma unroll
     for (int k = 0; k < SRD_PT; k++) {
-        int g = threadIdx.x + k * SRD_THREADS;
+        int g = threadIdx.x + k * TH;
         if (g < n) {
             const int c = rkey[k];
             if (s_start[c + 1] - s_start[c] > 1) {
@@ -454,7 +465,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     unsigned nearbits = 0;
 #pragma unroll
     for (int k = 0; k < SRD_PT; k++) {
-        int g = threadIdx.x + k * SRD_THREADS;
+        int g = threadIdx.x + k * TH;
         if (g < n) {
             int c = rkey[k];
             int N = s_start[c + 1] - s_start[c];
@@ -561,7 +572,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
         unsigned long long bal[SRD_PT];
 #pragma unroll
         for (int k = 0; k < SRD_PT; k++) {
-            bal[k] = __ballot((threadIdx.x + k * SRD_THREADS < n) && (nearbits & (1u << k)));
+            bal[k] = __ballot((threadIdx.x + k * TH < n) && (nearbits & (1u << k)));
             if (lane == 0) s_cc[k][w] = __popcll(bal[k]);
         }
         __syncthreads();
@@ -569,7 +580,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
 #pragma unroll
         for (int k = 0; k < SRD_PT; k++) {
             int before = 0, tot = 0;
-            for (int v = 0; v < SRD_THREADS / 64; v++) {
+            for (int v = 0; v < TH / 64; v++) {
                 int cnt = s_cc[k][v];
                 before += v < w ? cnt : 0;
                 tot += cnt;
@@ -582,7 +593,7 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
     bool far_drift = false;
 #pragma unroll
     for (int k = 0; k < SRD_PT; k++) {
-        int g = threadIdx.x + k * SRD_THREADS;
+        int g = threadIdx.x + k * TH;
         if (g < n) {
             int slot_in_tile = g;
             int cell = rcell[k];
@@ -632,6 +643,13 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
         d.count_next[T] = n;
         d.near_cnt[T] = n_near;
     }
+    if (BLOCK && COUNT_SHELL) {  // the near prefix in chunks of 64 for the per-substep pass
+        __shared__ int s_chunk0;
+        const int nch = (n_near + 63) >> 6;
+        if (threadIdx.x == 0) s_chunk0 = nch > 0 ? atomicAdd(d.chunk_cnt, nch) : 0;
+        __syncthreads();
+        for (int j = threadIdx.x; j < nch; j += blockDim.x) d.chunk_tile[s_chunk0 + j] = (T << 6) | j;
+    }
     if (COUNT_SHELL) {
         __syncthreads();
         for (int q = threadIdx.x; q < tc.n; q += blockDim.x)
@@ -644,12 +662,12 @@ __global__ __launch_bounds__(SRD_THREADS) void k_srd(Phys ph, Dev d, Fluid<Real>
 //           [streaming, cpp/update.cpp:56-60, with exact int-cell binning]
 //           [collision detection, cpp/fluid_colloid_collision.cpp:32-35]
 // Candidate colloids of a particle come from the tile's LDS stencil masks.
-template <class Real, bool KICK, bool STREAM, bool DETECT>
+template <class Real, bool KICK, bool STREAM, bool DETECT, class TC>
 struct StreamOp {
     const Phys &ph;
     const Dev &d;
     Fluid<Real> &f;
-    const TileColl &tc;
+    const TC &tc;
     const Region &rg;
     const AxisMask *mdet, *mkick;
     int L, r, lox, loy, loz, lnx, lny, lnz;
@@ -660,7 +678,7 @@ struct StreamOp {
         unpack_cell(c, ix, iy, iz);
         const int phantom = c & PHANTOM;
         const double Ld = (double)L;
-        if (KICK) {
+        if constexpr (KICK) {
             unsigned m = cell_mask(*mkick, rg, ix, iy, iz, L);
             if (m || tc.n > MASK_BITS) {
                 double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
@@ -732,10 +750,13 @@ struct StreamOp {
 };
 
 // near_only: the fused step's per-substep pass over the particles stored
-// first in each tile by the SRD kernel; otherwise every particle.
-template <class Real, bool KICK, bool STREAM, bool DETECT>
-__global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid<Real> f, int near_only) {
-    __shared__ TileColl tc;
+// first in each tile by the SRD kernel (a few per cent of a tile: one wave per
+// tile, so that every tile of a large box is resident at once); otherwise
+// every particle.  Only the kicking pass stages the colloids' run() data.
+template <class Real, bool KICK, bool STREAM, bool DETECT, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_stream(Phys ph, Dev d, Fluid<Real> f, int near_only) {
+    using TC = typename std::conditional<KICK, TileColl, TileCore>::type;
+    __shared__ TC tc;
     __shared__ AxisMask s_am[(KICK && DETECT) ? 2 : 1];
     const int T = blockIdx.x;
     Rep R;
@@ -746,17 +767,17 @@ __global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid
     const AxisMask *mkick = &s_am[0], *mdet = &s_am[(KICK && DETECT) ? 1 : 0];
     if (KICK || DETECT) {
         load_tile_colloids(d, ph, R, tg.r, T, KICK, tc);
-        if (KICK) build_axis_masks(tc, rg, L, true, 3, s_am[0]);
-        if (DETECT) build_axis_masks(tc, rg, L, false, 3, s_am[(KICK && DETECT) ? 1 : 0]);
+        if constexpr (KICK) build_axis_masks(tc.n, tc.qx, tc.qy, tc.qz, rg, L, 3, s_am[0]);
+        if (DETECT) build_axis_masks(tc.n, tc.cx, tc.cy, tc.cz, rg, L, 3, s_am[(KICK && DETECT) ? 1 : 0]);
     }
-    StreamOp<Real, KICK, STREAM, DETECT> op{ph, d, f, tc, rg, mdet, mkick, L, tg.r,
-                                            tg.tx * R.B[0] - TILE_REACH, tg.ty * R.B[1] - TILE_REACH,
-                                            tg.tz * R.B[2] - TILE_REACH, R.B[0] + 2 * TILE_REACH,
-                                            R.B[1] + 2 * TILE_REACH, R.B[2] + 2 * TILE_REACH, false, false};
+    StreamOp<Real, KICK, STREAM, DETECT, TC> op{ph, d, f, tc, rg, mdet, mkick, L, tg.r,
+                                                tg.tx * R.B[0] - TILE_REACH, tg.ty * R.B[1] - TILE_REACH,
+                                                tg.tz * R.B[2] - TILE_REACH, R.B[0] + 2 * TILE_REACH,
+                                                R.B[1] + 2 * TILE_REACH, R.B[2] + 2 * TILE_REACH, false, false};
     const int base_slot = T * ph.cap;
     // two particles per thread per iteration: all loads first
-    for (int i0 = threadIdx.x; i0 < n; i0 += 2 * STREAM_THREADS) {
-        const int i1 = i0 + STREAM_THREADS;
+    for (int i0 = threadIdx.x; i0 < n; i0 += 2 * THREADS) {
+        const int i1 = i0 + THREADS;
         const bool has1 = i1 < n;
         const int s0 = base_slot + i0, s1 = base_slot + (has1 ? i1 : i0);
         int c0 = f.cell[s0], c1 = f.cell[s1];
@@ -767,6 +788,78 @@ __global__ __launch_bounds__(STREAM_THREADS) void k_stream(Phys ph, Dev d, Fluid
     }
     if (op.drift) atomicOr(d.status, BW_FLAG_DRIFT);
     if (op.ch_over) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
+}
+
+// The fused step's per-substep pass over the near particles (stored first in
+// each tile by the SRD kernel; a few per cent of the fluid, most of it around
+// the colloids).  The SRD kernel lists them in chunks of 64 (tile, chunk);
+// every wave of a fixed grid takes chunks in turn, stages its tile's colloids
+// and masks and handles one particle per lane, so the work is spread evenly
+// whatever the clustering.  Waves synchronise only within themselves.
+template <class Real, bool KICK>
+__global__ __launch_bounds__(256, KICK ? 1 : 8) void k_stream_near(Phys ph, Dev d, Fluid<Real> f) {
+    static_assert(TCL_CAP == 64, "one lane per listed colloid");
+    using TC = typename std::conditional<KICK, TileColl, TileCore>::type;
+    __shared__ TC s_tc[4];
+    __shared__ AxisMask s_am[4][KICK ? 2 : 1];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    TC &tc = s_tc[w];
+    // the argument fields of the loop, fetched in one batch of scalar loads
+    // rather than one dependent round trip per first use
+    asm volatile("" ::"s"(d.chunk_tile), "s"(d.tile_rep), "s"(d.tcl_cnt), "s"(d.tcl_list), "s"(d.near_cnt),
+                 "s"(d.rep), "s"(d.tcl_stale), "s"(d.cpos), "s"(d.ccell), "s"(d.status), "s"(d.ev_cnt),
+                 "s"(d.ev_slot), "s"(d.chain_cnt), "s"(d.chain_slot));
+    asm volatile("" ::"s"(f.cell), "s"(f.ox), "s"(f.oy), "s"(f.oz), "s"(f.vx), "s"(f.vy), "s"(f.vz), "s"(ph.cap),
+                 "s"(ph.dt), "s"(ph.lim_coll));
+    const int nchunk = ld_const(d.chunk_cnt);
+    const int stride = gridDim.x * 4;
+    bool drift = false, ch_over = false;
+    int ch = blockIdx.x * 4 + w;
+    int e = ch < nchunk ? ld_const(d.chunk_tile + ch) : 0;
+    for (; ch < nchunk; ch += stride) {
+        const int T = e >> 6, i = ((e & 63) << 6) + lane;
+        if (ch + stride < nchunk) e = ld_const(d.chunk_tile + ch + stride);  // next chunk, ahead of its use
+        const int r = ld_const(d.tile_rep + T);
+        int cnt = min(ld_const(d.tcl_cnt + T), TCL_CAP);  // a stale list is rebuilt below
+        int gl = d.tcl_list[T * TCL_CAP + lane];
+        const int n = min(ld_const(d.near_cnt + T), ph.cap);
+        const int slot = T * ph.cap + i;
+        int c0 = 0;
+        Real ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
+        if (i < n) {
+            c0 = f.cell[slot];
+            ox = f.ox[slot]; oy = f.oy[slot]; oz = f.oz[slot];
+            vx = f.vx[slot]; vy = f.vy[slot]; vz = f.vz[slot];
+        }
+        wave_sync();  // the previous chunk's lanes are done with the LDS lists
+        // the listed colloids, loaded before the list is known to be current
+        // (it is rebuilt and loaded again when a colloid left its margin)
+        if (lane < cnt) load_tile_slot(d, ph, gl, lane, KICK, tc);
+        const Rep R = ld_const(d.rep + r);
+        const int L = R.L;
+        if (ld_const(d.tcl_stale + r)) {
+            wave_sync();
+            cnt = build_tile_list_wave(ph, d, R, T, tc.g);
+            gl = tc.g[lane];
+            if (lane < cnt) load_tile_slot(d, ph, gl, lane, KICK, tc);
+        }
+        if (lane == 0) tc.n = cnt;
+        wave_sync();
+        const int lt = T - R.first_tile;
+        const int tx = lt % R.nt[0], ty = (lt / R.nt[0]) % R.nt[1], tz = lt / (R.nt[0] * R.nt[1]);
+        const Region rg = tile_region(R, tx, ty, tz);
+        if constexpr (KICK) build_axis_masks_wave(cnt, tc.qx, tc.qy, tc.qz, rg, L, 3, s_am[w][0]);
+        build_axis_masks_wave(cnt, tc.cx, tc.cy, tc.cz, rg, L, 3, s_am[w][KICK ? 1 : 0]);
+        StreamOp<Real, KICK, true, true, TC> op{ph, d, f, tc, rg, &s_am[w][KICK ? 1 : 0], &s_am[w][0], L, r,
+                                                tx * R.B[0] - TILE_REACH, ty * R.B[1] - TILE_REACH,
+                                                tz * R.B[2] - TILE_REACH, R.B[0] + 2 * TILE_REACH,
+                                                R.B[1] + 2 * TILE_REACH, R.B[2] + 2 * TILE_REACH, false, false};
+        if (i < n) op(slot, c0, ox, oy, oz, vx, vy, vz);
+        drift = drift || op.drift;
+        ch_over = ch_over || op.ch_over;
+    }
+    if (drift) atomicOr(d.status, BW_FLAG_DRIFT);
+    if (ch_over) atomicOr(d.status, BW_FLAG_CHAIN_OVERFLOW);
 }
 
 // --------------------------------------------------------------- collisions
@@ -813,19 +906,20 @@ __device__ __forceinline__ void transfer(const double *dump, double vx, double v
 // inside g and no other colloid, ranked in the reference's neigh_fl order
 // (stencil position, then id) and reflected; the momentum/torque sums are taken
 // in that order.  A particle a reflection moved into a later colloid continues
-// the reference's colloid loop serially.  The last workgroup of a replica
-// replays the serial list: particles inside several colloids, every event of a
-// colloid whose bucket overflowed, and the continuations.
+// the reference's colloid loop serially (k_serial).  The pair partners j > g
+// (position and cell) are staged in LDS while the pair list is built, so the
+// continuation test of every event reads no global memory.
+struct PairColl {
+    double px, py, pz;
+    int cx, cy, cz;
+};
+
 template <class Real>
 __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Fluid<Real> f, int substep,
                                                              const int2 *wgt) {
     __shared__ double s_c[COLLIDE_THREADS / 64][EV_CAP][6];
-    __shared__ int s_ne[COLLIDE_THREADS / 64];
-    __shared__ int s_last;
-    __shared__ int s_chain[SERIAL_CAP], s_src[SERIAL_CAP];
-    __shared__ int s_nser, s_cand_n, s_serial_events, s_inj_tot;
-    __shared__ int s_cand[TCL_CAP];
-    __shared__ int s_wt[COLLIDE_THREADS / 64];
+    __shared__ PairColl s_nb[COLLIDE_THREADS / 64][MD_CAP];
+    __shared__ int s_ne[COLLIDE_THREADS / 64], s_nnb[COLLIDE_THREADS / 64];
     const bool fp32 = sizeof(Real) == 4;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int2 wt = wgt[blockIdx.x];
@@ -834,10 +928,41 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     const int L = R.L;
     const double Ld = (double)L;
     const int g = wt.y + w;
-    const int gend = R.first_colloid + R.nc;
-    const bool valid = g < gend;
+    const bool valid = g < R.first_colloid + R.nc;
     const long long step = *d.step;
-    if (valid) colloid_mdlist_wave(ph, d, R, r, g);
+    if (valid) {  // pair list of g (cpp/neighbour_list.cpp:27-39), partners j > g into LDS
+        const bool full = d.md_stale[r] != 0;
+        const int ncand = full ? R.nc : d.mdc_cnt[g];
+        const unsigned long long below = (1ull << lane) - 1ull;
+        int nmd = 0, nnb = 0;
+        for (int base = 0; base < ncand; base += 64) {
+            const int m = base + lane;
+            int j = -1;
+            bool p = false;
+            if (m < ncand) {
+                j = full ? R.first_colloid + m : d.mdc_list[g * MDC_CAP + m];
+                p = j != g && md_pair(ph, d, Ld, g, j);
+            }
+            const unsigned long long b = __ballot(p);
+            const int rank = __popcll(b & below);
+            if (p && nmd + rank < MD_CAP) d.md_list[g * MD_CAP + nmd + rank] = j;
+            nmd += __popcll(b);
+            const bool q = p && j > g && !(ph.surrogate && d.cmode[j]);
+            const unsigned long long bq = __ballot(q);
+            const int rq = __popcll(bq & below);
+            if (q && nnb + rq < MD_CAP) {
+                PairColl &e = s_nb[w][nnb + rq];
+                e.px = d.cpos[3 * j]; e.py = d.cpos[3 * j + 1]; e.pz = d.cpos[3 * j + 2];
+                e.cx = d.ccell[3 * j]; e.cy = d.ccell[3 * j + 1]; e.cz = d.ccell[3 * j + 2];
+            }
+            nnb += __popcll(bq);
+        }
+        if (lane == 0) {
+            if (nmd > MD_CAP) { atomicOr(d.status, BW_FLAG_LIST_OVERFLOW); nmd = MD_CAP; }
+            d.md_cnt[g] = nmd;
+            s_nnb[w] = min(nnb, MD_CAP);
+        }
+    }
     __syncthreads();  // pair lists visible to every lane
     int ne = 0;
     if (valid) {
@@ -887,16 +1012,15 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             // later colloid that now contains it is a pair partner of g; if
             // there is one, the rest of the colloid loop is replayed serially.
             const double ax = (double)ix + bx, ay = (double)iy + by, az = (double)iz + bz;
-            for (int m = 0; m < d.md_cnt[g]; m++) {
-                int j = d.md_list[g * MD_CAP + m];
-                if (j <= g || (ph.surrogate && d.cmode[j])) continue;
-                if (!in_stencil(ix, iy, iz, d.ccell[3 * j], d.ccell[3 * j + 1], d.ccell[3 * j + 2], L)) continue;
-                double rx = img1(d.cpos[3 * j] - ax, Ld), ry = img1(d.cpos[3 * j + 1] - ay, Ld),
-                       rz = img1(d.cpos[3 * j + 2] - az, Ld);
+            const int nnb = s_nnb[w];
+            for (int m = 0; m < nnb; m++) {
+                const PairColl &e = s_nb[w][m];
+                if (!in_stencil(ix, iy, iz, e.cx, e.cy, e.cz, L)) continue;
+                double rx = img1(e.px - ax, Ld), ry = img1(e.py - ay, Ld), rz = img1(e.pz - az, Ld);
                 if ((rx * rx + ry * ry) + rz * rz <= ph.lim_coll) {
-                    int e = atomicAdd(&d.cont_cnt[r], 1);
-                    if (e < CONT_CAP) {
-                        int q = r * CONT_CAP + e;
+                    int c = atomicAdd(&d.cont_cnt[r], 1);
+                    if (c < CONT_CAP) {
+                        int q = r * CONT_CAP + c;
                         d.cont_slot[q] = slot;
                         d.cont_from[q] = g + 1;
                         for (int kk = 0; kk < 3; kk++) d.cont_dump[3 * q + kk] = dump[kk];
@@ -922,30 +1046,37 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             d.cdw[3 * g + k] = a[3 + k];
         }
     }
-    // last workgroup of the replica
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int mine = min(COLLIDE_THREADS / 64, gend - wt.y);
-        unsigned t = atomicAdd(&d.ticket[r], (unsigned)mine);
-        s_last = (t + (unsigned)mine == (unsigned)R.nc) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+}
+
+// One workgroup per replica, after k_collide: the serial list -- particles
+// inside several colloids and the events of overflowed buckets (tested from
+// the replica's first colloid on, dump = current velocity), and the
+// continuations (from the colloid after the reflecting one, dump = velocity
+// before that reflection) -- replayed in ascending particle id, each particle
+// through the colloids whose stencil holds its binned cell in ascending order
+// (cpp/fluid_colloid_collision.cpp:29-49).  Then the injected-draw totals are
+// checked and the replica's counters cleared.
+template <class Real>
+__global__ __launch_bounds__(COLLIDE_THREADS) void k_serial(Phys ph, Dev d, Fluid<Real> f, int substep) {
+    __shared__ int s_chain[SERIAL_CAP], s_src[SERIAL_CAP];
+    __shared__ int s_nser, s_cand_n, s_serial_events, s_inj_tot;
+    __shared__ int s_cand[TCL_CAP];
+    __shared__ int s_wt[COLLIDE_THREADS / 64];
+    const bool fp32 = sizeof(Real) == 4;
+    const int r = blockIdx.x;
+    const Rep R = d.rep[r];
+    const int L = R.L;
+    const double Ld = (double)L;
+    const int gend = R.first_colloid + R.nc;
+    const long long step = *d.step;
     const int nchain = d.chain_cnt[r], ncont = d.cont_cnt[r], novf = min(d.ovf_cnt[r], OVF_CAP);
     if (threadIdx.x == 0) {
-        d.ticket[r] = 0;
         s_serial_events = 0;
         s_inj_tot = 0;
         s_nser = 0;
     }
     __syncthreads();
     if (nchain > 0 || ncont > 0 || novf > 0) {
-        // serial list: shared particles and overflowed buckets (tested from the
-        // replica's first colloid on, dump = current velocity) and
-        // continuations (from the colloid after the reflecting one, dump =
-        // velocity before that reflection); replayed in ascending particle id
         if (threadIdx.x == 0) {
             int n = 0;
             int nch = min(nchain, CHAIN_CAP);
@@ -1076,8 +1207,7 @@ __global__ __launch_bounds__(256) void k_colloid_update(Phys ph, Dev d, unsigned
     }
     __syncthreads();
     if (valid && lane == 0) {
-        if (flags & PH_APPLY) colloid_apply(ph, d, g);
-        if (flags & PH_ACTIVITY) colloid_activity(ph, d, g);
+        double fn[3] = {0, 0, 0};
         if (flags & PH_FORCE) {
             double f0 = 0, f1 = 0, f2 = 0;
             for (int m = 0; m < nmd; m++) {
@@ -1085,12 +1215,9 @@ __global__ __launch_bounds__(256) void k_colloid_update(Phys ph, Dev d, unsigned
                 if (s_sub[w][m]) { f0 = f0 - s_f[w][m][0]; f1 = f1 - s_f[w][m][1]; f2 = f2 - s_f[w][m][2]; }
                 else { f0 = f0 + s_f[w][m][0]; f1 = f1 + s_f[w][m][1]; f2 = f2 + s_f[w][m][2]; }
             }
-            d.cf[3 * g] = f0; d.cf[3 * g + 1] = f1; d.cf[3 * g + 2] = f2;
+            fn[0] = f0; fn[1] = f1; fn[2] = f2;
         }
-        if (flags & PH_VELOCITY) colloid_velocity(ph, d, g);
-        if (flags & PH_KICK) colloid_kick(ph, d, g);
-        if (flags & PH_OLDF) colloid_oldf(d, g);
-        if (flags & PH_POS) colloid_pos(ph, d, R.L, r, g, (flags & PH_MARGIN) != 0);
+        colloid_update(ph, d, R.L, r, g, flags, fn);
     }
 }
 
@@ -1416,8 +1543,7 @@ __global__ void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, lon
 
 size_t srd_lds_bytes(int ncell, int cap, bool fp64) {
     size_t rb = fp64 ? 8 : 4;
-    return (size_t)ncell * 7 * 8 + 3 * (size_t)cap * rb + (size_t)cap * 4 + (size_t)(2 * ncell + 1) * 4 +
-           (size_t)cap * 2 + 16;
+    return srd_cell_region(ncell, cap) + 3 * (size_t)cap * rb + (size_t)(2 * ncell + 1) * 4 + (size_t)cap * 2 + 16;
 }
 // static LDS of the SRD kernel's largest variant (colloid lists, masks)
 size_t srd_static_lds() { return sizeof(TileColl) + sizeof(NearColl) + 4 * TCL_CAP + 2 * sizeof(AxisMask) + 512; }
@@ -1429,12 +1555,28 @@ void launch_depart(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream
 template <class Real, bool C, bool M, bool B>
 static void srd_one(const Phys &ph, const Dev &d, const Fluid<Real> &src, const Fluid<Real> &dst, size_t lds,
                     hipStream_t st) {
-    LAUNCH((k_srd<Real, C, M, B>), ph.n_tiles, SRD_THREADS, lds, st, ph, d, src, dst);
+    switch (ph.srd_threads) {
+        case 256: LAUNCH((k_srd<Real, C, M, B, 256>), ph.n_tiles, 256, lds, st, ph, d, src, dst); break;
+        case 512: LAUNCH((k_srd<Real, C, M, B, 512>), ph.n_tiles, 512, lds, st, ph, d, src, dst); break;
+        default: LAUNCH((k_srd<Real, C, M, B, 1024>), ph.n_tiles, 1024, lds, st, ph, d, src, dst); break;
+    }
 }
 // Dynamic LDS up to the CU's 160 KB is granted at launch on gfx950 without an
 // opt-in attribute; the context checks the SRD pass's need against
 // 160 KB minus its static LDS when it is created.
-hipError_t prepare_kernels() { return hipSuccess; }
+// The near-particle pass loops over its chunks: as many workgroups as the
+// device holds at once (so that all start together), at most NEAR_BLOCKS.
+hipError_t prepare_kernels(Phys &ph, bool fp64) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = fp64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stream_near<double, false>, 256, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_stream_near<float, false>, 256, 0);
+    if (e != hipSuccess) return e;
+    ph.near_blocks = std::max(1, std::min(NEAR_BLOCKS, cus * std::max(per_cu, 1)));
+    return hipSuccess;
+}
 // count_shell: run() shell counts (colloids present); map: shifted-cell map of
 // every particle (the standalone rotation_mpcd); block: near/far split of the
 // fused step
@@ -1447,25 +1589,34 @@ void launch_srd(const Phys &ph, const Dev &d, const Fluid<Real> &src, const Flui
     else if (block) srd_one<Real, false, false, true>(ph, d, src, dst, lds, st);
     else srd_one<Real, false, false, false>(ph, d, src, dst, lds, st);
 }
+template <class Real, int TH>
+static void stream_one(const Phys &ph, const Dev &d, const Fluid<Real> &f, int m, int nf, hipStream_t st) {
+    switch (m) {
+        case 7: LAUNCH((k_stream<Real, true, true, true, TH>), ph.n_tiles, TH, 0, st, ph, d, f, nf); break;
+        case 6: LAUNCH((k_stream<Real, true, true, false, TH>), ph.n_tiles, TH, 0, st, ph, d, f, nf); break;
+        case 4: LAUNCH((k_stream<Real, true, false, false, TH>), ph.n_tiles, TH, 0, st, ph, d, f, nf); break;
+        case 3: LAUNCH((k_stream<Real, false, true, true, TH>), ph.n_tiles, TH, 0, st, ph, d, f, nf); break;
+        case 2: LAUNCH((k_stream<Real, false, true, false, TH>), ph.n_tiles, TH, 0, st, ph, d, f, nf); break;
+        case 1: LAUNCH((k_stream<Real, false, false, true, TH>), ph.n_tiles, TH, 0, st, ph, d, f, nf); break;
+        default: break;
+    }
+}
 template <class Real>
 void launch_stream(const Phys &ph, const Dev &d, const Fluid<Real> &f, bool kick, bool stream, bool detect,
                    bool near_only, hipStream_t st) {
     int m = (kick ? 4 : 0) | (stream ? 2 : 0) | (detect ? 1 : 0);
-    int nf = near_only ? 1 : 0;
-    switch (m) {
-        case 7: LAUNCH((k_stream<Real, true, true, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
-        case 6: LAUNCH((k_stream<Real, true, true, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
-        case 4: LAUNCH((k_stream<Real, true, false, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
-        case 3: LAUNCH((k_stream<Real, false, true, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
-        case 2: LAUNCH((k_stream<Real, false, true, false>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
-        case 1: LAUNCH((k_stream<Real, false, false, true>), ph.n_tiles, STREAM_THREADS, 0, st, ph, d, f, nf); break;
-        default: break;
-    }
+    stream_one<Real, STREAM_THREADS>(ph, d, f, m, near_only ? 1 : 0, st);
+}
+template <class Real>
+void launch_stream_near(const Phys &ph, const Dev &d, const Fluid<Real> &f, bool kick, hipStream_t st) {
+    if (kick) LAUNCH((k_stream_near<Real, true>), ph.near_blocks, 256, 0, st, ph, d, f);
+    else LAUNCH((k_stream_near<Real, false>), ph.near_blocks, 256, 0, st, ph, d, f);
 }
 template <class Real>
 void launch_collide(const Phys &ph, const Dev &d, const Fluid<Real> &f, int substep, const int2 *wgt, int nwg,
                     hipStream_t st) {
     LAUNCH(k_collide<Real>, nwg, COLLIDE_THREADS, 0, st, ph, d, f, substep, wgt);
+    LAUNCH(k_serial<Real>, ph.n_rep, COLLIDE_THREADS, 0, st, ph, d, f, substep);
 }
 template <class Real>
 void launch_run_count(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream_t st) {
@@ -1532,6 +1683,7 @@ void launch_copy(const void *a, void *b, long long nbytes, hipStream_t st) {
                                       hipStream_t);                                                                \
     template void launch_collide<Real>(const Phys &, const Dev &, const Fluid<Real> &, int, const int2 *, int,    \
                                        hipStream_t);                                                               \
+    template void launch_stream_near<Real>(const Phys &, const Dev &, const Fluid<Real> &, bool, hipStream_t);    \
     template void launch_run_count<Real>(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);            \
     template void launch_maps<Real>(const Phys &, const Dev &, const Fluid<Real> &, long long, int *,             \
                                     const int *, hipStream_t);                                                     \
