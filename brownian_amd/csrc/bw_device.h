@@ -156,6 +156,11 @@ __device__ __forceinline__ int pdiff(int a, int b, int L) {
     if (d < -(L / 2)) d += L;
     return d;
 }
+// a / b for 0 <= a < 1024 and 1 <= b <= 63 by one multiply: with
+// m = ceil(2^16 / b), a*m/2^16 exceeds a/b by less than a/2^16 < 1/b, the
+// least distance from a/b up to the next integer
+__device__ __forceinline__ unsigned div_small_magic(int b) { return (65536u + (unsigned)b - 1u) / (unsigned)b; }
+__device__ __forceinline__ int div_small(int a, unsigned m) { return (int)(((unsigned)a * m) >> 16); }
 // a mod n in [0, n); one conditional step covers a in [-n, 2n)
 __device__ __forceinline__ int wrapi(int a, int n) {
     if (a < 0) a += n;
@@ -219,6 +224,7 @@ __device__ __forceinline__ bool in_unit(float o) { return __float_as_uint(o) < 0
 __device__ __forceinline__ bool in_unit(double o) {
     return (unsigned long long)__double_as_longlong(o) < 0x3FF0000000000000ull;
 }
+
 
 // ------------------------------------------------------------ reflection
 // Stochastic (thermal-wall) reflection, cpp/fluid_colloid_collision.cpp:7-24:

@@ -129,6 +129,7 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
     const int L = tg.L;
     const int n = min(d.count[T], ph.cap);
     const double sx = d.shift[3 * tg.r], sy = d.shift[3 * tg.r + 1], sz = d.shift[3 * tg.r + 2];
+    const unsigned mx = div_small_magic(R.B[0]), my = div_small_magic(R.B[1]), mz = div_small_magic(R.B[2]);
     if (threadIdx.x < 28) s_off[threadIdx.x] = 0;
     if (threadIdx.x < 27) s_run[threadIdx.x] = 0;
     __syncthreads();
@@ -140,9 +141,9 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
         int kx = shifted_coord(ix, f.ox[slot], sx, L);
         int ky = shifted_coord(iy, f.oy[slot], sy, L);
         int kz = shifted_coord(iz, f.oz[slot], sz, L);
-        int dx = tile_delta(kx / R.B[0], tg.tx, R.nt[0]);
-        int dy = tile_delta(ky / R.B[1], tg.ty, R.nt[1]);
-        int dz = tile_delta(kz / R.B[2], tg.tz, R.nt[2]);
+        int dx = tile_delta(div_small(kx, mx), tg.tx, R.nt[0]);
+        int dy = tile_delta(div_small(ky, my), tg.ty, R.nt[1]);
+        int dz = tile_delta(div_small(kz, mz), tg.tz, R.nt[2]);
         if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) {
             drift = true;
             dx = max(-1, min(1, dx)); dy = max(-1, min(1, dy)); dz = max(-1, min(1, dz));
@@ -262,6 +263,14 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     for (int c = threadIdx.x; c < ncell + 1; c += blockDim.x) s_start[c] = 0;
     if (COUNT_SHELL)
         for (int k = threadIdx.x; k < TCL_CAP; k += blockDim.x) s_shell[k] = 0;
+    // the tile's colloid list, requested now so that it arrives with the
+    // particles (the step's list pass ran before this kernel; a list made
+    // stale since is rebuilt below)
+    int cl_n = 0, cl_g = 0;
+    if (COUNT_SHELL) {
+        cl_n = min(d.tcl_cnt[T], TCL_CAP);
+        if (threadIdx.x < cl_n) cl_g = d.tcl_list[T * TCL_CAP + threadIdx.x];
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         int acc = 0;
@@ -272,44 +281,16 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         }
         s_pre[27] = acc;
     }
-    Region rg = tile_region(R, tg.tx, tg.ty, tg.tz);
-    if (COUNT_SHELL && ph.surrogate) {
-        const int ns = min(d.tsl_cnt[T], TCL_CAP);
-        for (int q = threadIdx.x; q < ns; q += blockDim.x) {
-            const int g = d.tsl_list[T * TCL_CAP + q];
-            for (int k = 0; k < 3; k++) {
-                s_sp[q][k] = d.cpos[3 * g + k];
-                s_sp[q][3 + k] = d.cvel[3 * g + k];
-            }
-        }
-        if (threadIdx.x == 0) s_sn = ns;
-    }
-    if (COUNT_SHELL) {
-        load_tile_colloids(d, ph, R, tg.r, T, true, tc);
-        build_axis_masks(tc, rg, L, true, 3, s_am[0]);
-        if (NEAR_TEST) {
-            build_axis_masks(tc, rg, L, false, ph.near_reach, s_am[COUNT_SHELL ? 1 : 0]);
-            const double Ld = (double)L;
-            for (int q = threadIdx.x; q < tc.n; q += blockDim.x) {
-                nc[0].cx[q] = (float)img1(tc.px[q] - ox0, Ld);
-                nc[0].cy[q] = (float)img1(tc.py[q] - oy0, Ld);
-                nc[0].cz[q] = (float)img1(tc.pz[q] - oz0, Ld);
-                nc[0].rx[q] = (float)img1(tc.rx[q] - ox0, Ld);
-                nc[0].ry[q] = (float)img1(tc.ry[q] - oy0, Ld);
-                nc[0].rz[q] = (float)img1(tc.rz[q] - oz0, Ld);
-            }
-        }
-    }
     __syncthreads();
     int n = s_pre[27];
     if (n > cap) {
         if (threadIdx.x == 0) atomicOr(d.status, BW_FLAG_TILE_OVERFLOW);
         n = cap;
     }
-
+    // gather: the loads are issued here and consumed after the colloid set-up,
+    // which runs while they are in flight
     int rcell[SRD_PT], rpid[SRD_PT], rkey[SRD_PT];
     Real rox[SRD_PT], roy[SRD_PT], roz[SRD_PT], rvx[SRD_PT], rvy[SRD_PT], rvz[SRD_PT];
-    bool drift = false;
 #pragma unroll
     for (int k = 0; k < SRD_PT; k++) {
         int g = threadIdx.x + k * TH;
@@ -328,6 +309,48 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             rox[k] = src.ox[slot]; roy[k] = src.oy[slot]; roz[k] = src.oz[slot];
             rvx[k] = src.vx[slot]; rvy[k] = src.vy[slot]; rvz[k] = src.vz[slot];
             rpid[k] = src.pid[slot];
+        }
+    }
+    Region rg = tile_region(R, tg.tx, tg.ty, tg.tz);
+    if (COUNT_SHELL && ph.surrogate) {
+        const int ns = min(d.tsl_cnt[T], TCL_CAP);
+        for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+            const int g = d.tsl_list[T * TCL_CAP + q];
+            for (int k = 0; k < 3; k++) {
+                s_sp[q][k] = d.cpos[3 * g + k];
+                s_sp[q][3 + k] = d.cvel[3 * g + k];
+            }
+        }
+        if (threadIdx.x == 0) s_sn = ns;
+    }
+    if (COUNT_SHELL) {
+        if (d.tcl_stale[tg.r]) {
+            load_tile_colloids(d, ph, R, tg.r, T, true, tc);
+        } else {
+            if (threadIdx.x < cl_n) load_tile_slot(d, ph, cl_g, threadIdx.x, true, tc);
+            if (threadIdx.x == 0) tc.n = cl_n;
+            __syncthreads();
+        }
+        build_axis_masks(tc, rg, L, true, 3, s_am[0]);
+        if (NEAR_TEST) {
+            build_axis_masks(tc, rg, L, false, ph.near_reach, s_am[COUNT_SHELL ? 1 : 0]);
+            const double Ld = (double)L;
+            for (int q = threadIdx.x; q < tc.n; q += blockDim.x) {
+                nc[0].cx[q] = (float)img1(tc.px[q] - ox0, Ld);
+                nc[0].cy[q] = (float)img1(tc.py[q] - oy0, Ld);
+                nc[0].cz[q] = (float)img1(tc.pz[q] - oz0, Ld);
+                nc[0].rx[q] = (float)img1(tc.rx[q] - ox0, Ld);
+                nc[0].ry[q] = (float)img1(tc.ry[q] - oy0, Ld);
+                nc[0].rz[q] = (float)img1(tc.rz[q] - oz0, Ld);
+            }
+        }
+    }
+    // shifted cells and the per-cell histogram
+    bool drift = false;
+#pragma unroll
+    for (int k = 0; k < SRD_PT; k++) {
+        int g = threadIdx.x + k * TH;
+        if (g < n) {
             int ix, iy, iz;
             unpack_cell(rcell[k], ix, iy, iz);
             int lx = shifted_coord(ix, rox[k], sx, L) - ox0;
@@ -576,6 +599,9 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             if (lane == 0) s_cc[k][w] = __popcll(bal[k]);
         }
         __syncthreads();
+        if (COUNT_SHELL)  // the apply phase's shell counts are complete
+            for (int q = threadIdx.x; q < tc.n; q += blockDim.x)
+                if (s_shell[q]) atomicAdd(&d.ccnt[tc.g[q]], s_shell[q]);
         int base = 0;
 #pragma unroll
         for (int k = 0; k < SRD_PT; k++) {
@@ -589,6 +615,13 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             base += tot;
         }
         n_near = base;
+        if (COUNT_SHELL && w == 0) {  // the near prefix in chunks of 64 for the per-substep pass
+            const int nch = (n_near + 63) >> 6;
+            int c0 = 0;
+            if (lane == 0 && nch > 0) c0 = atomicAdd(d.chunk_cnt, nch);
+            c0 = __shfl(c0, 0);
+            for (int j = lane; j < nch; j += 64) d.chunk_tile[c0 + j] = (T << 6) | j;
+        }
     }
     bool far_drift = false;
 #pragma unroll
@@ -643,14 +676,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         d.count_next[T] = n;
         d.near_cnt[T] = n_near;
     }
-    if (BLOCK && COUNT_SHELL) {  // the near prefix in chunks of 64 for the per-substep pass
-        __shared__ int s_chunk0;
-        const int nch = (n_near + 63) >> 6;
-        if (threadIdx.x == 0) s_chunk0 = nch > 0 ? atomicAdd(d.chunk_cnt, nch) : 0;
-        __syncthreads();
-        for (int j = threadIdx.x; j < nch; j += blockDim.x) d.chunk_tile[s_chunk0 + j] = (T << 6) | j;
-    }
-    if (COUNT_SHELL) {
+    if (COUNT_SHELL && !BLOCK) {
         __syncthreads();
         for (int q = threadIdx.x; q < tc.n; q += blockDim.x)
             if (s_shell[q]) atomicAdd(&d.ccnt[tc.g[q]], s_shell[q]);
