@@ -602,10 +602,33 @@ __device__ inline void colloid_force_serial(const Phys &ph, const Dev &d, double
     }
     f[0] = f0; f[1] = f1; f[2] = f2;
 }
-// The per-colloid phases of one substep, in PH_* order, by one thread:
-// every input is loaded up front, the phases run in registers, and the
-// results are stored at the end (each phase feeds the next without a round
-// trip through memory).  fn is the new WCA force when PH_FORCE is set.
+struct ColIn {  // the inputs of colloid_update, loaded ahead of the force terms
+    double v[3], w[3], ra[3], f[3], of[3], p0[3], dv[3], dw[3], sv[6], disp[3], pmd[3], v0;
+    int cell[3], cmd[3];
+    bool sur;
+};
+__device__ inline void colloid_load(const Phys &ph, const Dev &d, int g, unsigned flags, ColIn &in) {
+    for (int k = 0; k < 3; k++) {
+        in.v[k] = d.cvel[3 * g + k];
+        in.w[k] = d.comega[3 * g + k];
+        in.ra[k] = d.cra[3 * g + k];
+        in.f[k] = (flags & PH_FORCE) ? 0.0 : d.cf[3 * g + k];
+        in.of[k] = d.coldf[3 * g + k];
+        in.p0[k] = d.cpos[3 * g + k];
+        in.dv[k] = (flags & PH_APPLY) ? d.cdv[3 * g + k] : 0.0;
+        in.dw[k] = (flags & PH_APPLY) ? d.cdw[3 * g + k] : 0.0;
+        in.cell[k] = (flags & PH_KICK) ? d.ccell[3 * g + k] : 0;
+        in.disp[k] = (flags & PH_POS) ? d.cdisp[3 * g + k] : 0.0;
+        in.pmd[k] = (flags & PH_POS) ? d.cpos_md[3 * g + k] : 0.0;
+        in.cmd[k] = (flags & PH_POS) ? d.ccell_md[3 * g + k] : 0;
+    }
+    in.sur = (flags & PH_APPLY) && ph.surrogate && d.cmode[g];
+    for (int k = 0; k < 6; k++) in.sv[k] = ((flags & PH_APPLY) && ph.surrogate) ? d.csur[6 * g + k] : 0.0;
+    in.v0 = (flags & PH_KICK) ? d.cv0[g] : 0.0;
+}
+// The per-colloid phases of one substep, in PH_* order, by one thread, on the
+// inputs loaded by colloid_load: the phases run in registers and the results
+// are stored at the end.  fn is the new WCA force when PH_FORCE is set.
 //  PH_APPLY     cpp/fluid_colloid_collision.cpp:46-47 (the sums dv, dw were
 //               formed in the reference's event order); a surrogate-driven
 //               colloid receives its sampled impulse instead, and the event
@@ -618,23 +641,19 @@ __device__ inline void colloid_force_serial(const Phys &ph, const Dev &d, double
 //               the replica's pair candidates / tile lists stale if the
 //               colloid left their margins
 __device__ inline void colloid_update(const Phys &ph, const Dev &d, int L, int r, int g, unsigned flags,
-                                      const double *fn) {
-    double v[3], w[3], ra[3], f[3], of[3], p0[3], dv[3] = {0, 0, 0}, dw[3] = {0, 0, 0};
+                                      const double *fn, const ColIn &in) {
+    double v[3], w[3], ra[3], f[3], of[3];
     for (int k = 0; k < 3; k++) {
-        v[k] = d.cvel[3 * g + k];
-        w[k] = d.comega[3 * g + k];
-        ra[k] = d.cra[3 * g + k];
-        f[k] = (flags & PH_FORCE) ? fn[k] : d.cf[3 * g + k];
-        of[k] = d.coldf[3 * g + k];
-        p0[k] = d.cpos[3 * g + k];
+        v[k] = in.v[k];
+        w[k] = in.w[k];
+        ra[k] = in.ra[k];
+        f[k] = (flags & PH_FORCE) ? fn[k] : in.f[k];
+        of[k] = in.of[k];
     }
     if (flags & PH_APPLY) {
-        const bool sur = ph.surrogate && d.cmode[g];
         for (int k = 0; k < 3; k++) {
-            dv[k] = d.cdv[3 * g + k];
-            dw[k] = d.cdw[3 * g + k];
-            const double iv = sur ? d.csur[6 * g + k] : dv[k];
-            const double iw = sur ? d.csur[6 * g + 3 + k] : dw[k];
+            const double iv = in.sur ? in.sv[k] : in.dv[k];
+            const double iw = in.sur ? in.sv[3 + k] : in.dw[k];
             v[k] = v[k] + iv * ph.mass_fl / ph.mass_colloid;
             w[k] = w[k] + iw * ph.mass_fl / ph.I_colloid;
         }
@@ -655,11 +674,10 @@ __device__ inline void colloid_update(const Phys &ph, const Dev &d, int L, int r
         for (int k = 0; k < 3; k++) v[k] = v[k] + (of[k] + f[k]) * dtb2;
     }
     if (flags & PH_KICK) {
-        double v0 = d.cv0[g];
         for (int k = 0; k < 3; k++) {
-            v[k] = v[k] + ra[k] * v0;
-            d.cpos_run[3 * g + k] = p0[k];
-            d.ccell_run[3 * g + k] = d.ccell[3 * g + k];
+            v[k] = v[k] + ra[k] * in.v0;
+            d.cpos_run[3 * g + k] = in.p0[k];
+            d.ccell_run[3 * g + k] = in.cell[k];
         }
         d.ccnt[g] = 0;
     }
@@ -671,15 +689,15 @@ __device__ inline void colloid_update(const Phys &ph, const Dev &d, int L, int r
         double m2 = 0;
         for (int k = 0; k < 3; k++) {
             double dx = v[k] * ph.dt + f[k] * ddt;
-            double p = p0[k] + dx;
-            d.cdisp[3 * g + k] += dx;
+            double p = in.p0[k] + dx;
+            d.cdisp[3 * g + k] = in.disp[k] + dx;
             p = mod1(p, (double)L);
             d.cpos_nxt[3 * g + k] = p;
             int cc = coord_cell(p, L);
             d.ccell_nxt[3 * g + k] = cc;
-            int dm = pdiff(cc, d.ccell_md[3 * g + k], L);
+            int dm = pdiff(cc, in.cmd[k], L);
             if (dm > 1 || dm < -1) far_tl = true;
-            double md = img1(p - d.cpos_md[3 * g + k], (double)L);
+            double md = img1(p - in.pmd[k], (double)L);
             m2 += md * md;
         }
         if (m2 > 0.25 * MD_SKIN * MD_SKIN) atomicOr(&d.md_stale[r], 1);
@@ -688,8 +706,8 @@ __device__ inline void colloid_update(const Phys &ph, const Dev &d, int L, int r
     }
     for (int k = 0; k < 3; k++) {
         if (flags & PH_APPLY) {
-            d.cimp[6 * g + k] += dv[k];
-            d.cimp[6 * g + 3 + k] += dw[k];
+            d.cimp[6 * g + k] += in.dv[k];
+            d.cimp[6 * g + 3 + k] += in.dw[k];
             d.cdv[3 * g + k] = 0;
             d.cdw[3 * g + k] = 0;
         }
@@ -731,9 +749,11 @@ __device__ inline void colloid_phases(const Phys &ph, const Dev &d, const Rep &R
     const double L = (double)R.L;
     for (int i = threadIdx.x; i < nc; i += blockDim.x) {
         int g = g0 + i;
+        ColIn in;
+        colloid_load(ph, d, g, flags, in);
         double fn[3] = {0, 0, 0};
         if (flags & PH_FORCE) colloid_force_serial(ph, d, L, g, fn);
-        colloid_update(ph, d, R.L, r, g, flags, fn);
+        colloid_update(ph, d, R.L, r, g, flags, fn, in);
     }
     __syncthreads();
     if (flags & PH_MDLIST)

@@ -83,12 +83,46 @@ __global__ __launch_bounds__(256) void k_md_candidates(Phys ph, Dev d, const int
     }
 }
 
+// Per-tile colloid lists of the step, built colloid-first: the counters are
+// cleared, then each colloid tests the tiles around its current and run()
+// cells with the list's own membership rule and appends itself.  A list's
+// order is then arbitrary; every user treats it as a set (masks, exact
+// tests, counts).
+__global__ __launch_bounds__(256) void k_tile_lists_clear(Phys ph, Dev d) {
+    for (int T = blockIdx.x * blockDim.x + threadIdx.x; T < ph.n_tiles; T += gridDim.x * blockDim.x) {
+        d.tcl_cnt[T] = 0;
+        if (ph.surrogate) d.tsl_cnt[T] = 0;
+    }
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < ph.n_rep; r += gridDim.x * blockDim.x) d.tcl_stale[r] = 0;
+}
 __global__ __launch_bounds__(256) void k_tile_colloids(Phys ph, Dev d) {
-    const int T = blockIdx.x;
-    const int r = d.tile_rep[T];
-    build_tile_list(ph, d, d.rep[r], T);
-    if (ph.surrogate) build_tile_list(ph, d, d.rep[r], T, true);
-    if (threadIdx.x == 0) d.tcl_stale[r] = 0;
+    // one workgroup per colloid, threads over its candidate tiles (at most 6
+    // per axis when the two cells are neighbours: the window spans B + 15
+    // cells, B >= 4)
+    const int g = blockIdx.x;
+    const Rep R = d.rep[d.crep[g]];
+    const bool sur = ph.surrogate && d.cmode[g];
+    int *list = sur ? d.tsl_list : d.tcl_list;
+    int *cnt = sur ? d.tsl_cnt : d.tcl_cnt;
+    int t0[3], nt[3];
+    for (int a = 0; a < 3; a++) {  // tiles whose window can hold either cell, per axis
+        const int B = R.B[a], c1 = d.ccell[3 * g + a], c2 = d.ccell_run[3 * g + a];
+        const int lo = min(c1, c2) - TILE_LIST_REACH - (B - 1), hi = max(c1, c2) + TILE_LIST_REACH;
+        const int tlo = lo >= 0 ? lo / B : -((-lo + B - 1) / B), thi = hi / B;
+        t0[a] = wrapi(tlo, R.nt[a]);
+        nt[a] = min(thi - tlo + 1, R.nt[a]);
+    }
+    for (int q = threadIdx.x; q < nt[0] * nt[1] * nt[2]; q += blockDim.x) {
+        const int i = q % nt[0], j = (q / nt[0]) % nt[1], k = q / (nt[0] * nt[1]);
+        const int tx = wrapi(t0[0] + i, R.nt[0]), ty = wrapi(t0[1] + j, R.nt[1]), tz = wrapi(t0[2] + k, R.nt[2]);
+        const int T = R.first_tile + tx + R.nt[0] * (ty + R.nt[1] * tz);
+        int lo3[3], ln3[3];
+        tile_list_window(R, T, lo3, ln3);
+        if (!tile_list_member(ph, d, R, g, lo3, ln3, sur)) continue;
+        const int e = atomicAdd(&cnt[T], 1);
+        if (e < TCL_CAP) list[T * TCL_CAP + e] = g;
+        else atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
+    }
 }
 
 // ------------------------------------------------------------ tile geometry
@@ -956,9 +990,39 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     const int g = wt.y + w;
     const bool valid = g < R.first_colloid + R.nc;
     const long long step = *d.step;
+    // The event bucket and the pair candidates are read together, then the
+    // events' particles and the candidates' positions, so that the two chains
+    // of dependent loads run side by side.
+    const bool full = d.md_stale[r] != 0;
+    const int ncand = valid ? (full ? R.nc : d.mdc_cnt[g]) : 0;
+    int j0 = -1;  // this lane's first candidate
+    if (lane < ncand) j0 = full ? R.first_colloid + lane : d.mdc_list[g * MDC_CAP + lane];
+    int ne = 0;
+    if (valid) {
+        ne = d.ev_cnt[g];
+        if (lane == 0) {
+            d.ev_cnt[g] = 0;
+            if (ne > EV_CAP) {
+                int k = atomicAdd(&d.ovf_cnt[r], 1);
+                if (k < OVF_CAP) d.ovf_list[r * OVF_CAP + k] = g;
+                else atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
+            } else if (ne > 0 && ph.draws == BW_DRAWS_INJECTED) {
+                atomicAdd(&d.ev_total[r], ne);
+            }
+        }
+        if (ne > EV_CAP) ne = 0;  // full bucket: all its events are replayed serially
+    }
+    int slot = -1, pid = 0, ix = 0, iy = 0, iz = 0;
+    long long key = 0x7fffffffffffffffll;
+    Real pox = 0, poy = 0, poz = 0, pvx = 0, pvy = 0, pvz = 0;
+    if (lane < ne) {
+        slot = d.ev_slot[g * EV_CAP + lane];
+        pid = f.pid[slot];
+        unpack_cell(f.cell[slot], ix, iy, iz);
+        pox = f.ox[slot]; poy = f.oy[slot]; poz = f.oz[slot];
+        pvx = f.vx[slot]; pvy = f.vy[slot]; pvz = f.vz[slot];
+    }
     if (valid) {  // pair list of g (cpp/neighbour_list.cpp:27-39), partners j > g into LDS
-        const bool full = d.md_stale[r] != 0;
-        const int ncand = full ? R.nc : d.mdc_cnt[g];
         const unsigned long long below = (1ull << lane) - 1ull;
         int nmd = 0, nnb = 0;
         for (int base = 0; base < ncand; base += 64) {
@@ -966,7 +1030,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             int j = -1;
             bool p = false;
             if (m < ncand) {
-                j = full ? R.first_colloid + m : d.mdc_list[g * MDC_CAP + m];
+                j = base == 0 ? j0 : (full ? R.first_colloid + m : d.mdc_list[g * MDC_CAP + m]);
                 p = j != g && md_pair(ph, d, Ld, g, j);
             }
             const unsigned long long b = __ballot(p);
@@ -990,27 +1054,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
         }
     }
     __syncthreads();  // pair lists visible to every lane
-    int ne = 0;
-    if (valid) {
-        ne = d.ev_cnt[g];
-        if (lane == 0) {
-            d.ev_cnt[g] = 0;
-            if (ne > EV_CAP) {
-                int k = atomicAdd(&d.ovf_cnt[r], 1);
-                if (k < OVF_CAP) d.ovf_list[r * OVF_CAP + k] = g;
-                else atomicOr(d.status, BW_FLAG_EVENT_OVERFLOW);
-            } else if (ne > 0 && ph.draws == BW_DRAWS_INJECTED) {
-                atomicAdd(&d.ev_total[r], ne);
-            }
-        }
-        if (ne > EV_CAP) ne = 0;  // full bucket: all its events are replayed serially
-    }
-    int slot = -1, pid = 0, ix = 0, iy = 0, iz = 0;
-    long long key = 0x7fffffffffffffffll;
     if (lane < ne) {
-        slot = d.ev_slot[g * EV_CAP + lane];
-        pid = f.pid[slot];
-        unpack_cell(f.cell[slot], ix, iy, iz);
         int dx = pdiff(ix, d.ccell[3 * g], L), dy = pdiff(iy, d.ccell[3 * g + 1], L),
             dz = pdiff(iz, d.ccell[3 * g + 2], L);
         key = ((long long)stencil_index(dx, dy, dz) << 32) | (unsigned)pid;
@@ -1021,8 +1065,8 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
         rank += (km < key) ? 1 : 0;
     }
     if (lane < ne) {
-        double bx = f.ox[slot], by = f.oy[slot], bz = f.oz[slot];
-        double vx = f.vx[slot], vy = f.vy[slot], vz = f.vz[slot];
+        double bx = pox, by = poy, bz = poz;
+        double vx = pvx, vy = pvy, vz = pvz;
         ReflDraw dr;
         double cont[6] = {0, 0, 0, 0, 0, 0};
         const double dump[3] = {vx, vy, vz};
@@ -1219,6 +1263,8 @@ __global__ __launch_bounds__(256) void k_colloid_update(Phys ph, Dev d, unsigned
     const int g = wt.y + w;
     const bool valid = g < R.first_colloid + R.nc;
     const double L = (double)R.L;
+    ColIn in;  // lane 0: the colloid's own state, in flight with the force terms
+    if (valid && lane == 0) colloid_load(ph, d, g, flags, in);
     int nmd = 0;
     if (valid && (flags & PH_FORCE)) {
         nmd = d.md_cnt[g];
@@ -1243,7 +1289,7 @@ __global__ __launch_bounds__(256) void k_colloid_update(Phys ph, Dev d, unsigned
             }
             fn[0] = f0; fn[1] = f1; fn[2] = f2;
         }
-        colloid_update(ph, d, R.L, r, g, flags, fn);
+        colloid_update(ph, d, R.L, r, g, flags, fn, in);
     }
 }
 
@@ -1682,7 +1728,8 @@ void launch_md_candidates(const Phys &ph, const Dev &d, const int2 *wgt, int nwg
     LAUNCH(k_md_candidates, nwg, 256, 0, st, ph, d, wgt);
 }
 void launch_tile_colloids(const Phys &ph, const Dev &d, hipStream_t st) {
-    LAUNCH(k_tile_colloids, ph.n_tiles, 256, 0, st, ph, d);
+    LAUNCH(k_tile_lists_clear, (ph.n_tiles + 255) / 256, 256, 0, st, ph, d);
+    if (ph.n_colloid_total > 0) LAUNCH(k_tile_colloids, ph.n_colloid_total, 256, 0, st, ph, d);
 }
 void launch_features(const Phys &ph, const Dev &d, const int *colloid_rep, hipStream_t st) {
     LAUNCH(k_features, (ph.n_colloid_total + 63) / 64, 64, 0, st, ph, d, colloid_rep);
