@@ -154,76 +154,89 @@ __device__ __forceinline__ int tile_delta(int tnew, int told, int nt) {
 // shift, as a direction 0..26; stable (array-order) lists per direction.
 template <class Real>
 __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid<Real> f) {
+    constexpr int NW = DEPART_THREADS / 64, U = 4;  // waves; 64-particle groups whose loads fly together
     __shared__ unsigned char s_dir[MAX_CAP];
-    __shared__ int s_off[28], s_run[27];
-    __shared__ int s_w[DEPART_THREADS / 64][27];
+    __shared__ int s_cnt[NW][27], s_base[NW][27];
     const int T = blockIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     Rep R;
     TileGeo tg = tile_geo(d, T, R);
     const int L = tg.L;
     const int n = min(d.count[T], ph.cap);
     const double sx = d.shift[3 * tg.r], sy = d.shift[3 * tg.r + 1], sz = d.shift[3 * tg.r + 2];
     const unsigned mx = div_small_magic(R.B[0]), my = div_small_magic(R.B[1]), mz = div_small_magic(R.B[2]);
-    if (threadIdx.x < 28) s_off[threadIdx.x] = 0;
-    if (threadIdx.x < 27) s_run[threadIdx.x] = 0;
-    __syncthreads();
+    // each wave owns a contiguous range of the tile's slots, in groups of 64
+    const int per = ((n + NW - 1) / NW + 63) & ~63;
+    const int lo = min(n, w * per), hi = min(n, lo + per);
+    for (int t = lane; t < 27; t += 64) s_cnt[w][t] = 0;
     bool drift = false;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        int slot = T * ph.cap + i;
-        int ix, iy, iz;
-        unpack_cell(f.cell[slot], ix, iy, iz);
-        int kx = shifted_coord(ix, f.ox[slot], sx, L);
-        int ky = shifted_coord(iy, f.oy[slot], sy, L);
-        int kz = shifted_coord(iz, f.oz[slot], sz, L);
-        int dx = tile_delta(div_small(kx, mx), tg.tx, R.nt[0]);
-        int dy = tile_delta(div_small(ky, my), tg.ty, R.nt[1]);
-        int dz = tile_delta(div_small(kz, mz), tg.tz, R.nt[2]);
-        if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) {
-            drift = true;
-            dx = max(-1, min(1, dx)); dy = max(-1, min(1, dy)); dz = max(-1, min(1, dz));
+    // pass 1: the departure direction of each particle, counts per wave
+    for (int g0 = lo; g0 < hi; g0 += 64 * U) {
+        int c[U];
+        Real ox[U], oy[U], oz[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = g0 + 64 * u + lane;
+            if (i < hi) {
+                const int slot = T * ph.cap + i;
+                c[u] = f.cell[slot];
+                ox[u] = f.ox[slot]; oy[u] = f.oy[slot]; oz[u] = f.oz[slot];
+            }
         }
-        int dir = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-        s_dir[i] = (unsigned char)dir;
-        atomicAdd(&s_off[dir], 1);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = g0 + 64 * u + lane;
+            if (i < hi) {
+                int ix, iy, iz;
+                unpack_cell(c[u], ix, iy, iz);
+                int kx = shifted_coord(ix, ox[u], sx, L);
+                int ky = shifted_coord(iy, oy[u], sy, L);
+                int kz = shifted_coord(iz, oz[u], sz, L);
+                int dx = tile_delta(div_small(kx, mx), tg.tx, R.nt[0]);
+                int dy = tile_delta(div_small(ky, my), tg.ty, R.nt[1]);
+                int dz = tile_delta(div_small(kz, mz), tg.tz, R.nt[2]);
+                if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) {
+                    drift = true;
+                    dx = max(-1, min(1, dx)); dy = max(-1, min(1, dy)); dz = max(-1, min(1, dz));
+                }
+                const int dir = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                s_dir[i] = (unsigned char)dir;
+                atomicAdd(&s_cnt[w][dir], 1);
+            }
+        }
     }
     if (drift) atomicOr(d.status, BW_FLAG_DRIFT);
     __syncthreads();
+    // list offsets: direction-major, waves in order within a direction
     if (threadIdx.x == 0) {
         int acc = 0;
-        for (int k = 0; k < 27; k++) {
-            int c = s_off[k];
-            s_off[k] = acc;
-            d.dep_off[T * 28 + k] = acc;
-            acc += c;
+        for (int t = 0; t < 27; t++) {
+            d.dep_off[T * 28 + t] = acc;
+            for (int v = 0; v < NW; v++) {
+                s_base[v][t] = acc;
+                acc += s_cnt[v][t];
+            }
         }
-        s_off[27] = acc;
         d.dep_off[T * 28 + 27] = acc;
     }
     __syncthreads();
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    for (int base = 0; base < n; base += blockDim.x) {
-        int i = base + threadIdx.x;
-        int dd = i < n ? (int)s_dir[i] : 255;
-        int myrank = 0;
-        for (int t = 0; t < 27; t++) {
-            unsigned long long m = __ballot(dd == t);
-            if (dd == t) myrank = __popcll(m & lt);
-            if (lane == 0) s_w[w][t] = __popcll(m);
+    // pass 2: each wave scatters its range in slot order (stable lists)
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int run = lane < 27 ? s_base[w][lane] : 0;  // lane t: next offset of direction t
+    for (int g0 = lo; g0 < hi; g0 += 64) {
+        const int i = g0 + lane;
+        const int dd = i < hi ? (int)s_dir[i] : -1;
+        int off = 0;
+        unsigned long long pending = __ballot(dd >= 0);
+        while (pending) {  // one round per direction present in the group
+            const int t = __shfl(dd, __ffsll((long long)pending) - 1);
+            const unsigned long long m = __ballot(dd == t);
+            const int rt = __shfl(run, t);
+            if (dd == t) off = rt + __popcll(m & below);
+            if (lane == t) run += __popcll(m);
+            pending &= ~m;
         }
-        __syncthreads();
-        if (i < n) {
-            int off = s_off[dd] + s_run[dd] + myrank;
-            for (int k = 0; k < w; k++) off += s_w[k][dd];
-            d.dep_idx[T * ph.cap + off] = i;
-        }
-        __syncthreads();
-        if (threadIdx.x < 27) {
-            int acc = 0;
-            for (int k = 0; k < DEPART_THREADS / 64; k++) acc += s_w[k][threadIdx.x];
-            s_run[threadIdx.x] += acc;
-        }
-        __syncthreads();
+        if (dd >= 0) d.dep_idx[T * ph.cap + off] = i;
     }
 }
 
