@@ -271,6 +271,8 @@ def main():
     ap.add_argument("--ensemble", type=int, default=256, help="replicas of the batched run (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=50)
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True)
+    ap.add_argument("--tile", type=int, nargs=3, metavar=("BX", "BY", "BZ"),
+                    help="tile shape (skips the autotuner)")
     ap.add_argument("--surrogate", action=argparse.BooleanOptionalAction, default=True)
     ap.add_argument("--surrogate-collect", type=int, default=40)
     ap.add_argument("--surrogate-epochs", type=int, default=8)
@@ -301,7 +303,9 @@ def main():
     rng = np.random.default_rng(args.seed)
     v0 = rng.uniform(0.0, 0.05, args.ncol)
     tune = None
-    if args.autotune:
+    if args.tile:
+        sprm["tile"] = tuple(args.tile)
+    elif args.autotune:
         from brownian_amd.autotune import autotune
         log("autotuning the tile shape (a few steps per candidate)")
         tune = autotune(sprm, steps=3, warmup=1, v0=v0, log=log)
