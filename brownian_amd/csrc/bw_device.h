@@ -509,23 +509,6 @@ __device__ inline void build_axis_masks(int n, const int *cx, const int *cy, con
     }
     __syncthreads();
 }
-// build_axis_masks by one wave
-__device__ inline void build_axis_masks_wave(int n, const int *cx, const int *cy, const int *cz, const Region &g,
-                                             int L, int reach, AxisMask &am) {
-    const int lane = threadIdx.x & 63;
-    for (int i = lane; i < 3 * AXIS_MAX; i += 64) (&am.m[0][0])[i] = 0u;
-    wave_sync();
-    const int nk = min(n, MASK_BITS);
-    for (int w = lane; w < nk * 3; w += 64) {
-        const int k = w / 3, a = w - 3 * k;
-        const int c = a == 0 ? cx[k] : a == 1 ? cy[k] : cz[k];
-        for (int dd = -reach; dd <= reach; dd++) {
-            int u = axis_index(g, a, wrapi(c + dd, L), L);
-            if (u >= 0) atomicOr(&am.m[a][u], 1u << k);
-        }
-    }
-    wave_sync();
-}
 __device__ inline void build_axis_masks(const TileColl &tc, const Region &g, int L, bool run_cells, int reach,
                                         AxisMask &am) {
     if (run_cells) build_axis_masks(tc.n, tc.qx, tc.qy, tc.qz, g, L, reach, am);

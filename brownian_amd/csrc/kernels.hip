@@ -735,7 +735,9 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
 //           [streaming, cpp/update.cpp:56-60, with exact int-cell binning]
 //           [collision detection, cpp/fluid_colloid_collision.cpp:32-35]
 // Candidate colloids of a particle come from the tile's LDS stencil masks.
-template <class Real, bool KICK, bool STREAM, bool DETECT, class TC>
+// DIRECT: no masks (null pointers); every listed colloid is a candidate and
+// the cheap cell tests come first -- for a few dozen particles per tile.
+template <class Real, bool KICK, bool STREAM, bool DETECT, class TC, bool DIRECT = false>
 struct StreamOp {
     const Phys &ph;
     const Dev &d;
@@ -746,13 +748,16 @@ struct StreamOp {
     int L, r, lox, loy, loz, lnx, lny, lnz;
     bool drift, ch_over;
 
+    __device__ __forceinline__ unsigned all_listed() const {
+        return tc.n >= MASK_BITS ? 0xffffffffu : ((1u << tc.n) - 1u);
+    }
     __device__ __forceinline__ void operator()(int slot, int c, Real ox, Real oy, Real oz, Real vx, Real vy, Real vz) {
         int ix, iy, iz;
         unpack_cell(c, ix, iy, iz);
         const int phantom = c & PHANTOM;
         const double Ld = (double)L;
         if constexpr (KICK) {
-            unsigned m = cell_mask(*mkick, rg, ix, iy, iz, L);
+            unsigned m = DIRECT ? all_listed() : cell_mask(*mkick, rg, ix, iy, iz, L);
             if (m || tc.n > MASK_BITS) {
                 double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
                 double wx = vx, wy = vy, wz = vz;
@@ -785,7 +790,7 @@ struct StreamOp {
                 drift = true;
         }
         if (DETECT) {
-            unsigned m = cell_mask(*mdet, rg, ix, iy, iz, L);
+            unsigned m = DIRECT ? all_listed() : cell_mask(*mdet, rg, ix, iy, iz, L);
             if (m == 0 && tc.n <= MASK_BITS) {
                 if (phantom) f.cell[slot] = pack_cell(ix, iy, iz);
                 return;
@@ -795,6 +800,10 @@ struct StreamOp {
             bool inside_any = false;  // by distance alone (the stencil skips a colloid's own cell)
             for_candidates(tc, m, [&](int k) {
                 if (k >= tc.n) return;
+                if (DIRECT) {  // inside a colloid: within 3 cells of its cell on every axis
+                    const int ax = pdiff(ix, tc.cx[k], L), ay = pdiff(iy, tc.cy[k], L), az = pdiff(iz, tc.cz[k], L);
+                    if (ax < -3 || ax > 3 || ay < -3 || ay > 3 || az < -3 || az > 3) return;
+                }
                 double rx = img1(tc.px[k] - x, Ld), ry = img1(tc.py[k] - y, Ld), rz = img1(tc.pz[k] - z, Ld);
                 if ((rx * rx + ry * ry) + rz * rz <= ph.lim_coll) {
                     inside_any = true;
@@ -874,7 +883,6 @@ __global__ __launch_bounds__(256, KICK ? 1 : 8) void k_stream_near(Phys ph, Dev 
     static_assert(TCL_CAP == 64, "one lane per listed colloid");
     using TC = typename std::conditional<KICK, TileColl, TileCore>::type;
     __shared__ TC s_tc[4];
-    __shared__ AxisMask s_am[4][KICK ? 2 : 1];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     TC &tc = s_tc[w];
     // the argument fields of the loop, fetched in one batch of scalar loads
@@ -921,12 +929,10 @@ __global__ __launch_bounds__(256, KICK ? 1 : 8) void k_stream_near(Phys ph, Dev 
         const int lt = T - R.first_tile;
         const int tx = lt % R.nt[0], ty = (lt / R.nt[0]) % R.nt[1], tz = lt / (R.nt[0] * R.nt[1]);
         const Region rg = tile_region(R, tx, ty, tz);
-        if constexpr (KICK) build_axis_masks_wave(cnt, tc.qx, tc.qy, tc.qz, rg, L, 3, s_am[w][0]);
-        build_axis_masks_wave(cnt, tc.cx, tc.cy, tc.cz, rg, L, 3, s_am[w][KICK ? 1 : 0]);
-        StreamOp<Real, KICK, true, true, TC> op{ph, d, f, tc, rg, &s_am[w][KICK ? 1 : 0], &s_am[w][0], L, r,
-                                                tx * R.B[0] - TILE_REACH, ty * R.B[1] - TILE_REACH,
-                                                tz * R.B[2] - TILE_REACH, R.B[0] + 2 * TILE_REACH,
-                                                R.B[1] + 2 * TILE_REACH, R.B[2] + 2 * TILE_REACH, false, false};
+        StreamOp<Real, KICK, true, true, TC, true> op{ph, d, f, tc, rg, nullptr, nullptr, L, r,
+                                                      tx * R.B[0] - TILE_REACH, ty * R.B[1] - TILE_REACH,
+                                                      tz * R.B[2] - TILE_REACH, R.B[0] + 2 * TILE_REACH,
+                                                      R.B[1] + 2 * TILE_REACH, R.B[2] + 2 * TILE_REACH, false, false};
         if (i < n) op(slot, c0, ox, oy, oz, vx, vy, vz);
         drift = drift || op.drift;
         ch_over = ch_over || op.ch_over;
