@@ -342,6 +342,18 @@ class System:
         self._chk(self.lib.bw_near_particles(self.h, replica, C.byref(n)))
         return n.value
 
+    def debug_geometry(self, a, L, ia, Li, b):
+        """Device img/mod of a (float64), ia mod Li and ia / b (int32)."""
+        a = np.ascontiguousarray(a, np.float64)
+        ia = np.ascontiguousarray(ia, np.int32)
+        n = len(a)
+        assert len(ia) == n
+        img, mod = np.zeros(n), np.zeros(n)
+        wrap, div = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        self._chk(self.lib.bw_debug_geometry(self.h, n, _ptr(a), C.c_double(L), _ptr(ia, _ip), Li, b, _ptr(img),
+                                             _ptr(mod), _ptr(wrap, _ip), _ptr(div, _ip)))
+        return img, mod, wrap, div
+
     def debug_copy_ms(self, nbytes, reps=10):
         ms = C.c_double(0)
         self._chk(self.lib.bw_debug_copy(self.h, C.c_longlong(nbytes), reps, C.byref(ms)))

@@ -39,6 +39,8 @@ void launch_colloid(const Phys &, const Dev &, unsigned, hipStream_t);
 void launch_colloid_update(const Phys &, const Dev &, unsigned, const int2 *, int, hipStream_t);
 void launch_md_candidates(const Phys &, const Dev &, const int2 *, int, hipStream_t);
 void launch_tile_colloids(const Phys &, const Dev &, hipStream_t);
+void launch_debug_geometry(int n, const double *a, double L, const int *ia, int Li, int b, double *img_out,
+                           double *mod_out, int *wrap_out, int *div_out, hipStream_t st);
 void launch_copy(const void *, void *, long long, hipStream_t);
 void launch_features(const Phys &, const Dev &, const int *, hipStream_t);
 template <class Real> void launch_mark_phantoms(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
@@ -1479,6 +1481,27 @@ BW_API int bw_near_particles(bw_ctx *c, int r, long long *n_near) {
     long long s = 0;
     for (int v : cnt) s += v;
     *n_near = s;
+    return BW_OK;
+}
+
+BW_API int bw_debug_geometry(bw_ctx *c, int n, const double *a, double L, const int *ia, int Li, int b,
+                             double *img_out, double *mod_out, int *wrap_out, int *div_out) {
+    if (!c || n < 1 || !a || !ia || !img_out || !mod_out || !wrap_out || !div_out || Li < 1 || b < 1 || b > 63)
+        return BW_ERR_ARG;
+    const size_t nd = sizeof(double) * n, ni = sizeof(int) * n;
+    char *buf = nullptr;
+    HIPCHK(c, hipMalloc((void **)&buf, 3 * nd + 3 * ni));
+    double *da = (double *)buf, *dimg = da + n, *dmod = dimg + n;
+    int *dia = (int *)(dmod + n), *dwrap = dia + n, *ddiv = dwrap + n;
+    HIPCHK(c, hipMemcpyAsync(da, a, nd, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(dia, ia, ni, hipMemcpyHostToDevice, c->st));
+    launch_debug_geometry(n, da, L, dia, Li, b, dimg, dmod, dwrap, ddiv, c->st);
+    HIPCHK(c, hipMemcpyAsync(img_out, dimg, nd, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(mod_out, dmod, nd, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(wrap_out, dwrap, ni, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(div_out, ddiv, ni, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    hipFree(buf);
     return BW_OK;
 }
 

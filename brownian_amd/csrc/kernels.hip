@@ -1623,6 +1623,18 @@ __global__ __launch_bounds__(256) void k_mark_phantoms(Phys ph, Dev d, Fluid<Rea
 }
 
 // plain copy for HBM counter calibration
+// The periodic-geometry helpers on given inputs, for their test against the
+// formulas they replace (img1/mod1 against a - L*round(a/L), wrapi against a
+// mod n, div_small against a / b).
+__global__ void k_debug_geometry(int n, const double *a, double L, const int *ia, int Li, int b, double *img_out,
+                                 double *mod_out, int *wrap_out, int *div_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    img_out[i] = img1(a[i], L);
+    mod_out[i] = mod1(a[i], L);
+    wrap_out[i] = wrapi(ia[i], Li);
+    div_out[i] = (ia[i] >= 0 && ia[i] < 1024) ? div_small(ia[i], div_small_magic(b)) : -1;
+}
 __global__ void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
@@ -1762,6 +1774,11 @@ void launch_mark_phantoms(const Phys &ph, const Dev &d, const Fluid<Real> &f, hi
 }
 template void launch_mark_phantoms<float>(const Phys &, const Dev &, const Fluid<float> &, hipStream_t);
 template void launch_mark_phantoms<double>(const Phys &, const Dev &, const Fluid<double> &, hipStream_t);
+void launch_debug_geometry(int n, const double *a, double L, const int *ia, int Li, int b, double *img_out,
+                           double *mod_out, int *wrap_out, int *div_out, hipStream_t st) {
+    LAUNCH(k_debug_geometry, (n + 255) / 256, 256, 0, st, n, a, L, ia, Li, b, img_out, mod_out, wrap_out, div_out);
+}
+
 void launch_copy(const void *a, void *b, long long nbytes, hipStream_t st) {
     long long n = nbytes / 16;
     LAUNCH(k_copy, 2048, 256, 0, st, (const float4 *)a, (float4 *)b, n);

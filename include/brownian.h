@@ -209,6 +209,11 @@ BW_API int bw_host_initialize(const bw_params *p, double *pos_fl, double *vel_fl
 
 /* Calibration: copy n bytes device to device with a plain vector kernel. */
 BW_API int bw_debug_copy(bw_ctx *ctx, long long nbytes, int reps, double *ms);
+/* Test hook: the device's periodic-geometry helpers on n given values --
+ * img(a, L) and mod(a, L) of cpp/parameters.hpp:74-76, ia mod Li, and ia / b
+ * for 0 <= ia < 1024, 1 <= b <= 63 (-1 elsewhere). */
+BW_API int bw_debug_geometry(bw_ctx *ctx, int n, const double *a, double L, const int *ia, int Li, int b,
+                             double *img_out, double *mod_out, int *wrap_out, int *div_out);
 
 /* ----------------------------------------------------------- observables */
 /* No counterpart in the reference (it computes energies and prints nothing,
