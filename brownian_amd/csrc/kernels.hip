@@ -662,53 +662,57 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             base += tot;
         }
         n_near = base;
-        if (COUNT_SHELL && w == 0) {  // the near prefix in chunks of 64 for the per-substep pass
-            const int nch = (n_near + 63) >> 6;
-            int c0 = 0;
-            if (lane == 0 && nch > 0) c0 = atomicAdd(d.chunk_cnt, nch);
-            c0 = __shfl(c0, 0);
-            for (int j = lane; j < nch; j += 64) d.chunk_tile[c0 + j] = (T << 6) | j;
+    }
+    // The near prefix is published in chunks of 64 for the per-substep pass;
+    // the chunk range is requested here and written after the far particles'
+    // streaming, which hides the atomic's round trip.
+    const bool publish = COUNT_SHELL && BLOCK && (threadIdx.x >> 6) == 0;
+    int chunk0 = 0;
+    if (publish && (threadIdx.x & 63) == 0 && n_near > 0) chunk0 = atomicAdd(d.chunk_cnt, (n_near + 63) >> 6);
+    bool far_drift = false;
+    if (BLOCK) {
+        // far particles: every substep of the step as the streaming kernel
+        // does it (normalise() is the identity while an offset is in [0,1))
+#pragma unroll
+        for (int k = 0; k < SRD_PT; k++) {
+            if (threadIdx.x + k * TH < n && !(nearbits & (1u << k))) {
+                int ix, iy, iz;
+                unpack_cell(rcell[k], ix, iy, iz);
+                Real ox = rox[k], oy = roy[k], oz = roz[k];
+                const Real dt = (Real)ph.dt;
+                const Real dx = rvx[k] * dt, dy = rvy[k] * dt, dz = rvz[k] * dt;
+                for (int l = 0; l < ph.n_sub; l++) {
+                    ox = ox + dx;
+                    oy = oy + dy;
+                    oz = oz + dz;
+                    if (!(in_unit(ox) && in_unit(oy) && in_unit(oz))) {
+                        normalise(ix, ox, L);
+                        normalise(iy, oy, L);
+                        normalise(iz, oz, L);
+                    }
+                }
+                rcell[k] = pack_cell(ix, iy, iz) | (rcell[k] & PHANTOM);
+                rox[k] = ox; roy[k] = oy; roz[k] = oz;
+                if (!(near_interval(ix, ox0 - TILE_REACH, B0 + 2 * TILE_REACH, L) &&
+                      near_interval(iy, oy0 - TILE_REACH, B1 + 2 * TILE_REACH, L) &&
+                      near_interval(iz, oz0 - TILE_REACH, B2 + 2 * TILE_REACH, L)))
+                    far_drift = true;
+            }
         }
     }
-    bool far_drift = false;
+    if (publish) {
+        chunk0 = __shfl(chunk0, 0);
+        for (int j = threadIdx.x; j < (n_near + 63) >> 6; j += 64) d.chunk_tile[chunk0 + j] = (T << 6) | j;
+    }
 #pragma unroll
     for (int k = 0; k < SRD_PT; k++) {
         int g = threadIdx.x + k * TH;
         if (g < n) {
             int slot_in_tile = g;
-            int cell = rcell[k];
-            Real ox = rox[k], oy = roy[k], oz = roz[k];
-            if (BLOCK) {
-                if (nearbits & (1u << k)) {
-                    slot_in_tile = nrk[k];
-                } else {
-                    slot_in_tile = n_near + g - nrk[k];
-                    // every substep of the step as the streaming kernel does it
-                    // (normalise() is the identity while an offset is in [0,1))
-                    int ix, iy, iz;
-                    unpack_cell(cell, ix, iy, iz);
-                    const Real dt = (Real)ph.dt;
-                    const Real dx = rvx[k] * dt, dy = rvy[k] * dt, dz = rvz[k] * dt;
-                    for (int l = 0; l < ph.n_sub; l++) {
-                        ox = ox + dx;
-                        oy = oy + dy;
-                        oz = oz + dz;
-                        if (!(in_unit(ox) && in_unit(oy) && in_unit(oz))) {
-                            normalise(ix, ox, L);
-                            normalise(iy, oy, L);
-                            normalise(iz, oz, L);
-                        }
-                    }
-                    cell = pack_cell(ix, iy, iz) | (rcell[k] & PHANTOM);
-                    if (!(near_interval(ix, ox0 - TILE_REACH, B0 + 2 * TILE_REACH, L) &&
-                          near_interval(iy, oy0 - TILE_REACH, B1 + 2 * TILE_REACH, L) &&
-                          near_interval(iz, oz0 - TILE_REACH, B2 + 2 * TILE_REACH, L)))
-                        far_drift = true;
-                }
-            }
+            if (BLOCK) slot_in_tile = (nearbits & (1u << k)) ? nrk[k] : n_near + g - nrk[k];
             int o = T * cap + slot_in_tile;
-            dst.cell[o] = cell;
-            dst.ox[o] = ox; dst.oy[o] = oy; dst.oz[o] = oz;
+            dst.cell[o] = rcell[k];
+            dst.ox[o] = rox[k]; dst.oy[o] = roy[k]; dst.oz[o] = roz[k];
             dst.vx[o] = rvx[k]; dst.vy[o] = rvy[k]; dst.vz[o] = rvz[k];
             dst.pid[o] = rpid[k];
             if (MAP) {
