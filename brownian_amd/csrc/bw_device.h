@@ -701,9 +701,10 @@ __device__ inline void colloid_update(const Phys &ph, const Dev &d, int L, int r
         if (flags & PH_OLDF) d.coldf[3 * g + k] = of[k];
     }
 }
-// pair test of cpp/neighbour_list.cpp:33-34 (symmetric: img(-a) = -img(a))
-__device__ __forceinline__ bool md_pair(const Phys &ph, const Dev &d, double L, int g, int j) {
-    const double *pi = d.cpos + 3 * g, *pj = d.cpos + 3 * j;
+// pair test of cpp/neighbour_list.cpp:33-34 (symmetric: img(-a) = -img(a)) on
+// the positions pi of g and pj of j
+__device__ __forceinline__ bool md_pair_at(const Phys &ph, double L, int g, int j, const double *pi,
+                                           const double *pj) {
     double tx, ty, tz;
     if (j > g) {
         tx = img1(pi[0] - pj[0], L); ty = img1(pi[1] - pj[1], L); tz = img1(pi[2] - pj[2], L);
@@ -711,6 +712,9 @@ __device__ __forceinline__ bool md_pair(const Phys &ph, const Dev &d, double L, 
         tx = img1(pj[0] - pi[0], L); ty = img1(pj[1] - pi[1], L); tz = img1(pj[2] - pi[2], L);
     }
     return (tx * tx + ty * ty) + tz * tz < ph.lim_md;
+}
+__device__ __forceinline__ bool md_pair(const Phys &ph, const Dev &d, double L, int g, int j) {
+    return md_pair_at(ph, L, g, j, d.cpos + 3 * g, d.cpos + 3 * j);
 }
 // the pair list of colloid g (all partners, ascending), by one thread
 __device__ inline void colloid_mdlist_serial(const Phys &ph, const Dev &d, const Rep &R, int r, int g) {

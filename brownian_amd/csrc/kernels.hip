@@ -1005,24 +1005,40 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     __shared__ int s_ne[COLLIDE_THREADS / 64], s_nnb[COLLIDE_THREADS / 64];
     const bool fp32 = sizeof(Real) == 4;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int2 wt = wgt[blockIdx.x];
+    const int2 wt = ph.n_rep == 1 ? make_int2(0, blockIdx.x * (COLLIDE_THREADS / 64)) : wgt[blockIdx.x];
     const int r = wt.x;
+    const int g = wt.y + w;
+    // Loads that do not need the replica record go first, at an in-range
+    // index (a workgroup's last colloids may lie past its replica).  Entries
+    // of the candidate list and of the event bucket past their counts are
+    // stale but valid indices, so the candidates' positions and the events'
+    // particles are requested with the counts and discarded when unused.
+    const int gs = min(g, ph.n_colloid_total - 1);
+    const int mdc_n = d.mdc_cnt[gs], ne_s = d.ev_cnt[gs];
+    const int js = d.mdc_list[gs * MDC_CAP + lane], slot_s = d.ev_slot[gs * EV_CAP + lane];
+    double pg[3], pj[3], vg[3], wg[3];
+    int cg[3], cj[3];
+    for (int k = 0; k < 3; k++) {
+        pg[k] = d.cpos[3 * gs + k];
+        pj[k] = d.cpos[3 * js + k];
+        vg[k] = d.cvel[3 * gs + k];
+        wg[k] = d.comega[3 * gs + k];
+        cg[k] = d.ccell[3 * gs + k];
+        cj[k] = d.ccell[3 * js + k];
+    }
+    const int pid_s = f.pid[slot_s], cell_s = f.cell[slot_s];
+    const Real pox_s = f.ox[slot_s], poy_s = f.oy[slot_s], poz_s = f.oz[slot_s];
+    const Real pvx_s = f.vx[slot_s], pvy_s = f.vy[slot_s], pvz_s = f.vz[slot_s];
     const Rep R = d.rep[r];
     const int L = R.L;
     const double Ld = (double)L;
-    const int g = wt.y + w;
     const bool valid = g < R.first_colloid + R.nc;
     const long long step = *d.step;
-    // The event bucket and the pair candidates are read together, then the
-    // events' particles and the candidates' positions, so that the two chains
-    // of dependent loads run side by side.
     const bool full = d.md_stale[r] != 0;
-    const int ncand = valid ? (full ? R.nc : d.mdc_cnt[g]) : 0;
-    int j0 = -1;  // this lane's first candidate
-    if (lane < ncand) j0 = full ? R.first_colloid + lane : d.mdc_list[g * MDC_CAP + lane];
+    const int ncand = valid ? (full ? R.nc : mdc_n) : 0;
     int ne = 0;
     if (valid) {
-        ne = d.ev_cnt[g];
+        ne = ne_s;
         if (lane == 0) {
             d.ev_cnt[g] = 0;
             if (ne > EV_CAP) {
@@ -1039,11 +1055,11 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     long long key = 0x7fffffffffffffffll;
     Real pox = 0, poy = 0, poz = 0, pvx = 0, pvy = 0, pvz = 0;
     if (lane < ne) {
-        slot = d.ev_slot[g * EV_CAP + lane];
-        pid = f.pid[slot];
-        unpack_cell(f.cell[slot], ix, iy, iz);
-        pox = f.ox[slot]; poy = f.oy[slot]; poz = f.oz[slot];
-        pvx = f.vx[slot]; pvy = f.vy[slot]; pvz = f.vz[slot];
+        slot = slot_s;
+        pid = pid_s;
+        unpack_cell(cell_s, ix, iy, iz);
+        pox = pox_s; poy = poy_s; poz = poz_s;
+        pvx = pvx_s; pvy = pvy_s; pvz = pvz_s;
     }
     if (valid) {  // pair list of g (cpp/neighbour_list.cpp:27-39), partners j > g into LDS
         const unsigned long long below = (1ull << lane) - 1ull;
@@ -1052,9 +1068,10 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             const int m = base + lane;
             int j = -1;
             bool p = false;
+            const bool pre = base == 0 && !full;  // this lane's candidate is js, loaded above
             if (m < ncand) {
-                j = base == 0 ? j0 : (full ? R.first_colloid + m : d.mdc_list[g * MDC_CAP + m]);
-                p = j != g && md_pair(ph, d, Ld, g, j);
+                j = pre ? js : (full ? R.first_colloid + m : d.mdc_list[g * MDC_CAP + m]);
+                p = j != g && (pre ? md_pair_at(ph, Ld, g, j, pg, pj) : md_pair(ph, d, Ld, g, j));
             }
             const unsigned long long b = __ballot(p);
             const int rank = __popcll(b & below);
@@ -1065,8 +1082,13 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             const int rq = __popcll(bq & below);
             if (q && nnb + rq < MD_CAP) {
                 PairColl &e = s_nb[w][nnb + rq];
-                e.px = d.cpos[3 * j]; e.py = d.cpos[3 * j + 1]; e.pz = d.cpos[3 * j + 2];
-                e.cx = d.ccell[3 * j]; e.cy = d.ccell[3 * j + 1]; e.cz = d.ccell[3 * j + 2];
+                if (pre) {
+                    e.px = pj[0]; e.py = pj[1]; e.pz = pj[2];
+                    e.cx = cj[0]; e.cy = cj[1]; e.cz = cj[2];
+                } else {
+                    e.px = d.cpos[3 * j]; e.py = d.cpos[3 * j + 1]; e.pz = d.cpos[3 * j + 2];
+                    e.cx = d.ccell[3 * j]; e.cy = d.ccell[3 * j + 1]; e.cz = d.ccell[3 * j + 2];
+                }
             }
             nnb += __popcll(bq);
         }
@@ -1078,8 +1100,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     }
     __syncthreads();  // pair lists visible to every lane
     if (lane < ne) {
-        int dx = pdiff(ix, d.ccell[3 * g], L), dy = pdiff(iy, d.ccell[3 * g + 1], L),
-            dz = pdiff(iz, d.ccell[3 * g + 2], L);
+        int dx = pdiff(ix, cg[0], L), dy = pdiff(iy, cg[1], L), dz = pdiff(iz, cg[2], L);
         key = ((long long)stencil_index(dx, dy, dz) << 32) | (unsigned)pid;
     }
     int rank = 0;
@@ -1095,8 +1116,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
         const double dump[3] = {vx, vy, vz};
         if (draw_reflection(d, ph, R, step, substep, g, pid, dr)) {
             double rs[3];
-            reflect_event(ph, L, ix, iy, iz, bx, by, bz, vx, vy, vz, d.cpos + 3 * g, d.cvel + 3 * g,
-                          d.comega + 3 * g, dr, rs, fp32);
+            reflect_event(ph, L, ix, iy, iz, bx, by, bz, vx, vy, vz, pg, vg, wg, dr, rs, fp32);
             transfer(dump, vx, vy, vz, rs, cont);
             f.ox[slot] = (Real)bx; f.oy[slot] = (Real)by; f.oz[slot] = (Real)bz;
             f.vx[slot] = (Real)vx; f.vy[slot] = (Real)vy; f.vz[slot] = (Real)vz;
@@ -1280,21 +1300,41 @@ __global__ __launch_bounds__(256) void k_colloid_update(Phys ph, Dev d, unsigned
     __shared__ double s_f[4][MD_CAP][3];
     __shared__ unsigned char s_sub[4][MD_CAP], s_has[4][MD_CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int2 wt = wgt[blockIdx.x];
+    const int2 wt = ph.n_rep == 1 ? make_int2(0, blockIdx.x * 4) : wgt[blockIdx.x];
     const int r = wt.x;
-    const Rep R = d.rep[r];
     const int g = wt.y + w;
+    // Loads that do not need the replica record go first, at an in-range
+    // index (a workgroup's last colloids may lie past its replica).  Pair-list
+    // entries past the count are stale but valid colloid indices, so the first
+    // 64 partners' positions are requested before the count arrives.
+    const int gs = min(g, ph.n_colloid_total - 1);
+    ColIn in;  // lane 0: the colloid's own state, in flight with the force terms
+    if (lane == 0) colloid_load(ph, d, gs, flags, in);
+    int nmd_s = 0, js = 0;
+    double pg[3] = {0, 0, 0}, pj[3] = {0, 0, 0};
+    if (flags & PH_FORCE) {
+        nmd_s = d.md_cnt[gs];
+        js = d.md_list[gs * MD_CAP + lane];
+        for (int k = 0; k < 3; k++) {
+            pg[k] = d.cpos[3 * gs + k];
+            pj[k] = d.cpos[3 * js + k];
+        }
+    }
+    const Rep R = d.rep[r];
     const bool valid = g < R.first_colloid + R.nc;
     const double L = (double)R.L;
-    ColIn in;  // lane 0: the colloid's own state, in flight with the force terms
-    if (valid && lane == 0) colloid_load(ph, d, g, flags, in);
     int nmd = 0;
     if (valid && (flags & PH_FORCE)) {
-        nmd = d.md_cnt[g];
+        nmd = nmd_s;
         for (int m = lane; m < nmd; m += 64) {
             double c[3] = {0, 0, 0};
-            bool sub;
-            bool has = force_term(ph, d, L, g, d.md_list[g * MD_CAP + m], c, sub);
+            bool sub, has;
+            if (m < 64) {  // force_term on the positions loaded above
+                sub = js < g;
+                has = sub ? pair_force(ph, L, pj, pg, c[0], c[1], c[2]) : pair_force(ph, L, pg, pj, c[0], c[1], c[2]);
+            } else {
+                has = force_term(ph, d, L, g, d.md_list[g * MD_CAP + m], c, sub);
+            }
             s_has[w][m] = has;
             s_sub[w][m] = sub;
             for (int k = 0; k < 3; k++) s_f[w][m][k] = c[k];
