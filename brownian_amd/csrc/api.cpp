@@ -632,7 +632,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         (rc = dalloc(c, &d.tcl_cnt, tiles)) || (rc = dalloc(c, &d.tcl_list, (size_t)tiles * TCL_CAP)) ||
         (rc = dalloc(c, &d.tsl_cnt, tiles)) || (rc = dalloc(c, &d.tsl_list, (size_t)tiles * TCL_CAP)) ||
         (rc = dalloc(c, &d.ccnt, NC)) || (rc = dalloc(c, &d.md_cnt, NC)) ||
-        (rc = dalloc(c, &d.md_list, NC * MD_CAP)) || (rc = dalloc(c, &d.mdc_cnt, NC)) ||
+        (rc = dalloc(c, &d.md_list, NC * MD_CAP)) || (rc = dalloc(c, &d.md_f, NC * MD_CAP * 3)) || (rc = dalloc(c, &d.mdc_cnt, NC)) ||
         (rc = dalloc(c, &d.mdc_list, NC * MDC_CAP)) || (rc = dalloc(c, &d.cpot, n_rep)) ||
         (rc = dalloc(c, &d.ev_cnt, NC)) || (rc = dalloc(c, &d.ev_slot, NC * EV_CAP)) ||
         (rc = dalloc(c, &d.chain_cnt, n_rep)) || (rc = dalloc(c, &d.chain_slot, (size_t)n_rep * CHAIN_CAP)) ||

@@ -118,6 +118,7 @@ struct Dev {
     int *tcl_cnt, *tcl_list;  // per tile: colloids that can touch it this step
     int *tsl_cnt, *tsl_list;  // per tile: surrogate-driven colloids near it this step
     int *md_cnt, *md_list, *mdc_cnt, *mdc_list;
+    double *md_f;        // [NC * MD_CAP * 3] per pair-list entry: its signed WCA term of the force
     double *cpot;        // [n_rep] potential
     int *ev_cnt, *ev_slot, *chain_cnt, *chain_slot;
     int *cont_cnt, *cont_slot, *cont_from;  // continuations of the colloid loop

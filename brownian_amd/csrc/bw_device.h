@@ -572,6 +572,17 @@ __device__ __forceinline__ bool force_term(const Phys &ph, const Dev &d, double 
     return sub ? pair_force(ph, L, d.cpos + 3 * j, d.cpos + 3 * g, c[0], c[1], c[2])
                : pair_force(ph, L, d.cpos + 3 * g, d.cpos + 3 * j, c[0], c[1], c[2]);
 }
+// The same term as one addend t: f_g + t is f_g - ff(j,g) or f_g + ff(g,j)
+// bit for bit (x - y is x + (-y)), and t = +0 beyond r_cutoff leaves f_g
+// unchanged (a sum of such terms started at +0 never holds -0).  pi, pj: the
+// positions of g and j.
+__device__ __forceinline__ void force_addend(const Phys &ph, double L, int g, int j, const double *pi,
+                                             const double *pj, double *t) {
+    double c[3] = {0, 0, 0};
+    const bool sub = j < g;
+    const bool has = sub ? pair_force(ph, L, pj, pi, c[0], c[1], c[2]) : pair_force(ph, L, pi, pj, c[0], c[1], c[2]);
+    for (int k = 0; k < 3; k++) t[k] = has ? (sub ? -c[k] : c[k]) : 0.0;
+}
 // cpp/update.cpp:14-32 for one colloid, serially in list order
 __device__ inline void colloid_force_serial(const Phys &ph, const Dev &d, double L, int g, double *f) {
     double f0 = 0, f1 = 0, f2 = 0;
@@ -586,7 +597,7 @@ __device__ inline void colloid_force_serial(const Phys &ph, const Dev &d, double
     f[0] = f0; f[1] = f1; f[2] = f2;
 }
 struct ColIn {  // the inputs of colloid_update, loaded ahead of the force terms
-    double v[3], w[3], ra[3], f[3], of[3], p0[3], dv[3], dw[3], sv[6], disp[3], pmd[3], v0;
+    double v[3], w[3], ra[3], f[3], of[3], p0[3], dv[3], dw[3], sv[6], imp[6], disp[3], pmd[3], v0;
     int cell[3], cmd[3];
     bool sur;
 };
@@ -606,12 +617,17 @@ __device__ inline void colloid_load(const Phys &ph, const Dev &d, int g, unsigne
         in.cmd[k] = (flags & PH_POS) ? d.ccell_md[3 * g + k] : 0;
     }
     in.sur = (flags & PH_APPLY) && ph.surrogate && d.cmode[g];
-    for (int k = 0; k < 6; k++) in.sv[k] = ((flags & PH_APPLY) && ph.surrogate) ? d.csur[6 * g + k] : 0.0;
+    for (int k = 0; k < 6; k++) {
+        in.sv[k] = ((flags & PH_APPLY) && ph.surrogate) ? d.csur[6 * g + k] : 0.0;
+        in.imp[k] = (flags & PH_APPLY) ? d.cimp[6 * g + k] : 0.0;
+    }
     in.v0 = (flags & PH_KICK) ? d.cv0[g] : 0.0;
 }
 // The per-colloid phases of one substep, in PH_* order, by one thread, on the
 // inputs loaded by colloid_load: the phases run in registers and the results
-// are stored at the end.  fn is the new WCA force when PH_FORCE is set.
+// are stored at the end.  fn is the new WCA force when PH_FORCE is set; trig,
+// when not null, holds sin and cos of the activity rotation angles (x, y, z)
+// as this function would form them.
 //  PH_APPLY     cpp/fluid_colloid_collision.cpp:46-47 (the sums dv, dw were
 //               formed in the reference's event order); a surrogate-driven
 //               colloid receives its sampled impulse instead, and the event
@@ -624,7 +640,7 @@ __device__ inline void colloid_load(const Phys &ph, const Dev &d, int g, unsigne
 //               the replica's pair candidates / tile lists stale if the
 //               colloid left their margins
 __device__ inline void colloid_update(const Phys &ph, const Dev &d, int L, int r, int g, unsigned flags,
-                                      const double *fn, const ColIn &in) {
+                                      const double *fn, const ColIn &in, const double *trig = nullptr) {
     double v[3], w[3], ra[3], f[3], of[3];
     for (int k = 0; k < 3; k++) {
         v[k] = in.v[k];
@@ -642,8 +658,13 @@ __device__ inline void colloid_update(const Phys &ph, const Dev &d, int L, int r
         }
     }
     if (flags & PH_ACTIVITY) {
-        double bx = w[0] * ph.dt, by = w[1] * ph.dt, bz = w[2] * ph.dt;
-        double sbx = sin(bx), sby = sin(by), sbz = sin(bz), cbx = cos(bx), cby = cos(by), cbz = cos(bz);
+        double sbx, sby, sbz, cbx, cby, cbz;
+        if (trig) {
+            sbx = trig[0]; sby = trig[1]; sbz = trig[2]; cbx = trig[3]; cby = trig[4]; cbz = trig[5];
+        } else {
+            double bx = w[0] * ph.dt, by = w[1] * ph.dt, bz = w[2] * ph.dt;
+            sbx = sin(bx); sby = sin(by); sbz = sin(bz); cbx = cos(bx); cby = cos(by); cbz = cos(bz);
+        }
         double m1x = cby * cbz, m1y = -cby * sbz, m1z = sby;
         double m2x = sbx * sby * cbz + cbx * sbz, m2y = -sbx * sby * sbz + cbx * cbz, m2z = -sbx * cby;
         double m3x = -cbx * sby * cbz + sbx * sbz, m3y = cbx * sby * sbz + sbx * cbz, m3z = cbx * cby;
@@ -689,8 +710,8 @@ __device__ inline void colloid_update(const Phys &ph, const Dev &d, int L, int r
     }
     for (int k = 0; k < 3; k++) {
         if (flags & PH_APPLY) {
-            d.cimp[6 * g + k] += in.dv[k];
-            d.cimp[6 * g + 3 + k] += in.dw[k];
+            d.cimp[6 * g + k] = in.imp[k] + in.dv[k];
+            d.cimp[6 * g + 3 + k] = in.imp[3 + k] + in.dw[k];
             d.cdv[3 * g + k] = 0;
             d.cdw[3 * g + k] = 0;
         }

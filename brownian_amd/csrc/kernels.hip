@@ -1075,7 +1075,15 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             }
             const unsigned long long b = __ballot(p);
             const int rank = __popcll(b & below);
-            if (p && nmd + rank < MD_CAP) d.md_list[g * MD_CAP + nmd + rank] = j;
+            if (p && nmd + rank < MD_CAP) {
+                const int e = g * MD_CAP + nmd + rank;
+                d.md_list[e] = j;
+                // the pair's force addend, summed in list order by k_colloid_update
+                double pjv[3], t[3];
+                for (int k = 0; k < 3; k++) pjv[k] = pre ? pj[k] : d.cpos[3 * j + k];
+                force_addend(ph, Ld, g, j, pg, pjv, t);
+                for (int k = 0; k < 3; k++) d.md_f[3 * e + k] = t[k];
+            }
             nmd += __popcll(b);
             const bool q = p && j > g && !(ph.surrogate && d.cmode[j]);
             const unsigned long long bq = __ballot(q);
@@ -1293,32 +1301,41 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_serial(Phys ph, Dev d, Flui
 }
 
 // ----------------------------------------------------- colloid update (fused)
-// One wave per colloid: collision impulse, orientation, WCA force (pair terms
-// in parallel, summed in the reference's order by lane 0), velocity, [run()
-// kick], old force, position into the other buffer.
+// One wave per colloid: collision impulse, orientation, WCA force (the pair
+// addends k_collide stored with this substep's pair list, summed in list
+// order by lane 0), velocity, [run() kick], old force, position into the
+// other buffer.  Follows k_collide of the same substep.
 __global__ __launch_bounds__(256) void k_colloid_update(Phys ph, Dev d, unsigned flags, const int2 *wgt) {
     __shared__ double s_f[4][MD_CAP][3];
-    __shared__ unsigned char s_sub[4][MD_CAP], s_has[4][MD_CAP];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int2 wt = ph.n_rep == 1 ? make_int2(0, blockIdx.x * 4) : wgt[blockIdx.x];
     const int r = wt.x;
     const int g = wt.y + w;
     // Loads that do not need the replica record go first, at an in-range
-    // index (a workgroup's last colloids may lie past its replica).  Pair-list
-    // entries past the count are stale but valid colloid indices, so the first
-    // 64 partners' positions are requested before the count arrives.
+    // index (a workgroup's last colloids may lie past its replica); the first
+    // 64 force addends are requested before their count arrives.
     const int gs = min(g, ph.n_colloid_total - 1);
     ColIn in;  // lane 0: the colloid's own state, in flight with the force terms
     if (lane == 0) colloid_load(ph, d, gs, flags, in);
-    int nmd_s = 0, js = 0;
-    double pg[3] = {0, 0, 0}, pj[3] = {0, 0, 0};
+    int nmd_s = 0;
+    double tl[3] = {0, 0, 0};
     if (flags & PH_FORCE) {
         nmd_s = d.md_cnt[gs];
-        js = d.md_list[gs * MD_CAP + lane];
-        for (int k = 0; k < 3; k++) {
-            pg[k] = d.cpos[3 * gs + k];
-            pj[k] = d.cpos[3 * js + k];
+        for (int k = 0; k < 3; k++) tl[k] = d.md_f[3 * (gs * MD_CAP + lane) + k];
+    }
+    // lanes 0-5: sin (lanes 0-2) or cos (lanes 3-5) of the activity rotation
+    // angle about axis lane % 3, from the angular velocity after the impulse,
+    // formed as colloid_update forms them; lane 0 gathers them below
+    double tv = 0.0;
+    if ((flags & PH_ACTIVITY) && lane < 6) {
+        const int k = lane % 3;
+        double wk = d.comega[3 * gs + k];
+        if (flags & PH_APPLY) {
+            const double iw = (ph.surrogate && d.cmode[gs]) ? d.csur[6 * gs + 3 + k] : d.cdw[3 * gs + k];
+            wk = wk + iw * ph.mass_fl / ph.I_colloid;
         }
+        const double b = wk * ph.dt;
+        tv = lane < 3 ? sin(b) : cos(b);
     }
     const Rep R = d.rep[r];
     const bool valid = g < R.first_colloid + R.nc;
@@ -1326,33 +1343,24 @@ __global__ __launch_bounds__(256) void k_colloid_update(Phys ph, Dev d, unsigned
     int nmd = 0;
     if (valid && (flags & PH_FORCE)) {
         nmd = nmd_s;
-        for (int m = lane; m < nmd; m += 64) {
-            double c[3] = {0, 0, 0};
-            bool sub, has;
-            if (m < 64) {  // force_term on the positions loaded above
-                sub = js < g;
-                has = sub ? pair_force(ph, L, pj, pg, c[0], c[1], c[2]) : pair_force(ph, L, pg, pj, c[0], c[1], c[2]);
-            } else {
-                has = force_term(ph, d, L, g, d.md_list[g * MD_CAP + m], c, sub);
-            }
-            s_has[w][m] = has;
-            s_sub[w][m] = sub;
-            for (int k = 0; k < 3; k++) s_f[w][m][k] = c[k];
-        }
+        for (int m = lane; m < nmd; m += 64)
+            for (int k = 0; k < 3; k++) s_f[w][m][k] = m < 64 ? tl[k] : d.md_f[3 * (g * MD_CAP + m) + k];
     }
+    double trig[6];
+    for (int k = 0; k < 6; k++) trig[k] = __shfl(tv, k);
     __syncthreads();
     if (valid && lane == 0) {
         double fn[3] = {0, 0, 0};
         if (flags & PH_FORCE) {
             double f0 = 0, f1 = 0, f2 = 0;
-            for (int m = 0; m < nmd; m++) {
-                if (!s_has[w][m]) continue;
-                if (s_sub[w][m]) { f0 = f0 - s_f[w][m][0]; f1 = f1 - s_f[w][m][1]; f2 = f2 - s_f[w][m][2]; }
-                else { f0 = f0 + s_f[w][m][0]; f1 = f1 + s_f[w][m][1]; f2 = f2 + s_f[w][m][2]; }
+            for (int m = 0; m < nmd; m++) {  // cpp/update.cpp:28 in list order
+                f0 = f0 + s_f[w][m][0];
+                f1 = f1 + s_f[w][m][1];
+                f2 = f2 + s_f[w][m][2];
             }
             fn[0] = f0; fn[1] = f1; fn[2] = f2;
         }
-        colloid_update(ph, d, R.L, r, g, flags, fn, in);
+        colloid_update(ph, d, R.L, r, g, flags, fn, in, (flags & PH_ACTIVITY) ? trig : nullptr);
     }
 }
 
