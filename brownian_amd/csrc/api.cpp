@@ -34,11 +34,11 @@ void launch_maps(const Phys &, const Dev &, const Fluid<Real> &, long long, int 
 template <class Real> void launch_audit(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
 template <class Real>
 void launch_gather(const Phys &, const Dev &, const Fluid<Real> &, double *, double *, int *, hipStream_t);
-void launch_step_begin(const Phys &, const Dev &, hipStream_t);
+void launch_step_begin(const Phys &, const Dev &, hipStream_t, bool clear_lists = false);
 void launch_colloid(const Phys &, const Dev &, unsigned, hipStream_t);
 void launch_colloid_update(const Phys &, const Dev &, unsigned, const int2 *, int, hipStream_t);
 void launch_md_candidates(const Phys &, const Dev &, const int2 *, int, hipStream_t);
-void launch_tile_colloids(const Phys &, const Dev &, hipStream_t);
+void launch_tile_colloids(const Phys &, const Dev &, hipStream_t, bool cleared = false);
 void launch_debug_geometry(int n, const double *a, double L, const int *ia, int Li, int b, double *img_out,
                            double *mod_out, int *wrap_out, int *div_out, hipStream_t st);
 void launch_copy(const void *, void *, long long, hipStream_t);
@@ -322,10 +322,10 @@ int enqueue_step(bw_ctx *c, int cur, int cb, bool final, Prof *prof) {
         }
         if (prof) prof->mark(st, BW_KERNEL_COLLOID);
     }
-    launch_step_begin(ph, d0, st);
+    launch_step_begin(ph, d0, st, col);  // with colloids it also clears the tile lists
     if (col) {
         launch_md_candidates(ph, d0, c->wgt, c->nwg, st);
-        launch_tile_colloids(ph, d0, st);
+        launch_tile_colloids(ph, d0, st, true);
     }
     if (prof) prof->mark(st, BW_KERNEL_COLLOID);
     launch_depart<Real>(ph, d0, src, st);

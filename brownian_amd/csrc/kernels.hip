@@ -20,7 +20,18 @@
 namespace bw {
 
 // ------------------------------------------------------------- step begin
-__global__ void k_step_begin(Phys ph, Dev d) {
+// Block 0: the step's shifts and counters.  clear_lists: every block also
+// clears the per-tile colloid lists (k_tile_lists_clear, folded in).
+__global__ void k_step_begin(Phys ph, Dev d, int clear_lists) {
+    if (clear_lists) {
+        for (int T = blockIdx.x * blockDim.x + threadIdx.x; T < ph.n_tiles; T += gridDim.x * blockDim.x) {
+            d.tcl_cnt[T] = 0;
+            if (ph.surrogate) d.tsl_cnt[T] = 0;
+        }
+        for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < ph.n_rep; r += gridDim.x * blockDim.x)
+            d.tcl_stale[r] = 0;
+    }
+    if (blockIdx.x != 0) return;
     long long s = *d.step + 1;
     for (int r = threadIdx.x; r < ph.n_rep; r += blockDim.x) {
         double u[3];
@@ -59,18 +70,30 @@ __global__ __launch_bounds__(256) void k_md_candidates(Phys ph, Dev d, const int
     const double L = R.L;
     const double *pi = d.cpos + 3 * g;
     int cnt = 0;
-    for (int base = 0; base < R.nc; base += 64) {
-        int j = R.first_colloid + base + lane;
-        bool p = false;
-        if (base + lane < R.nc && j != g) {
-            const double *pj = d.cpos + 3 * j;
-            double tx = img1(pi[0] - pj[0], L), ty = img1(pi[1] - pj[1], L), tz = img1(pi[2] - pj[2], L);
-            p = (tx * tx + ty * ty) + tz * tz < ph.lim_md_skin;
+    // four rounds of 64 partners per pass: their positions are requested
+    // together, then tested and appended in ascending order
+    constexpr int U = 4;
+    for (int base = 0; base < R.nc; base += 64 * U) {
+        double pj[U][3];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int m = base + 64 * u + lane;
+            const int j = R.first_colloid + min(m, R.nc - 1);
+            for (int k = 0; k < 3; k++) pj[u][k] = d.cpos[3 * j + k];
         }
-        unsigned long long m = __ballot(p);
-        int rank = __popcll(m & ((1ull << lane) - 1ull));
-        if (p && cnt + rank < MDC_CAP) d.mdc_list[g * MDC_CAP + cnt + rank] = j;
-        cnt += __popcll(m);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int m = base + 64 * u + lane, j = R.first_colloid + m;
+            bool p = false;
+            if (m < R.nc && j != g) {
+                double tx = img1(pi[0] - pj[u][0], L), ty = img1(pi[1] - pj[u][1], L), tz = img1(pi[2] - pj[u][2], L);
+                p = (tx * tx + ty * ty) + tz * tz < ph.lim_md_skin;
+            }
+            unsigned long long b = __ballot(p);
+            int rank = __popcll(b & ((1ull << lane) - 1ull));
+            if (p && cnt + rank < MDC_CAP) d.mdc_list[g * MDC_CAP + cnt + rank] = j;
+            cnt += __popcll(b);
+        }
     }
     if (lane == 0) {
         if (cnt > MDC_CAP) { atomicOr(d.status, BW_FLAG_LIST_OVERFLOW); cnt = MDC_CAP; }
@@ -207,17 +230,32 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
     }
     if (drift) atomicOr(d.status, BW_FLAG_DRIFT);
     __syncthreads();
-    // list offsets: direction-major, waves in order within a direction
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < 27; t++) {
-            d.dep_off[T * 28 + t] = acc;
+    // list offsets: direction-major, waves in order within a direction; lane
+    // t of wave 0 owns direction t, the directions' totals are scanned across
+    // lanes
+    if (threadIdx.x < 64) {
+        int c[NW], tot = 0;
+#pragma unroll
+        for (int v = 0; v < NW; v++) {
+            c[v] = lane < 27 ? s_cnt[v][lane] : 0;
+            tot += c[v];
+        }
+        int x = tot;  // inclusive prefix over directions
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane < 27) {
+            int acc = x - tot;
+            d.dep_off[T * 28 + lane] = acc;
+#pragma unroll
             for (int v = 0; v < NW; v++) {
-                s_base[v][t] = acc;
-                acc += s_cnt[v][t];
+                s_base[v][lane] = acc;
+                acc += c[v];
             }
         }
-        d.dep_off[T * 28 + 27] = acc;
+        if (lane == 26) d.dep_off[T * 28 + 27] = x;
     }
     __syncthreads();
     // pass 2: each wave scatters its range in slot order (stable lists)
@@ -1800,7 +1838,9 @@ void launch_gather(const Phys &ph, const Dev &d, const Fluid<Real> &f, double *p
                    hipStream_t st) {
     LAUNCH(k_gather<Real>, ph.n_tiles, 256, 0, st, ph, d, f, pos, vel, cell);
 }
-void launch_step_begin(const Phys &ph, const Dev &d, hipStream_t st) { LAUNCH(k_step_begin, 1, 256, 0, st, ph, d); }
+void launch_step_begin(const Phys &ph, const Dev &d, hipStream_t st, bool clear_lists) {
+    LAUNCH(k_step_begin, clear_lists ? std::max(1, (ph.n_tiles + 255) / 256) : 1, 256, 0, st, ph, d, (int)clear_lists);
+}
 void launch_colloid(const Phys &ph, const Dev &d, unsigned flags, hipStream_t st) {
     LAUNCH(k_colloid, ph.n_rep, 256, 0, st, ph, d, flags);
 }
@@ -1810,8 +1850,8 @@ void launch_colloid_update(const Phys &ph, const Dev &d, unsigned flags, const i
 void launch_md_candidates(const Phys &ph, const Dev &d, const int2 *wgt, int nwg, hipStream_t st) {
     LAUNCH(k_md_candidates, nwg, 256, 0, st, ph, d, wgt);
 }
-void launch_tile_colloids(const Phys &ph, const Dev &d, hipStream_t st) {
-    LAUNCH(k_tile_lists_clear, (ph.n_tiles + 255) / 256, 256, 0, st, ph, d);
+void launch_tile_colloids(const Phys &ph, const Dev &d, hipStream_t st, bool cleared) {
+    if (!cleared) LAUNCH(k_tile_lists_clear, (ph.n_tiles + 255) / 256, 256, 0, st, ph, d);
     if (ph.n_colloid_total > 0) LAUNCH(k_tile_colloids, ph.n_colloid_total, 256, 0, st, ph, d);
 }
 void launch_features(const Phys &ph, const Dev &d, const int *colloid_rep, hipStream_t st) {
