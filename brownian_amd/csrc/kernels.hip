@@ -381,12 +381,12 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         int g = threadIdx.x + k * TH;
         rkey[k] = -1;
         if (g < n) {
-            int lo = 0, hi = 26;  // the source list holding gather index g
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (s_pre[mid] <= g) lo = mid;
-                else hi = mid - 1;
-            }
+            // the source list holding gather index g: the last t with
+            // s_pre[t] <= g, by five fixed halvings (the slots' searches interleave)
+            int lo = 0;
+#pragma unroll
+            for (int h = 16; h > 0; h >>= 1)
+                if (lo + h <= 26 && s_pre[lo + h] <= g) lo += h;
             int S = s_src[lo];
             int li = d.dep_idx[S * cap + s_base[lo] + (g - s_pre[lo])];
             int slot = S * cap + li;
