@@ -376,6 +376,7 @@ def main():
                              f"{len(r['substep_s'])} substep(s) of mean {np.mean(r['substep_s']):.2f} s, step time = "
                              f"rotation + run + {args.nsub} x substep = {r['step_s']:.1f} s; the reference fixes the "
                              f"SRD angle at 130 deg and one v0 (the mean)",
+                   "host_cores_total": os.cpu_count(),
                    "gpu_over_cpu": value / (nf / r["step_s"])}
         elif r:
             cpu = {"value": None, "unit": "fluid particle-steps/s", "cores": 1, "kind": "reference",
