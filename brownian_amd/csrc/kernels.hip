@@ -600,9 +600,45 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 const double x = (double)ix + (double)rox[k], y = (double)iy + (double)roy[k],
                              z = (double)iz + (double)roz[k];
                 unsigned m = cell_mask(s_am[0], rg, ix, iy, iz, L);
-                for_candidates(tc, m, [&](int q) {
-                    if (q < tc.n && shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
-                });
+                if (NEAR_TEST) {
+                    // conservative test in FP32, tile-local coordinates (2e-2 cell slack),
+                    // on the position before any mirroring below
+                    const float Lf = (float)L, iLf = 1.0f / Lf;
+                    const float px = (float)pdiff(ix, ox0, L) + (float)rox[k];
+                    const float py = (float)pdiff(iy, oy0, L) + (float)roy[k];
+                    const float pz = (float)pdiff(iz, oz0, L) + (float)roz[k];
+                    const float vx = (float)rvx[k], vy = (float)rvy[k], vz = (float)rvz[k];
+                    const float tr = sqrtf((vx * vx + vy * vy) + vz * vz) * (float)ph.ndt;
+                    const bool inside01 = rox[k] >= Real(0) && rox[k] < Real(1) && roy[k] >= Real(0) &&
+                                          roy[k] < Real(1) && roz[k] >= Real(0) && roz[k] < Real(1);
+                    near = !(tr <= (float)ph.vcap_ndt) || !inside01;
+                    const float rs = sqrtf((float)ph.near_shell2) + 0.02f, rs2 = rs * rs;
+                    const float rc = (float)ph.near_base + tr + 0.02f, rc2 = rc * rc;
+                    auto d2 = [&](float cx, float cy, float cz) {
+                        float ax = px - cx, ay = py - cy, az = pz - cz;
+                        ax -= Lf * rintf(ax * iLf); ay -= Lf * rintf(ay * iLf); az -= Lf * rintf(az * iLf);
+                        return (ax * ax + ay * ay) + az * az;
+                    };
+                    // run() shells: the FP32 test selects the candidates that get the
+                    // exact test (it accepts every particle the exact test accepts)
+                    // and makes them near
+                    for_candidates(tc, m, [&](int q) {
+                        if (q < tc.n && d2(nc[0].rx[q], nc[0].ry[q], nc[0].rz[q]) <= rs2) {
+                            near = true;
+                            if (shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
+                        }
+                    });
+                    if (!near) {  // within reach of a collision during the step
+                        unsigned m2 = cell_mask(s_am[COUNT_SHELL ? 1 : 0], rg, ix, iy, iz, L);
+                        for_candidates(tc, m2, [&](int q) {
+                            if (q < tc.n && !near) near = d2(nc[0].cx[q], nc[0].cy[q], nc[0].cz[q]) <= rc2;
+                        });
+                    }
+                } else {
+                    for_candidates(tc, m, [&](int q) {
+                        if (q < tc.n && shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
+                    });
+                }
                 if (ph.surrogate) {
                     // fluid that entered a surrogate-driven colloid during the last
                     // step is mirrored back out through its surface, its velocity
@@ -637,35 +673,6 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                         double ax = img1(x - tc.px[q], Ld), ay = img1(y - tc.py[q], Ld), az = img1(z - tc.pz[q], Ld);
                         if ((ax * ax + ay * ay) + az * az <= ph.lim_coll) rcell[k] |= PHANTOM;
                     });
-                }
-                if (NEAR_TEST) {
-                    // conservative test in FP32, tile-local coordinates (2e-2 cell slack)
-                    const float Lf = (float)L, iLf = 1.0f / Lf;
-                    const float px = (float)pdiff(ix, ox0, L) + (float)rox[k];
-                    const float py = (float)pdiff(iy, oy0, L) + (float)roy[k];
-                    const float pz = (float)pdiff(iz, oz0, L) + (float)roz[k];
-                    const float vx = (float)rvx[k], vy = (float)rvy[k], vz = (float)rvz[k];
-                    const float tr = sqrtf((vx * vx + vy * vy) + vz * vz) * (float)ph.ndt;
-                    const bool inside01 = rox[k] >= Real(0) && rox[k] < Real(1) && roy[k] >= Real(0) &&
-                                          roy[k] < Real(1) && roz[k] >= Real(0) && roz[k] < Real(1);
-                    near = !(tr <= (float)ph.vcap_ndt) || !inside01;
-                    const float rs = sqrtf((float)ph.near_shell2) + 0.02f, rs2 = rs * rs;
-                    const float rc = (float)ph.near_base + tr + 0.02f, rc2 = rc * rc;
-                    auto d2 = [&](float cx, float cy, float cz) {
-                        float ax = px - cx, ay = py - cy, az = pz - cz;
-                        ax -= Lf * rintf(ax * iLf); ay -= Lf * rintf(ay * iLf); az -= Lf * rintf(az * iLf);
-                        return (ax * ax + ay * ay) + az * az;
-                    };
-                    if (!near)  // within reach of a run() kick
-                        for_candidates(tc, m, [&](int q) {
-                            if (q < tc.n && !near) near = d2(nc[0].rx[q], nc[0].ry[q], nc[0].rz[q]) <= rs2;
-                        });
-                    if (!near) {  // within reach of a collision during the step
-                        unsigned m2 = cell_mask(s_am[COUNT_SHELL ? 1 : 0], rg, ix, iy, iz, L);
-                        for_candidates(tc, m2, [&](int q) {
-                            if (q < tc.n && !near) near = d2(nc[0].cx[q], nc[0].cy[q], nc[0].cz[q]) <= rc2;
-                        });
-                    }
                 }
             }
             if (near || moved) nearbits |= 1u << k;
