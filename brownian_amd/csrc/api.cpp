@@ -18,6 +18,7 @@ namespace bw {
 // kernel launchers (kernels.hip)
 size_t srd_lds_bytes(int ncell, int cap, bool fp64);
 size_t srd_static_lds();
+bool srd_needs_wrap(int L, int bmax);
 template <class Real> void launch_depart(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
 template <class Real>
 void launch_srd(const Phys &, const Dev &, const Fluid<Real> &, const Fluid<Real> &, bool, bool, bool, size_t,
@@ -537,6 +538,9 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
     }
     ph.cap = cap;
     ph.srd_threads = cap <= 4 * 256 ? 256 : cap <= 4 * 512 ? 512 : 1024;
+    ph.near_wrap = 0;
+    for (auto &R : c->reps)
+        if (srd_needs_wrap(R.L, std::max(R.B[0], std::max(R.B[1], R.B[2])))) ph.near_wrap = 1;
     ph.n_tiles = (int)tiles;
     ph.n_colloid_total = nc;
     c->total_slots = tiles * cap;

@@ -75,6 +75,7 @@ struct Phys {
     int cap;                          // slots per tile
     int srd_threads;                  // SRD workgroup size (4 slots per thread)
     int near_blocks;                  // grid of the near-particle pass
+    int near_wrap;                    // the SRD near test needs the periodic wrap (a small box)
     int n_rep, n_tiles, n_colloid_total;
     int draws;                        // BW_DRAWS_*
 };

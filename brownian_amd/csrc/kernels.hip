@@ -300,7 +300,9 @@ __host__ __device__ inline size_t srd_cell_region(int ncell, int cap) {
     return a > b ? a : b;
 }
 
-template <class Real, bool COUNT_SHELL, bool MAP, bool BLOCK, int TH>
+// WRAP: the near test applies the periodic wrap to its FP32 differences
+// (srd_needs_wrap); otherwise they stay below L/2, where the wrap is the identity.
+template <class Real, bool COUNT_SHELL, bool MAP, bool BLOCK, int TH, bool WRAP>
 __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Fluid<Real> dst) {
     constexpr bool NEAR_TEST = COUNT_SHELL && BLOCK;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -616,7 +618,9 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                     const float rc = (float)ph.near_base + tr + 0.02f, rc2 = rc * rc;
                     auto d2 = [&](float cx, float cy, float cz) {
                         float ax = px - cx, ay = py - cy, az = pz - cz;
-                        ax -= Lf * rintf(ax * iLf); ay -= Lf * rintf(ay * iLf); az -= Lf * rintf(az * iLf);
+                        if constexpr (WRAP) {
+                            ax -= Lf * rintf(ax * iLf); ay -= Lf * rintf(ay * iLf); az -= Lf * rintf(az * iLf);
+                        }
                         return (ax * ax + ay * ay) + az * az;
                     };
                     // run() shells: the FP32 test selects the candidates that get the
@@ -1752,15 +1756,26 @@ template <class Real>
 void launch_depart(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream_t st) {
     LAUNCH(k_depart<Real>, ph.n_tiles, DEPART_THREADS, 0, st, ph, d, f);
 }
+template <class Real, bool C, bool M, bool B, bool W>
+static void srd_th(const Phys &ph, const Dev &d, const Fluid<Real> &src, const Fluid<Real> &dst, size_t lds,
+                   hipStream_t st) {
+    switch (ph.srd_threads) {
+        case 256: LAUNCH((k_srd<Real, C, M, B, 256, W>), ph.n_tiles, 256, lds, st, ph, d, src, dst); break;
+        case 512: LAUNCH((k_srd<Real, C, M, B, 512, W>), ph.n_tiles, 512, lds, st, ph, d, src, dst); break;
+        default: LAUNCH((k_srd<Real, C, M, B, 1024, W>), ph.n_tiles, 1024, lds, st, ph, d, src, dst); break;
+    }
+}
 template <class Real, bool C, bool M, bool B>
 static void srd_one(const Phys &ph, const Dev &d, const Fluid<Real> &src, const Fluid<Real> &dst, size_t lds,
                     hipStream_t st) {
-    switch (ph.srd_threads) {
-        case 256: LAUNCH((k_srd<Real, C, M, B, 256>), ph.n_tiles, 256, lds, st, ph, d, src, dst); break;
-        case 512: LAUNCH((k_srd<Real, C, M, B, 512>), ph.n_tiles, 512, lds, st, ph, d, src, dst); break;
-        default: LAUNCH((k_srd<Real, C, M, B, 1024>), ph.n_tiles, 1024, lds, st, ph, d, src, dst); break;
-    }
+    if (C && B && !ph.near_wrap) srd_th<Real, C, M, B, false>(ph, d, src, dst, lds, st);
+    else srd_th<Real, C, M, B, true>(ph, d, src, dst, lds, st);
 }
+// The SRD pass's FP32 near test compares a particle (within TILE_REACH + 1
+// cells of its tile box) with listed colloids (within TILE_LIST_REACH + 1
+// cells of it): every difference stays below L/2, where the periodic wrap is
+// the identity, once L >= 4 (Bmax + TILE_LIST_REACH + TILE_REACH + 2).
+bool srd_needs_wrap(int L, int bmax) { return L < 4 * (bmax + TILE_LIST_REACH + TILE_REACH + 2); }
 // Dynamic LDS up to the CU's 160 KB is granted at launch on gfx950 without an
 // opt-in attribute; the context checks the SRD pass's need against
 // 160 KB minus its static LDS when it is created.
