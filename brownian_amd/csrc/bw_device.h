@@ -220,6 +220,24 @@ __device__ __forceinline__ void normalise(int &i, Real &o, int L) {
 }
 
 // o in [0,1) (+0 included, -0 and NaN excluded) with one integer compare
+// normalise() for a particle that moves less than one cell in all: the cell
+// index changes by at most one in the whole run of substeps, so its periodic
+// wrap (which the offset does not depend on) is left to the caller, once
+// after the run (i in [-1, L] then).
+template <class Real>
+__device__ __forceinline__ void normalise_nowrap(int &i, Real &o) {
+    Real f = floor(o);
+    if (f != Real(0)) {
+        o -= f;
+        i += (int)f;
+        if (o >= Real(1)) {  // o + 1 rounded up to 1 for a tiny negative o
+            o = Real(0);
+            i += 1;
+        }
+    }
+}
+__device__ __forceinline__ int wrap_once(int i, int L) { return i >= L ? i - L : (i < 0 ? i + L : i); }
+
 __device__ __forceinline__ bool in_unit(float o) { return __float_as_uint(o) < 0x3F800000u; }
 __device__ __forceinline__ bool in_unit(double o) {
     return (unsigned long long)__double_as_longlong(o) < 0x3FF0000000000000ull;

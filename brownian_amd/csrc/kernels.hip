@@ -730,16 +730,20 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 Real ox = rox[k], oy = roy[k], oz = roz[k];
                 const Real dt = (Real)ph.dt;
                 const Real dx = rvx[k] * dt, dy = rvy[k] * dt, dz = rvz[k] * dt;
+                // (a far particle travels less than a cell in the step: each axis
+                // crosses a cell face at most once, so the periodic wrap of the
+                // cell index is applied once, after the substeps)
                 for (int l = 0; l < ph.n_sub; l++) {
                     ox = ox + dx;
                     oy = oy + dy;
                     oz = oz + dz;
                     if (!(in_unit(ox) && in_unit(oy) && in_unit(oz))) {
-                        normalise(ix, ox, L);
-                        normalise(iy, oy, L);
-                        normalise(iz, oz, L);
+                        normalise_nowrap(ix, ox);
+                        normalise_nowrap(iy, oy);
+                        normalise_nowrap(iz, oz);
                     }
                 }
+                ix = wrap_once(ix, L); iy = wrap_once(iy, L); iz = wrap_once(iz, L);
                 rcell[k] = pack_cell(ix, iy, iz) | (rcell[k] & PHANTOM);
                 rox[k] = ox; roy[k] = oy; roz[k] = oz;
                 if (!(near_interval(ix, ox0 - TILE_REACH, B0 + 2 * TILE_REACH, L) &&
