@@ -472,6 +472,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         if (g < n) {
             const int b = s_start[rkey[k]], e = s_start[rkey[k] + 1];
             int rank = 0;
+#pragma unroll 4
             for (int j = b; j < e; j++) rank += s_pk[j] < rpid[k] ? 1 : 0;
             s_ord[b + rank] = (unsigned short)g;
         }
@@ -487,6 +488,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             if (comp < 3) {
                 const Real *sv = s_v + comp * cap;
                 double u = 0;
+#pragma unroll 4
                 for (int i = b; i < e; i++) u = u + (double)sv[s_ord[i]] / N;
                 s_u[3 * c + comp] = u;
             } else {
@@ -536,6 +538,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         const int b = s_start[c], e = s_start[c + 1], N = e - b;
         if (N > 1) {
             double var = 0.0;
+#pragma unroll 4
             for (int i = b; i < e; i++) var += s_e2[s_ord[i]];
             double scale = sqrt(3.0 * (N - 1) * ph.kbt / (ph.mass_fl * var));
             if (!(var > 0)) {
