@@ -555,6 +555,13 @@ __device__ __forceinline__ void for_candidates(const TileCore &tc, unsigned m, F
     }
     for (int k = MASK_BITS; k < tc.n; k++) f(k);
 }
+// every listed colloid, ascending (the order for_candidates gives an all-ones
+// mask), unrolled so that the LDS reads of several colloids are in flight together
+template <class F>
+__device__ __forceinline__ void for_listed(const TileCore &tc, F &&f) {
+#pragma unroll 4
+    for (int k = 0; k < tc.n; k++) f(k);
+}
 
 // ------------------------------------------------------------ colloid work
 // Every colloid-only reference entry point, as per-colloid functions shared by

@@ -822,14 +822,16 @@ struct StreamOp {
                 double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
                 double wx = vx, wy = vy, wz = vz;
                 bool hit = false;
-                for_candidates(tc, m, [&](int k) {
+                auto kick = [&](int k) {
                     if (k < tc.n && shell_test(ph, tc, k, L, ix, iy, iz, x, y, z)) {
                         wx = wx - tc.kx[k];
                         wy = wy - tc.ky[k];
                         wz = wz - tc.kz[k];
                         hit = true;
                     }
-                });
+                };
+                if constexpr (DIRECT) for_listed(tc, kick);
+                else for_candidates(tc, m, kick);
                 if (hit) {
                     vx = (Real)wx; vy = (Real)wy; vz = (Real)wz;
                     f.vx[slot] = vx; f.vy[slot] = vy; f.vz[slot] = vz;
@@ -858,7 +860,7 @@ struct StreamOp {
             double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
             int n_in = 0, gin = -1;
             bool inside_any = false;  // by distance alone (the stencil skips a colloid's own cell)
-            for_candidates(tc, m, [&](int k) {
+            auto test = [&](int k) {
                 if (k >= tc.n) return;
                 if (DIRECT) {  // inside a colloid: within 3 cells of its cell on every axis
                     const int ax = pdiff(ix, tc.cx[k], L), ay = pdiff(iy, tc.cy[k], L), az = pdiff(iz, tc.cz[k], L);
@@ -872,7 +874,9 @@ struct StreamOp {
                         gin = tc.g[k];
                     }
                 }
-            });
+            };
+            if constexpr (DIRECT) for_listed(tc, test);
+            else for_candidates(tc, m, test);
             if (phantom) {  // no events until it is inside no colloid
                 if (!inside_any) f.cell[slot] = pack_cell(ix, iy, iz);
                 return;
@@ -1168,6 +1172,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
         key = ((long long)stencil_index(dx, dy, dz) << 32) | (unsigned)pid;
     }
     int rank = 0;
+#pragma unroll 4
     for (int m = 0; m < ne; m++) {
         long long km = __shfl(key, m);
         rank += (km < key) ? 1 : 0;
@@ -1190,6 +1195,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             // there is one, the rest of the colloid loop is replayed serially.
             const double ax = (double)ix + bx, ay = (double)iy + by, az = (double)iz + bz;
             const int nnb = s_nnb[w];
+#pragma unroll 2
             for (int m = 0; m < nnb; m++) {
                 const PairColl &e = s_nb[w][m];
                 if (!in_stencil(ix, iy, iz, e.cx, e.cy, e.cz, L)) continue;
@@ -1216,6 +1222,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     __syncthreads();
     if (lane == 0 && valid) {
         double a[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 2
         for (int e = 0; e < s_ne[w]; e++)
             for (int k = 0; k < 6; k++) a[k] = a[k] + s_c[w][e][k];
         for (int k = 0; k < 3; k++) {
@@ -1409,6 +1416,7 @@ __global__ __launch_bounds__(256) void k_colloid_update(Phys ph, Dev d, unsigned
         double fn[3] = {0, 0, 0};
         if (flags & PH_FORCE) {
             double f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll 4
             for (int m = 0; m < nmd; m++) {  // cpp/update.cpp:28 in list order
                 f0 = f0 + s_f[w][m][0];
                 f1 = f1 + s_f[w][m][1];
