@@ -532,6 +532,30 @@ __device__ inline void build_axis_masks(const TileColl &tc, const Region &g, int
     if (run_cells) build_axis_masks(tc.n, tc.qx, tc.qy, tc.qz, g, L, reach, am);
     else build_axis_masks(tc.n, tc.cx, tc.cy, tc.cz, g, L, reach, am);
 }
+// both mask sets of a tile with one pair of barriers: run() cells within
+// reach0 into am0, current cells within reach1 into am1 (as two calls would)
+__device__ inline void build_axis_masks2(const TileColl &tc, const Region &g, int L, int reach0, AxisMask &am0,
+                                         int reach1, AxisMask &am1) {
+    for (int i = threadIdx.x; i < 3 * AXIS_MAX; i += blockDim.x) {
+        (&am0.m[0][0])[i] = 0u;
+        (&am1.m[0][0])[i] = 0u;
+    }
+    __syncthreads();
+    const int nk = min(tc.n, MASK_BITS);
+    for (int w = threadIdx.x; w < nk * 6; w += blockDim.x) {
+        const bool cur = w >= nk * 3;
+        const int v = cur ? w - nk * 3 : w, k = v / 3, a = v - 3 * k;
+        const int c = cur ? (a == 0 ? tc.cx[k] : a == 1 ? tc.cy[k] : tc.cz[k])
+                          : (a == 0 ? tc.qx[k] : a == 1 ? tc.qy[k] : tc.qz[k]);
+        const int reach = cur ? reach1 : reach0;
+        AxisMask &am = cur ? am1 : am0;
+        for (int dd = -reach; dd <= reach; dd++) {
+            int u = axis_index(g, a, wrapi(c + dd, L), L);
+            if (u >= 0) atomicOr(&am.m[a][u], 1u << k);
+        }
+    }
+    __syncthreads();
+}
 
 // run() rear-shell test of listed colloid k for a particle at binned cell
 // (ix,iy,iz) and absolute position (x,y,z) (cpp/run.cpp:17-18)

@@ -418,9 +418,9 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             if (threadIdx.x == 0) tc.n = cl_n;
             __syncthreads();
         }
-        build_axis_masks(tc, rg, L, true, 3, s_am[0]);
+        if (NEAR_TEST) build_axis_masks2(tc, rg, L, 3, s_am[0], ph.near_reach, s_am[COUNT_SHELL ? 1 : 0]);
+        else build_axis_masks(tc, rg, L, true, 3, s_am[0]);
         if (NEAR_TEST) {
-            build_axis_masks(tc, rg, L, false, ph.near_reach, s_am[COUNT_SHELL ? 1 : 0]);
             const double Ld = (double)L;
             for (int q = threadIdx.x; q < tc.n; q += blockDim.x) {
                 nc[0].cx[q] = (float)img1(tc.px[q] - ox0, Ld);
