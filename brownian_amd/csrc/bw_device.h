@@ -22,6 +22,19 @@ __device__ __forceinline__ T ld_const(const T *p) {
     return v;
 }
 
+// Value of lane l (the same l in every lane) of a wave, through the scalar
+// unit rather than a cross-lane LDS permute.
+__device__ __forceinline__ int lane_value(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ long long lane_value(long long v, int l) {
+    const unsigned long long u = (unsigned long long)v;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+    return __longlong_as_double(lane_value((long long)__double_as_longlong(v), l));
+}
+
 // ------------------------------------------------------------------ generator
 // Counter-based generator for the device draw source: Philox4x32-10
 // (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",

@@ -267,9 +267,9 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
         int off = 0;
         unsigned long long pending = __ballot(dd >= 0);
         while (pending) {  // one round per direction present in the group
-            const int t = __shfl(dd, __ffsll((long long)pending) - 1);
+            const int t = lane_value(dd, __ffsll((long long)pending) - 1);
             const unsigned long long m = __ballot(dd == t);
-            const int rt = __shfl(run, t);
+            const int rt = lane_value(run, t);
             if (dd == t) off = rt + __popcll(m & below);
             if (lane == t) run += __popcll(m);
             pending &= ~m;
@@ -757,7 +757,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         }
     }
     if (publish) {
-        chunk0 = __shfl(chunk0, 0);
+        chunk0 = lane_value(chunk0, 0);
         for (int j = threadIdx.x; j < (n_near + 63) >> 6; j += 64) d.chunk_tile[chunk0 + j] = (T << 6) | j;
     }
 #pragma unroll
@@ -1174,7 +1174,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
     int rank = 0;
 #pragma unroll 4
     for (int m = 0; m < ne; m++) {
-        long long km = __shfl(key, m);
+        long long km = lane_value(key, m);
         rank += (km < key) ? 1 : 0;
     }
     if (lane < ne) {
