@@ -1402,7 +1402,6 @@ __global__ __launch_bounds__(256) void k_colloid_update(Phys ph, Dev d, unsigned
     }
     const Rep R = d.rep[r];
     const bool valid = g < R.first_colloid + R.nc;
-    const double L = (double)R.L;
     int nmd = 0;
     if (valid && (flags & PH_FORCE)) {
         nmd = nmd_s;
