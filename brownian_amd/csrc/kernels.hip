@@ -1172,7 +1172,6 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
         key = ((long long)stencil_index(dx, dy, dz) << 32) | (unsigned)pid;
     }
     int rank = 0;
-#pragma unroll 4
     for (int m = 0; m < ne; m++) {
         long long km = lane_value(key, m);
         rank += (km < key) ? 1 : 0;
