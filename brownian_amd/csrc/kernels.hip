@@ -736,6 +736,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 // (a far particle travels less than a cell in the step: each axis
                 // crosses a cell face at most once, so the periodic wrap of the
                 // cell index is applied once, after the substeps)
+#pragma unroll 2
                 for (int l = 0; l < ph.n_sub; l++) {
                     ox = ox + dx;
                     oy = oy + dy;
