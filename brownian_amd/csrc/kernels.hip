@@ -1167,7 +1167,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
             s_nnb[w] = min(nnb, MD_CAP);
         }
     }
-    __syncthreads();  // pair lists visible to every lane
+    wave_sync();  // the wave's pair partners and list visible to its lanes (LDS is per wave)
     if (lane < ne) {
         int dx = pdiff(ix, cg[0], L), dy = pdiff(iy, cg[1], L), dz = pdiff(iz, cg[2], L);
         key = ((long long)stencil_index(dx, dy, dz) << 32) | (unsigned)pid;
@@ -1219,7 +1219,7 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_collide(Phys ph, Dev d, Flu
         for (int k = 0; k < 6; k++) s_c[w][rank][k] = cont[k];
     }
     if (lane == 0) s_ne[w] = ne;
-    __syncthreads();
+    wave_sync();
     if (lane == 0 && valid) {
         double a[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll 2
