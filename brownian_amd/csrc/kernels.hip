@@ -193,7 +193,10 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
     const int lo = min(n, w * per), hi = min(n, lo + per);
     for (int t = lane; t < 27; t += 64) s_cnt[w][t] = 0;
     bool drift = false;
-    // pass 1: the departure direction of each particle, counts per wave
+    // pass 1: the departure direction of each particle, counts per wave (the
+    // particles that stay in the tile, most of them, are counted by ballot
+    // rather than by contended LDS atomics)
+    int stay = 0;  // lane 0: direction 13 (same tile) of this wave
     for (int g0 = lo; g0 < hi; g0 += 64 * U) {
         int c[U];
         Real ox[U], oy[U], oz[U];
@@ -209,6 +212,7 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int i = g0 + 64 * u + lane;
+            int dir = -1;
             if (i < hi) {
                 int ix, iy, iz;
                 unpack_cell(c[u], ix, iy, iz);
@@ -222,13 +226,16 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
                     drift = true;
                     dx = max(-1, min(1, dx)); dy = max(-1, min(1, dy)); dz = max(-1, min(1, dz));
                 }
-                const int dir = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                dir = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
                 s_dir[i] = (unsigned char)dir;
-                atomicAdd(&s_cnt[w][dir], 1);
+                if (dir != 13) atomicAdd(&s_cnt[w][dir], 1);
             }
+            const unsigned long long m13 = __ballot(dir == 13);
+            if (lane == 0) stay += __popcll(m13);
         }
     }
     if (drift) atomicOr(d.status, BW_FLAG_DRIFT);
+    if (lane == 0) s_cnt[w][13] += stay;
     __syncthreads();
     // list offsets: direction-major, waves in order within a direction; lane
     // t of wave 0 owns direction t, the directions' totals are scanned across
