@@ -11,9 +11,11 @@ uniform on the sphere.  Synthetic initial state from a fixed seed.
 One line of JSON: fluid particle-steps/s of the fused, graph-replayed step
 (inputs resident in HBM); the HBM roofline of the step's dominant kernel from
 HIP events on the launch stream (algorithmic bytes per launch / average launch
-time); and the reference cpp/ program (compiled from its sources into
-oracle/_ref) timed on one host core on a bounded sample of the same state
-(rotation + run() + the first substeps, extrapolated to one MPCD step).
+time); the plain method (every particle streamed every substep) timed beside
+it; and the CPU oracle (the serial restatement of the reference cpp/ step,
+built from oracle/oracle.c at -O2 and -O1) timed on one host core on a bounded
+sample of the same state (rotation + run() + the first substeps, extrapolated
+to one MPCD step).
 
   python bench.py [--steps K --warmup W --L 128 --ncol 2400]
   python bench.py --pmc-summary FETCH_DIR WRITE_DIR --out FILE   (rocprofv3 --pmc outputs -> bytes per launch)
@@ -54,14 +56,16 @@ def workload_params(args):
                 mass_colloid=50.0, sig_colloid=4.2, eps=1.0, sigma=4.0, theta_deg=90.0, v0=0.0, seed=args.seed)
 
 
-def cpu_reference_baseline(args):
-    """Time the reference cpp/ (compiled from its sources into oracle/_ref) on
-    one host core, in a child process, on a bounded sample of this workload:
-    the same start-up state (recomputed on the host), one rotation_mpcd + run()
-    and the first substep(s); the step time is rotation + run + n_sub x the mean
-    substep."""
+def cpu_baseline(args, opt):
+    """Time the CPU oracle -- the project's serial FP64 C restatement of the
+    reference cpp/ step (oracle/oracle.c, bit-exact with the compiled reference
+    on the golden vectors), built at -O2 or -O1 (the reference Makefile's
+    level, cpp/Makefile:3) -- on one host core, in a child process, on a
+    bounded sample of this workload: the same start-up state (recomputed on
+    the host), one rotation_mpcd + run() and the first substeps; the step time
+    is rotation + run + n_sub x the mean substep."""
     import subprocess
-    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--cpu-worker", "--L", str(args.L), "--ncol",
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--cpu-worker", opt, "--L", str(args.L), "--ncol",
            str(args.ncol), "--nsub", str(args.nsub), "--seed", str(args.seed)]
     p = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
     if p.returncode != 0:
@@ -73,30 +77,39 @@ def cpu_worker(args):
     """Child process: no GPU use.  Prints progress on stderr, one JSON line."""
     import oracle as O
     import brownian_amd as B
-    if not os.path.exists(O.REF_SO):
-        print(json.dumps(dict(error="oracle/_ref not built")))
+    opt = args.cpu_worker
+    so = {"O2": O.ORACLE_SO, "O1": O.ORACLE_SO_O1, "ref": O.REF_SO}[opt]
+    if not os.path.exists(so):
+        print(json.dumps(dict(error=f"{so} not built")))
         return
     prm = workload_params(args)
     t0 = time.time()
     st = B.host_initialize(vel_law=B.BW_VEL_MAXWELL, orient_law=B.BW_ORIENT_SPHERE, **prm)
     v0 = np.random.default_rng(args.seed).uniform(0.0, 0.05, args.ncol)
-    log(f"cpu worker: start-up state recomputed on the host ({time.time() - t0:.1f} s)")
-    r = O.Reference(**{**prm, "v0": float(np.mean(v0)), "theta_deg": 130.0})
-    r.initialize()
-    r.set_fluid(st["pos"], st["vel"])
+    log(f"cpu worker {opt}: start-up state recomputed on the host ({time.time() - t0:.1f} s)")
     z = np.zeros_like(st["c_pos"])
+    if opt == "ref":  # the compiled reference: 130 deg and one v0 (the mean) are fixed in its source
+        r = O.Reference(**{**prm, "v0": float(np.mean(v0)), "theta_deg": 130.0})
+        r.initialize()
+    else:
+        r = O.Oracle(so=so, **prm)
+        r.set_v0(v0)
+    r.set_fluid(st["pos"], st["vel"])
     r.set_colloid(st["c_pos"], st["c_vel"], st["c_omega"], z, z, st["c_ra"])
     del st
     r.create_box()
     r.neighbour_list_mpcd()
     r.neighbour_list_md()
     r.compute_force_md()
-    log(f"cpu worker: reference loaded ({time.time() - t0:.1f} s); timing rotation_mpcd + run")
+    log(f"cpu worker {opt}: state loaded ({time.time() - t0:.1f} s); timing rotation_mpcd + run")
     c0 = time.process_time()
-    r.rotation_mpcd_bigstack()
+    if opt == "ref":
+        r.rotation_mpcd_bigstack()
+    else:
+        r.rotation_mpcd()
     r.run()
     t_rot = time.process_time() - c0
-    log(f"cpu worker: rotation_mpcd + run {t_rot:.2f} s; timing substeps")
+    log(f"cpu worker {opt}: rotation_mpcd + run {t_rot:.2f} s; timing substeps")
     subs = []
     # one substep as in cpp/active_brownian.cpp:27-37 (no updown_velocity: nn <= 10000)
     while len(subs) < args.nsub and (not subs or sum(subs) < 10.0):
@@ -111,7 +124,7 @@ def cpu_worker(args):
         r.compute_force_md()
         r.update_velocity_colloid()
         subs.append(time.process_time() - c0)
-        log(f"cpu worker: substep {len(subs)} {subs[-1]:.2f} s")
+        log(f"cpu worker {opt}: substep {len(subs)} {subs[-1]:.2f} s")
     step_s = t_rot + args.nsub * float(np.mean(subs))
     print(json.dumps(dict(step_s=step_s, rot_run_s=t_rot, substep_s=subs)))
 
@@ -277,8 +290,10 @@ def main():
     ap.add_argument("--surrogate-collect", type=int, default=40)
     ap.add_argument("--surrogate-epochs", type=int, default=8)
     ap.add_argument("--surrogate-steps", type=int, default=100)
+    ap.add_argument("--plain-steps", type=int, default=10, help="timed steps of per-substep streaming (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-ref", action="store_true", help="also time the compiled reference (oracle/_ref)")
+    ap.add_argument("--cpu-worker", choices=["O2", "O1", "ref"], help=argparse.SUPPRESS)
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--out")
     ap.add_argument("--command", default="")
@@ -363,27 +378,55 @@ def main():
     copy_ms = s.debug_copy_ms(COPY_BYTES, reps=args.copy_reps)
     copy_bw = 2 * COPY_BYTES / (copy_ms / 1e3)
 
+    # the plain method for comparison: every particle streamed every substep
+    plain = None
+    if args.plain_steps > 0 and args.streaming == "blocked":
+        s2 = B.System(**dict(sprm, streaming=B.BW_STREAM_PER_SUBSTEP))
+        del s  # one full-size context at a time
+        s2.initialize(); s2.initialize_colloid(); s2.initialize_fluid(); s2.create_box()
+        s2.neighbour_list_md(); s2.compute_force_md(); s2.tumble(); s2.set_v0(v0)
+        s2.step(2)
+        s2.status()
+        pms = s2.time_steps(args.plain_steps)
+        s2.status()
+        pval = nf / (pms / 1e3 / args.plain_steps)
+        plain = {"streaming": "per-substep", "steps": args.plain_steps, "ms_per_step": pms / args.plain_steps,
+                 "fluid_particle_steps_per_s": pval, "alg_bytes_per_particle_step": 44 * args.nsub + 40,
+                 "hbm_frac": (44 * args.nsub + 40) * pval / HBM_PEAK}
+        log(f"plain per-substep streaming: {pms / args.plain_steps:.3f} ms/step, "
+            f"{plain['hbm_frac']:.3f} of HBM peak at {44 * args.nsub + 40} B/particle-step")
+        del s2
+    else:
+        del s
+
     cpu = None
     if not args.no_cpu_baseline:
-        s_ref = time.time()
-        log("reference cpp/ on one host core: bounded sample (progress follows)")
-        r = cpu_reference_baseline(args)
-        log(f"reference sample done in {time.time() - s_ref:.1f} s: {r}")
+        runs = {}
+        for opt in ["O2", "O1"] + (["ref"] if args.cpu_ref else []):
+            s_ref = time.time()
+            log(f"CPU oracle ({opt}) on one host core: bounded sample (progress follows)")
+            runs[opt] = cpu_baseline(args, opt)
+            log(f"CPU sample {opt} done in {time.time() - s_ref:.1f} s: {runs[opt]}")
+        r = runs["O2"]
         if r and "step_s" in r:
-            cpu = {"value": nf / r["step_s"], "unit": "fluid particle-steps/s", "cores": 1, "kind": "reference",
-                   "sample": f"reference cpp/ compiled -O2 (oracle/_ref), one host core, process CPU time, on this "
-                             f"workload's start-up state: rotation_mpcd + run {r['rot_run_s']:.2f} s and "
-                             f"{len(r['substep_s'])} substep(s) of mean {np.mean(r['substep_s']):.2f} s, step time = "
-                             f"rotation + run + {args.nsub} x substep = {r['step_s']:.1f} s; the reference fixes the "
-                             f"SRD angle at 130 deg and one v0 (the mean)",
+            cpu = {"value": nf / r["step_s"], "unit": "fluid particle-steps/s", "cores": 1, "kind": "port",
+                   "sample": f"CPU oracle (oracle/oracle.c, the serial FP64 restatement of the reference cpp/ "
+                             f"step, bit-exact with it on the golden vectors) compiled -O2, one host core, process "
+                             f"CPU time, on this workload's start-up state (90 deg, per-colloid v0): rotation_mpcd "
+                             f"+ run {r['rot_run_s']:.2f} s and {len(r['substep_s'])} substep(s) of mean "
+                             f"{np.mean(r['substep_s']):.2f} s; step time = rotation + run + {args.nsub} x substep "
+                             f"= {r['step_s']:.1f} s",
                    "host_cores_total": os.cpu_count(),
                    "gpu_over_cpu": value / (nf / r["step_s"])}
+            for opt, ro in runs.items():
+                if opt != "O2" and ro and "step_s" in ro:
+                    cpu[f"value_{opt}"] = nf / ro["step_s"]
+                    cpu[f"step_s_{opt}"] = ro["step_s"]
         elif r:
-            cpu = {"value": None, "unit": "fluid particle-steps/s", "cores": 1, "kind": "reference",
+            cpu = {"value": None, "unit": "fluid particle-steps/s", "cores": 1, "kind": "port",
                    "sample": "failed: " + str(r.get("error"))}
     ens = None
     if args.ensemble > 0:
-        del s
         try:
             ens = run_ensemble(B, args.ensemble, args.ensemble_steps, 5)
         except Exception as e:  # reported, the single-system figure stands
@@ -421,8 +464,8 @@ def main():
         "kernels": kern,
         "near_fraction": n_near / nf,
         "step_bytes_per_particle": {"plain_per_substep": 44 * args.nsub + 40, "fused_declared": 84,
-                                    "plain_equivalent_hbm_frac": (44 * args.nsub + 40) * value / HBM_PEAK,
                                     "fused_hbm_frac": 84 * value / HBM_PEAK},
+        "plain_per_substep": plain,
         "autotune": None if tune is None else {"best": list(tune["best"]),
                                                 "ms_per_step": {str(k): v for k, v in tune["ms_per_step"].items()}},
         "ensemble": ens,

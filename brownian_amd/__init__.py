@@ -132,6 +132,8 @@ class System:
         rc = self.lib.bw_ctx_create_ensemble(arr, len(replicas), C.byref(h))
         if rc:
             msg = self.lib.bw_last_error(h).decode() if h.value else "create failed"
+            if h.value:  # a partly built context holds device memory and a stream
+                self.lib.bw_ctx_destroy(h)
             raise BrownianError(rc, msg)
         self.h = h
         self.n_rep = len(replicas)

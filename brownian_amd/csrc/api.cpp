@@ -492,7 +492,10 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         const bw_params &p = ps[r];
         ReplicaHost &R = c->reps[r];
         R.p = p;
-        if (p.L < 7 || p.L >= (1 << CELL_BITS) || p.density < 1 || p.n_colloid < 0 || p.n_sub < 1) {
+        // particle ids and counts are int: the fluid of one replica (and of
+        // all together) must stay below 2^31
+        if (p.L < 7 || p.L >= (1 << CELL_BITS) || p.density < 1 || p.n_colloid < 0 || p.n_sub < 1 ||
+            (long long)p.L * p.L * p.L * p.density > INT_MAX || nf + (long long)p.L * p.L * p.L * p.density > INT_MAX) {
             delete c;
             return BW_ERR_ARG;
         }
@@ -1325,7 +1328,8 @@ BW_API int bw_audit(bw_ctx *c, int r, double *out) {
 // -------------------------------------------------------------- host only
 BW_API int bw_host_initialize(const bw_params *p, double *pos_fl, double *vel_fl, double *pos_c, double *vel_c,
                               double *omega, double *ra, int *rng_out36) {
-    if (!p || p->L < 1) return BW_ERR_ARG;
+    if (!p || p->L < 1 || p->density < 1 || p->n_colloid < 0 || (long long)p->L * p->L * p->L * p->density > INT_MAX)
+        return BW_ERR_ARG;
     ReplicaHost R;
     R.p = *p;
     R.L = p->L;

@@ -17,6 +17,7 @@
 
 static void die(bw_ctx *c, int rc, const char *what) {
     fprintf(stderr, "%s failed (%d): %s\n", what, rc, c ? bw_last_error(c) : "");
+    if (c) bw_ctx_destroy(c);
     exit(1);
 }
 

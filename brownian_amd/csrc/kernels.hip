@@ -1311,7 +1311,12 @@ __global__ __launch_bounds__(COLLIDE_THREADS) void k_serial(Phys ph, Dev d, Flui
                 int idx = block_compact(p, s_wt, tot);
                 if (p && s_cand_n + idx < TCL_CAP) s_cand[s_cand_n + idx] = gg;
                 __syncthreads();
-                if (threadIdx.x == 0) s_cand_n = min(s_cand_n + tot, TCL_CAP);
+                if (threadIdx.x == 0) {
+                    // more candidate colloids than the list holds: the replay
+                    // would miss collisions, so the capacity error is raised
+                    if (s_cand_n + tot > TCL_CAP) atomicOr(d.status, BW_FLAG_LIST_OVERFLOW);
+                    s_cand_n = min(s_cand_n + tot, TCL_CAP);
+                }
                 __syncthreads();
             }
             if (threadIdx.x == 0) {

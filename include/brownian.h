@@ -84,6 +84,9 @@ typedef struct bw_params {
 typedef struct bw_ctx bw_ctx;
 
 /* ---------------------------------------------------------------- context */
+/* bw_ctx_create*: on an error after the context was allocated, *out still
+ * holds it so that bw_last_error can say what failed; the caller then
+ * releases it with bw_ctx_destroy (*out is NULL when nothing was allocated). */
 BW_API void bw_default_params(bw_params *p);
 BW_API int bw_ctx_create(const bw_params *p, bw_ctx **out);
 /* n_replica independent systems stepped together (one launch per kernel) */

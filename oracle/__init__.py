@@ -20,6 +20,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "lib", "liboracle.so")
+ORACLE_SO_O1 = os.path.join(HERE, "lib", "liboracle_O1.so")  # -O1, the reference Makefile's level
 REF_SO = os.path.join(HERE, "_ref", "libbrownian_ref.so")
 REF_SO_O1 = os.path.join(HERE, "_ref", "libbrownian_ref_O1.so")
 
@@ -62,15 +63,14 @@ def _ptr(a, t=_dp):
     return a.ctypes.data_as(t)
 
 
-_lib = None
+_libs = {}
 
 
-def _load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE_SO):
-            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run oracle.build())")
-        lib = C.CDLL(ORACLE_SO)
+def _load(so: str = ORACLE_SO):
+    if so not in _libs:
+        if not os.path.exists(so):
+            raise RuntimeError(f"oracle library missing: {so} (run oracle.build())")
+        lib = C.CDLL(so)
         lib.orc_create.restype = C.c_void_p
         lib.orc_create.argtypes = [C.POINTER(Params)]
         lib.orc_ran.restype = C.c_double
@@ -85,17 +85,17 @@ def _load():
                      "orc_nbox", "orc_drawlog_count"]:
             getattr(lib, name).argtypes = [C.c_void_p]
         lib.orc_rng_reset.argtypes = [C.c_void_p, C.c_int]
-        _lib = lib
-    return _lib
+        _libs[so] = lib
+    return _libs[so]
 
 
 class Oracle:
     """One oracle instance: the reference's global state, 0-based."""
 
-    def __init__(self, **params):
+    def __init__(self, so: str = ORACLE_SO, **params):
         p = reference_defaults(**params)
         self.params = p
-        self.lib = _load()
+        self.lib = _load(so)
         self._p = Params(**p)
         self.h = self.lib.orc_create(C.byref(self._p))
         self.L = p["L"]
@@ -111,7 +111,7 @@ class Oracle:
 
     def __getattr__(self, name):
         # entry points: o.rotation_mpcd(), o.step(), ...
-        fn = getattr(_load(), "orc_" + name, None)
+        fn = getattr(self.__dict__.get("lib") or _load(), "orc_" + name, None)
         if fn is None:
             raise AttributeError(name)
         return lambda: fn(self.h)
