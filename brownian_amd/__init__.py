@@ -356,6 +356,26 @@ class System:
                                              _ptr(mod), _ptr(wrap, _ip), _ptr(div, _ip)))
         return img, mod, wrap, div
 
+    def debug_philox(self, ctr, key):
+        """Device Philox4x32-10: ctr (n x 4), key (n x 2) uint32 -> n x 4."""
+        ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 4)
+        key = np.ascontiguousarray(key, np.uint32).reshape(-1, 2)
+        assert len(ctr) == len(key)
+        out = np.zeros_like(ctr)
+        up = C.POINTER(C.c_uint)
+        self._chk(self.lib.bw_debug_philox(self.h, len(ctr), _ptr(ctr, up), _ptr(key, up), _ptr(out, up)))
+        return out
+
+    def debug_reflections(self, n, step, substep, rf, rs):
+        """n device-drawn reflection tuples and the velocities they give at
+        particle position rf with surface vector rs."""
+        rf = np.ascontiguousarray(rf, np.float64)
+        rs = np.ascontiguousarray(rs, np.float64)
+        vals, vel = np.zeros((n, 6)), np.zeros((n, 3))
+        self._chk(self.lib.bw_debug_reflections(self.h, n, C.c_longlong(step), substep, _ptr(rf), _ptr(rs),
+                                                _ptr(vals), _ptr(vel)))
+        return vals, vel
+
     def debug_copy_ms(self, nbytes, reps=10):
         ms = C.c_double(0)
         self._chk(self.lib.bw_debug_copy(self.h, C.c_longlong(nbytes), reps, C.byref(ms)))

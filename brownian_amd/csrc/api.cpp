@@ -43,6 +43,9 @@ void launch_tile_colloids(const Phys &, const Dev &, hipStream_t, bool cleared =
 void launch_debug_geometry(int n, const double *a, double L, const int *ia, int Li, int b, double *img_out,
                            double *mod_out, int *wrap_out, int *div_out, hipStream_t st);
 void launch_copy(const void *, void *, long long, hipStream_t);
+void launch_debug_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st);
+void launch_debug_reflect(const Phys &ph, const Dev &d, int n, long long step, int substep, double L,
+                          const double *rf, const double *rs, double *vals6, double *vel3, hipStream_t st);
 void launch_features(const Phys &, const Dev &, const int *, hipStream_t);
 template <class Real> void launch_mark_phantoms(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
 hipError_t prepare_kernels(Phys &ph, bool fp64);
@@ -1508,6 +1511,38 @@ BW_API int bw_debug_geometry(bw_ctx *c, int n, const double *a, double L, const 
     HIPCHK(c, hipMemcpyAsync(mod_out, dmod, nd, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(wrap_out, dwrap, ni, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(div_out, ddiv, ni, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    hipFree(buf);
+    return BW_OK;
+}
+
+BW_API int bw_debug_philox(bw_ctx *c, int n, const unsigned *ctr, const unsigned *key, unsigned *out) {
+    if (!c || n < 1 || !ctr || !key || !out) return BW_ERR_ARG;
+    char *buf = nullptr;
+    const size_t b4 = 16 * (size_t)n, b2 = 8 * (size_t)n;
+    HIPCHK(c, hipMalloc((void **)&buf, 2 * b4 + b2));
+    uint4 *dc = (uint4 *)buf, *dout = (uint4 *)(buf + b4);
+    uint2 *dk = (uint2 *)(buf + 2 * b4);
+    HIPCHK(c, hipMemcpyAsync(dc, ctr, b4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(dk, key, b2, hipMemcpyHostToDevice, c->st));
+    launch_debug_philox(n, dc, dk, dout, c->st);
+    HIPCHK(c, hipMemcpyAsync(out, dout, b4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    hipFree(buf);
+    return BW_OK;
+}
+
+BW_API int bw_debug_reflections(bw_ctx *c, int n, long long step, int substep, const double *rf3, const double *rs3,
+                                double *vals6, double *vel3) {
+    if (!c || n < 1 || !rf3 || !rs3 || !vals6 || !vel3 || substep < 0 || substep > 255) return BW_ERR_ARG;
+    if (c->reps[0].nc < 1) return fail(c, BW_ERR_ARG, "bw_debug_reflections: replica 0 has no colloid");
+    Phys ph = c->ph;
+    ph.draws = BW_DRAWS_DEVICE;  // the device source, whatever the context's draws
+    double *buf = nullptr;
+    HIPCHK(c, hipMalloc((void **)&buf, sizeof(double) * 9 * (size_t)n));
+    launch_debug_reflect(ph, c->dv, n, step, substep, (double)c->reps[0].L, rf3, rs3, buf, buf + 6 * (size_t)n, c->st);
+    HIPCHK(c, hipMemcpyAsync(vals6, buf, sizeof(double) * 6 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(vel3, buf + 6 * (size_t)n, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     hipFree(buf);
     return BW_OK;

@@ -1762,6 +1762,28 @@ __global__ void k_debug_geometry(int n, const double *a, double L, const int *ia
     wrap_out[i] = wrapi(ia[i], Li);
     div_out[i] = (ia[i] >= 0 && ia[i] < 1024) ? div_small(ia[i], div_small_magic(b)) : -1;
 }
+// Test hooks for the device draw source: Philox4x32-10 on given (counter,
+// key) pairs, and device-drawn reflections of replica 0 applied at a fixed
+// surface point (raw tuple and the reflected velocity per particle id).
+__global__ void k_debug_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = philox(ctr[i], key[i]);
+}
+__global__ void k_debug_reflect(Phys ph, Dev d, int n, long long step, int substep, double L, double rfx,
+                                double rfy, double rfz, double rsx, double rsy, double rsz, double *vals6,
+                                double *vel3) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Rep R = d.rep[0];
+    ReflDraw dr;
+    draw_reflection(d, ph, R, step, substep, R.first_colloid, i, dr);
+    double ux, uy, uz;
+    stochastic_reflection(ph, L, rfx, rfy, rfz, rsx, rsy, rsz, dr, ux, uy, uz);
+    for (int k = 0; k < 6; k++) vals6[6 * i + k] = dr.v[k];
+    vel3[3 * i] = ux;
+    vel3[3 * i + 1] = uy;
+    vel3[3 * i + 2] = uz;
+}
 __global__ void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
@@ -1917,6 +1939,14 @@ template void launch_mark_phantoms<double>(const Phys &, const Dev &, const Flui
 void launch_debug_geometry(int n, const double *a, double L, const int *ia, int Li, int b, double *img_out,
                            double *mod_out, int *wrap_out, int *div_out, hipStream_t st) {
     LAUNCH(k_debug_geometry, (n + 255) / 256, 256, 0, st, n, a, L, ia, Li, b, img_out, mod_out, wrap_out, div_out);
+}
+void launch_debug_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st) {
+    LAUNCH(k_debug_philox, (n + 255) / 256, 256, 0, st, n, ctr, key, out);
+}
+void launch_debug_reflect(const Phys &ph, const Dev &d, int n, long long step, int substep, double L,
+                          const double *rf, const double *rs, double *vals6, double *vel3, hipStream_t st) {
+    LAUNCH(k_debug_reflect, (n + 255) / 256, 256, 0, st, ph, d, n, step, substep, L, rf[0], rf[1], rf[2], rs[0],
+           rs[1], rs[2], vals6, vel3);
 }
 
 void launch_copy(const void *a, void *b, long long nbytes, hipStream_t st) {

@@ -218,6 +218,17 @@ BW_API int bw_debug_copy(bw_ctx *ctx, long long nbytes, int reps, double *ms);
 BW_API int bw_debug_geometry(bw_ctx *ctx, int n, const double *a, double L, const int *ia, int Li, int b,
                              double *img_out, double *mod_out, int *wrap_out, int *div_out);
 
+/* Test hook: the device draw source's Philox4x32-10 (Salmon et al., SC'11) on
+ * n (counter, key) pairs: ctr n x 4 words, key n x 2 words -> out n x 4. */
+BW_API int bw_debug_philox(bw_ctx *ctx, int n, const unsigned *ctr, const unsigned *key, unsigned *out);
+/* Test hook: n device-drawn reflection tuples of replica 0 -- step `step`,
+ * substep `substep`, its first colloid, particle ids 0..n-1 -- and the
+ * stochastic reflection (cpp/fluid_colloid_collision.cpp:7-24) each gives at
+ * particle position rf3 with surface vector rs3: vals6 n x 6 raw uniforms
+ * (U_e, t.x, t.y, t.z, a, b), vel3 n x 3 velocities. */
+BW_API int bw_debug_reflections(bw_ctx *ctx, int n, long long step, int substep, const double *rf3,
+                                const double *rs3, double *vals6, double *vel3);
+
 /* ----------------------------------------------------------- observables */
 /* No counterpart in the reference (it computes energies and prints nothing,
  * cpp/active_brownian.cpp:39-55).  bw_observables_reset sets the time origin

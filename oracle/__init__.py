@@ -178,6 +178,12 @@ class Oracle:
         self.lib.orc_get_fluid_no(C.c_void_p(self.h), _ptr(o, _ip))
         return o
 
+    def cell_no(self) -> np.ndarray:
+        """Per-cell counts of the last shifted binning (rotation_mpcd)."""
+        o = np.zeros(self.cells, np.int32)
+        self.lib.orc_get_cell_no(C.c_void_p(self.h), _ptr(o, _ip))
+        return o
+
     def cell_lists(self, which: int) -> np.ndarray:
         o = np.zeros(self.nf, np.int32)
         self.lib.orc_get_cell_lists(C.c_void_p(self.h), which, _ptr(o, _ip))

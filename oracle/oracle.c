@@ -659,6 +659,7 @@ void orc_get_box_neigh(const orc_state *s, int *out) {
         for (int k = 0; k < s->nbox; k++) out[(size_t)c * s->nbox + k] = s->box_neigh[(size_t)c * 343 + k];
 }
 void orc_get_fluid_no(const orc_state *s, int *out) { memcpy(out, s->fluid_no, sizeof(int) * (size_t)s->cells); }
+void orc_get_cell_no(const orc_state *s, int *out) { memcpy(out, s->cell_no, sizeof(int) * (size_t)s->cells); }
 int orc_get_cell_lists(const orc_state *s, int which, int *out) {
     const int *lst = which ? s->cell_list : s->box_list;
     memcpy(out, lst, sizeof(int) * (size_t)s->nf);

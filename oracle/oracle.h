@@ -88,6 +88,7 @@ void orc_substep(orc_state *s);
 int orc_nbox(const orc_state *s);
 void orc_get_box_neigh(const orc_state *s, int *out);          /* cells x nbox, 0-based */
 void orc_get_fluid_no(const orc_state *s, int *out);           /* L^3, last binning     */
+void orc_get_cell_no(const orc_state *s, int *out);            /* L^3, shifted binning  */
 int orc_get_cell_lists(const orc_state *s, int which, int *out);/* 0 box_part, 1 cell_part */
 int orc_get_neigh_fl(const orc_state *s, int *counts, int *out);
 int orc_get_md_list(const orc_state *s, int *counts, int *out);

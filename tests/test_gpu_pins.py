@@ -1,0 +1,145 @@
+"""GPU pins that the golden vectors cannot give, all through the C-ABI:
+
+* the device draw source: Philox4x32-10 against the known-answer vectors
+  published with Random123 (Salmon et al., SC'11), and device-drawn
+  reflections against the distributions cpp/fluid_colloid_collision.cpp:7-24
+  defines (Rayleigh normal speed, Gaussian tangential speed, tangent
+  perpendicular to the normal);
+* the SRD rotation at 90 degrees (the reference hard-codes 130,
+  cpp/rotation_mpcd.cpp:19): per cell an orthonormal, right-handed rotation by
+  theta about the drawn axis, and the thermostat's relative kinetic energy
+  3 (N - 1) kT / 2 (cpp/rotation_mpcd.cpp:33-49);
+* the driver CLI with the reference's argv (cpp/active_brownian.cpp:9-60).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import brownian_amd as B
+from tests.test_oracle_pins import rotation_axes_check, rotation_invariants
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+# Random123 kat_vectors, philox4x32 with 10 rounds: counter, key -> output
+PHILOX_KAT = [
+    ((0x00000000, 0x00000000, 0x00000000, 0x00000000), (0x00000000, 0x00000000),
+     (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff), (0xffffffff, 0xffffffff),
+     (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+def _small(**kw):
+    prm = dict(L=8, density=2, n_colloid=1, sig_colloid=4.2, sigma=4.0, mass_colloid=50.0, n_sub=10,
+               theta_deg=90.0, seed=99, precision=B.BW_FP64, draws=B.BW_DRAWS_DEVICE)
+    prm.update(kw)
+    return B.System(**prm)
+
+
+def test_device_philox_known_answers():
+    s = _small()
+    ctr = np.array([c for c, _, _ in PHILOX_KAT], np.uint32)
+    key = np.array([k for _, k, _ in PHILOX_KAT], np.uint32)
+    out = s.debug_philox(ctr, key)
+    np.testing.assert_array_equal(out, np.array([o for _, _, o in PHILOX_KAT], np.uint32))
+
+
+@pytest.mark.parametrize("kbt,mass_fl", [(1.0, 1.0), (2.0, 0.5)])
+def test_device_reflection_distributions(kbt, mass_fl):
+    """n device draws reflected at one surface point: the normal speed is
+    Rayleigh with sigma^2 = kT/m (mean sigma sqrt(pi/2), second moment
+    2 sigma^2), the tangential speed Gaussian with variance sigma^2 (fourth
+    moment 3 sigma^4) in a direction perpendicular to the normal and uniform
+    around it; the raw uniforms are U(0,1) and the accepted polar pair lies in
+    the unit disc."""
+    from scipy import stats
+    s = _small(kbt=kbt, mass_fl=mass_fl)
+    n = 200000
+    rf = np.array([2.3, 4.1, 5.7])
+    rs = np.array([0.6, -1.2, 1.5])
+    vals, vel = s.debug_reflections(n, step=17, substep=3, rf=rf, rs=rs)
+    sig = np.sqrt(kbt / mass_fl)
+    nhat = rs / np.linalg.norm(rs)
+    un = vel @ nhat
+    assert np.all(un > 0)
+    se = lambda x: x.std() / np.sqrt(n)
+    assert abs(un.mean() - sig * np.sqrt(np.pi / 2)) < 5 * se(un)
+    assert abs((un ** 2).mean() - 2 * sig ** 2) < 5 * se(un ** 2)
+    # Rayleigh CDF 1 - exp(-u^2 / (2 sigma^2))
+    assert stats.kstest(un, lambda u: 1 - np.exp(-u ** 2 / (2 * sig ** 2))).pvalue > 1e-4
+    wt = vel - un[:, None] * nhat
+    wn = np.linalg.norm(wt, axis=1)
+    assert abs((wn ** 2).mean() - sig ** 2) < 5 * se(wn ** 2)
+    assert abs((wn ** 4).mean() - 3 * sig ** 4) < 5 * se(wn ** 4)
+    # |N(0, sigma^2)| has CDF erf(w / (sigma sqrt 2))
+    from scipy.special import erf
+    assert stats.kstest(wn, lambda w: erf(w / (sig * np.sqrt(2)))).pvalue > 1e-4
+    d = wt / wn[:, None]
+    assert np.abs(d @ nhat).max() < 1e-12
+    assert np.abs(d.mean(0)).max() < 5 / np.sqrt(n)
+    # raw uniforms
+    for k in range(4):
+        assert vals[:, k].min() >= 0 and vals[:, k].max() < 1
+        assert stats.kstest(vals[:, k], "uniform").pvalue > 1e-4
+    x1, x2 = 2 * vals[:, 4] - 1, 2 * vals[:, 5] - 1
+    z = x1 ** 2 + x2 ** 2
+    assert np.all((z > 0) & (z <= 1))
+    # particles (draw keys) are independent
+    assert abs(np.corrcoef(vals[:-1, 0], vals[1:, 0])[0, 1]) < 5 / np.sqrt(n)
+    # another substep draws other numbers
+    vals2, _ = s.debug_reflections(1000, step=17, substep=4, rf=rf, rs=rs)
+    assert not np.any(vals2[:, 0] == vals[:1000, 0])
+
+
+@pytest.mark.parametrize("precision,tol", [(B.BW_FP64, 1e-10), (B.BW_FP32, 2e-5)])
+def test_rotation_90_degree_invariants(precision, tol):
+    """rotation_mpcd at 90 degrees with injected axes: per shifted cell the map
+    v - u -> v' - u is s R with R orthonormal, det +1, angle 90 degrees,
+    right-handed about the injected axis, and the thermostat target."""
+    L = 10
+    s = B.System(L=L, density=10, n_colloid=1, sig_colloid=4.2, sigma=4.0, mass_colloid=50.0, n_sub=10,
+                 theta_deg=90.0, seed=5, precision=precision, draws=B.BW_DRAWS_INJECTED,
+                 vel_law=B.BW_VEL_MAXWELL)
+    s.startup()
+    rng = np.random.default_rng(8)
+    shift = rng.uniform(0, 1, 3)
+    axes = rng.uniform(0, 1, (L ** 3, 2))
+    _, vel0, _ = s.download_fluid()
+    s.inject_rotation(shift, axes)
+    s.rotation_mpcd()
+    assert s.status() == 0
+    _, vel1, _ = s.download_fluid()
+    cells = s.shifted_cells().astype(np.int64)
+    w = rotation_invariants(vel0, vel1, cells, 90.0)
+    assert w["cells"] > 800
+    for k in ("orth", "det", "angle", "energy"):
+        assert w[k] < tol, (k, w)
+    assert w["mean"] < tol
+    assert rotation_axes_check(vel0, vel1, cells, axes, 90.0) < 10 * tol
+
+
+def test_driver_cli_reference_argv():
+    """active_brownian <density> <no_of_colloid>: the reference's arguments,
+    a zero exit and the elapsed seconds on the last line (cpp/active_brownian.cpp:57-58)."""
+    exe = os.path.join(ROOT, "brownian_amd", "bin", "active_brownian")
+    assert os.path.exists(exe), "driver not built"
+    p = subprocess.run([exe, "5", "2", "--L", "12", "--niter", "4", "--nsub", "5"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    last = p.stdout.strip().splitlines()[-1]
+    assert float(last) >= 0.0
+    p = subprocess.run([exe, "5", "2", "--L", "12", "--niter", "4", "--nsub", "5", "--audit-every", "2",
+                        "--theta", "90", "--v0", "0.02"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    lines = p.stdout.strip().splitlines()
+    audits = [l for l in lines if l.startswith("step ")]
+    assert len(audits) == 2 and audits[-1].startswith("step 4 ")
+    float(lines[-1])
+    # a bad argument is an error, not a crash
+    p = subprocess.run([exe, "5", "2", "--L", "5"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 1 and "bw_ctx_create" in p.stderr
