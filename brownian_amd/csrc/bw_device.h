@@ -255,6 +255,17 @@ __device__ __forceinline__ bool in_unit(float o) { return __float_as_uint(o) < 0
 __device__ __forceinline__ bool in_unit(double o) {
     return (unsigned long long)__double_as_longlong(o) < 0x3FF0000000000000ull;
 }
+// true only if all three lie in [0,1): the OR of the bit patterns is at least
+// each of them, so any value outside (sign bit set, or >= 1.0) makes it false;
+// it may also be false for three values inside, where the caller's
+// normalise() is the identity
+__device__ __forceinline__ bool in_unit3(float a, float b, float c) {
+    return (__float_as_uint(a) | __float_as_uint(b) | __float_as_uint(c)) < 0x3F800000u;
+}
+__device__ __forceinline__ bool in_unit3(double a, double b, double c) {
+    return ((unsigned long long)__double_as_longlong(a) | (unsigned long long)__double_as_longlong(b) |
+            (unsigned long long)__double_as_longlong(c)) < 0x3FF0000000000000ull;
+}
 
 
 // ------------------------------------------------------------ reflection

@@ -300,6 +300,13 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
 // shell -- is streamed through all n_sub substeps here, in registers, with
 // the arithmetic of the streaming kernel.  The others ("near") are stored
 // first in the tile, and only they are touched by the per-substep kernels.
+// fixed-point (2^-40) difference a - b of two velocity components, exact for
+// |a|, |b| < 1024 (larger ones are flagged by the caller and count as 0)
+__device__ __forceinline__ unsigned long long fixq_diff(float a, float b) {
+    const long long qa = fabsf(a) < 1024.0f ? __double2ll_rn((double)a * 0x1p40) : 0;
+    const long long qb = fabsf(b) < 1024.0f ? __double2ll_rn((double)b * 0x1p40) : 0;
+    return (unsigned long long)(qa - qb);
+}
 // bytes of the SRD pass's per-cell region (7 doubles per cell), which also
 // holds the cap ids of the sort
 __host__ __device__ inline size_t srd_cell_region(int ncell, int cap) {
@@ -312,6 +319,9 @@ __host__ __device__ inline size_t srd_cell_region(int ncell, int cap) {
 template <class Real, bool COUNT_SHELL, bool MAP, bool BLOCK, int TH, bool WRAP>
 __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Fluid<Real> dst) {
     constexpr bool NEAR_TEST = COUNT_SHELL && BLOCK;
+    // FP32 storage: per-cell sums by exact fixed-point LDS atomics (no sort);
+    // FP64 storage: the reference's id-ordered sums (bit parity)
+    constexpr bool FAST = sizeof(Real) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_pre[28], s_src[27], s_base[27];
     __shared__ int s_wtot[16];
@@ -341,6 +351,15 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     int *s_start = (int *)(s_v + 3 * cap);              // ncell + 1
     int *s_fill = s_start + ncell + 1;                  // ncell
     unsigned short *s_ord = (unsigned short *)(s_fill + ncell);  // cap: cell-sorted, id-ranked
+    // FAST carve-up (the same buffer): count, fixed-point sums (momentum 3,
+    // energy 1), the momentum change of the rounded collision (3) and the
+    // smallest particle id
+    int *f_cnt = (int *)smem;                                                      // ncell
+    int *f_minpid = f_cnt + ncell;                                                 // ncell
+    unsigned long long *f_sum = (unsigned long long *)(smem + 8 * (size_t)ncell); // 4 ncell
+    unsigned long long *f_dp = f_sum + 4 * ncell;                                  // 3 ncell
+    float *f_u = (float *)(f_dp + 3 * ncell);                                      // 3 ncell
+    float *f_m = f_u + 3 * ncell;                                                  // 9 ncell
 
     const long long step = *d.step;
     const double sx = d.shift[3 * tg.r], sy = d.shift[3 * tg.r + 1], sz = d.shift[3 * tg.r + 2];
@@ -354,7 +373,16 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         s_base[t] = d.dep_off[S * 28 + t];
         s_pre[t] = d.dep_off[S * 28 + t + 1] - d.dep_off[S * 28 + t];
     }
-    for (int c = threadIdx.x; c < ncell + 1; c += blockDim.x) s_start[c] = 0;
+    if constexpr (FAST) {
+        for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+            f_cnt[c] = 0;
+            f_minpid[c] = 0x7fffffff;
+            f_sum[4 * c] = 0; f_sum[4 * c + 1] = 0; f_sum[4 * c + 2] = 0; f_sum[4 * c + 3] = 0;
+            f_dp[3 * c] = 0; f_dp[3 * c + 1] = 0; f_dp[3 * c + 2] = 0;
+        }
+    } else {
+        for (int c = threadIdx.x; c < ncell + 1; c += blockDim.x) s_start[c] = 0;
+    }
     if (COUNT_SHELL)
         for (int k = threadIdx.x; k < TCL_CAP; k += blockDim.x) s_shell[k] = 0;
     // the tile's colloid list, requested now so that it arrives with the
@@ -456,12 +484,76 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             }
             int key = lx + B0 * (ly + B1 * lz);
             rkey[k] = key;
-            s_v[g] = rvx[k]; s_v[cap + g] = rvy[k]; s_v[2 * cap + g] = rvz[k];
-            atomicAdd(&s_start[key], 1);
+            if constexpr (FAST) {
+                // exact sums: velocities in units of 2^-40, |v|^2 in units of
+                // 2^-28 (each particle's terms rounded once, the same whatever
+                // the order of the additions; with |v| < 1024 per component and
+                // at most 4096 particles a cell's sums stay inside int64).  A
+                // component of 1024 or more (or NaN) is a particle that moves
+                // over a hundred cells per step, past its tile: flagged
+                const float vx = (float)rvx[k], vy = (float)rvy[k], vz = (float)rvz[k];
+                const bool ok = fabsf(vx) < 1024.0f && fabsf(vy) < 1024.0f && fabsf(vz) < 1024.0f;
+                if (!ok) drift = true;
+                const double dx = ok ? (double)vx : 0.0, dy = ok ? (double)vy : 0.0, dz = ok ? (double)vz : 0.0;
+                const double e = (dx * dx + dy * dy) + dz * dz;
+                atomicAdd(&f_cnt[key], 1);
+                atomicMin(&f_minpid[key], rpid[k]);
+                atomicAdd(&f_sum[4 * key], (unsigned long long)__double2ll_rn(dx * 0x1p40));
+                atomicAdd(&f_sum[4 * key + 1], (unsigned long long)__double2ll_rn(dy * 0x1p40));
+                atomicAdd(&f_sum[4 * key + 2], (unsigned long long)__double2ll_rn(dz * 0x1p40));
+                atomicAdd(&f_sum[4 * key + 3], (unsigned long long)__double2ll_rn(e * 0x1p28));
+            } else {
+                s_v[g] = rvx[k]; s_v[cap + g] = rvy[k]; s_v[2 * cap + g] = rvz[k];
+                atomicAdd(&s_start[key], 1);
+            }
         }
     }
     if (drift) atomicOr(d.status, BW_FLAG_DRIFT);
     __syncthreads();
+    bool zero_var = false;
+    if constexpr (FAST) {
+        // per cell (cpp/rotation_mpcd.cpp:14-49): mean u = sum v / N and the
+        // relative kinetic energy sum |v - u|^2 = sum |v|^2 - N |u|^2 (which the
+        // rotation leaves unchanged) in FP64 from the exact sums; the random
+        // axis, the rotation by theta about it and the thermostat factor,
+        // stored as one FP32 matrix M = scale R with u
+        bool missing = false;
+        const float ct = (float)ph.ct, st = (float)ph.st, ict = (float)ph.ict;
+        for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+            const int N = f_cnt[c];
+            if (N < 2) continue;
+            const double sx = (double)(long long)f_sum[4 * c] * 0x1p-40;
+            const double sy = (double)(long long)f_sum[4 * c + 1] * 0x1p-40;
+            const double sz = (double)(long long)f_sum[4 * c + 2] * 0x1p-40;
+            const double E = (double)(long long)f_sum[4 * c + 3] * 0x1p-28;
+            const double ux = sx / N, uy = sy / N, uz = sz / N;
+            const double var = E - ((sx * ux + sy * uy) + sz * uz);
+            double scale = sqrt(3.0 * (N - 1) * ph.kbt / (ph.mass_fl * var));
+            if (!(var > 0)) {
+                zero_var = true;
+                scale = 1.0;
+            }
+            const int kx = ox0 + c % B0, ky = oy0 + (c / B0) % B1, kz = oz0 + c / (B0 * B1);
+            double u1, u2;
+            if (!draw_axis_raw(d, ph, R, step, linear_cell(kx, ky, kz, L), u1, u2)) {
+                missing = true;
+                u1 = 0.5; u2 = 0.0;
+            }
+            // cpp/rotation_mpcd.cpp:19-25: axis (rho = 2U - 1, phi = 2 pi U) and Rodrigues
+            const float rho = (float)(2.0 * u1 - 1.0);
+            float sp, cp;
+            sincosf((float)(6.283185307179586 * u2), &sp, &cp);
+            const float sq = sqrtf(fmaxf(0.0f, 1.0f - rho * rho));
+            const float r1 = cp * sq, r2 = sp * sq, r3 = rho;
+            const float ir1 = ict * r1, ir2 = ict * r2, ir3 = ict * r3, sc = (float)scale;
+            float *m = f_m + 9 * c;
+            m[0] = sc * (ir1 * r1 + ct); m[1] = sc * (ir1 * r2 - st * r3); m[2] = sc * (ir1 * r3 + st * r2);
+            m[3] = sc * (ir2 * r1 + st * r3); m[4] = sc * (ir2 * r2 + ct); m[5] = sc * (ir2 * r3 - st * r1);
+            m[6] = sc * (ir3 * r1 - st * r2); m[7] = sc * (ir3 * r2 + st * r1); m[8] = sc * (ir3 * r3 + ct);
+            f_u[3 * c] = (float)ux; f_u[3 * c + 1] = (float)uy; f_u[3 * c + 2] = (float)uz;
+        }
+        if (missing) atomicOr(d.status, BW_FLAG_DRAW_MISSING);
+    } else {
     block_exclusive_scan(s_start, ncell, s_wtot);
     if (threadIdx.x == 0) s_start[ncell] = n;
     for (int c = threadIdx.x; c < ncell; c += blockDim.x) s_fill[c] = s_start[c];
@@ -540,7 +632,6 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     }
     __syncthreads();
     // per cell: variance in id order, thermostat factor
-    bool zero_var = false;
     for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
         const int b = s_start[c], e = s_start[c + 1], N = e - b;
         if (N > 1) {
@@ -555,6 +646,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             s_sc[c] = scale;
         }
     }
+    }  // FP64 path
     if (zero_var) atomicOr(d.status, BW_FLAG_ZERO_VARIANCE);
     if (ph.moments) {
         // coarse-cell moments (2x2x2 shifted cells) for the surrogate, from the
@@ -567,9 +659,16 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             double n = 0, px = 0, py = 0, pz = 0, en = 0;
             for (int o = 0; o < 8; o++) {
                 const int c = (2 * qx + (o & 1)) + B0 * ((2 * qy + ((o >> 1) & 1)) + B1 * (2 * qz + (o >> 2)));
-                const int N = s_start[c + 1] - s_start[c];
+                const int N = FAST ? f_cnt[c] : s_start[c + 1] - s_start[c];
                 if (N < 1) continue;
-                const double ux = s_u[3 * c], uy = s_u[3 * c + 1], uz = s_u[3 * c + 2];
+                double ux, uy, uz;
+                if constexpr (FAST) {
+                    ux = (double)(long long)f_sum[4 * c] * 0x1p-40 / N;
+                    uy = (double)(long long)f_sum[4 * c + 1] * 0x1p-40 / N;
+                    uz = (double)(long long)f_sum[4 * c + 2] * 0x1p-40 / N;
+                } else {
+                    ux = s_u[3 * c]; uy = s_u[3 * c + 1]; uz = s_u[3 * c + 2];
+                }
                 n += N;
                 px += N * ux; py += N * uy; pz += N * uz;
                 en += N * ((ux * ux + uy * uy) + uz * uz) + (N > 1 ? 3.0 * (N - 1) * ph.kbt / ph.mass_fl : 0.0);
@@ -588,8 +687,23 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         int g = threadIdx.x + k * TH;
         if (g < n) {
             int c = rkey[k];
-            int N = s_start[c + 1] - s_start[c];
-            if (N > 1) {
+            int N = FAST ? f_cnt[c] : s_start[c + 1] - s_start[c];
+            if (FAST && N > 1) {
+                const float *m = f_m + 9 * c;
+                const float ux = f_u[3 * c], uy = f_u[3 * c + 1], uz = f_u[3 * c + 2];
+                const float vx = (float)rvx[k], vy = (float)rvy[k], vz = (float)rvz[k];
+                const float dx = vx - ux, dy = vy - uy, dz = vz - uz;
+                const float nx = ux + ((m[0] * dx + m[1] * dy) + m[2] * dz);
+                const float ny = uy + ((m[3] * dx + m[4] * dy) + m[5] * dz);
+                const float nz = uz + ((m[6] * dx + m[7] * dy) + m[8] * dz);
+                // the particle's exact momentum change; the cell's sum, which the
+                // collision makes zero up to the roundings, is taken back from
+                // the cell's smallest id below
+                atomicAdd(&f_dp[3 * c], fixq_diff(nx, vx));
+                atomicAdd(&f_dp[3 * c + 1], fixq_diff(ny, vy));
+                atomicAdd(&f_dp[3 * c + 2], fixq_diff(nz, vz));
+                rvx[k] = (Real)nx; rvy[k] = (Real)ny; rvz[k] = (Real)nz;
+            } else if (N > 1) {
                 double ux = s_u[3 * c], uy = s_u[3 * c + 1], uz = s_u[3 * c + 2];
                 double r1 = s_ax[3 * c], r2 = s_ax[3 * c + 1], r3 = s_ax[3 * c + 2];
                 double ir1 = ph.ict * r1, ir2 = ph.ict * r2, ir3 = ph.ict * r3;
@@ -722,6 +836,24 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         }
         n_near = base;
     }
+    if constexpr (FAST) {
+        // per cell: the rounded collision's momentum residual (exact) is taken
+        // from the particle of smallest id, so that the cell's momentum
+        // changes by that particle's one rounding only (cpp/rotation_mpcd.cpp
+        // conserves it exactly)
+        if (!BLOCK) __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SRD_PT; k++) {
+            if (threadIdx.x + k * TH < n) {
+                const int c = rkey[k];
+                if (f_cnt[c] > 1 && rpid[k] == f_minpid[c]) {
+                    rvx[k] = (Real)((double)rvx[k] - (double)(long long)f_dp[3 * c] * 0x1p-40);
+                    rvy[k] = (Real)((double)rvy[k] - (double)(long long)f_dp[3 * c + 1] * 0x1p-40);
+                    rvz[k] = (Real)((double)rvz[k] - (double)(long long)f_dp[3 * c + 2] * 0x1p-40);
+                }
+            }
+        }
+    }
     // The near prefix is published in chunks of 64 for the per-substep pass;
     // the chunk range is requested here and written after the far particles'
     // streaming, which hides the atomic's round trip.
@@ -748,7 +880,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                     ox = ox + dx;
                     oy = oy + dy;
                     oz = oz + dz;
-                    if (!(in_unit(ox) && in_unit(oy) && in_unit(oz))) {
+                    if (!in_unit3(ox, oy, oz)) {
                         normalise_nowrap(ix, ox);
                         normalise_nowrap(iy, oy);
                         normalise_nowrap(iz, oz);
@@ -1794,8 +1926,9 @@ __global__ void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, lon
 #define LAUNCH(kern, grid, block, lds, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, st, __VA_ARGS__)
 
 size_t srd_lds_bytes(int ncell, int cap, bool fp64) {
-    size_t rb = fp64 ? 8 : 4;
-    return srd_cell_region(ncell, cap) + 3 * (size_t)cap * rb + (size_t)(2 * ncell + 1) * 4 + (size_t)cap * 2 + 16;
+    if (!fp64)  // count, smallest id, 4 + 3 fixed-point sums, mean (3) and matrix (9) per cell
+        return (size_t)ncell * (8 + 56 + 48) + 16;
+    return srd_cell_region(ncell, cap) + 3 * (size_t)cap * 8 + (size_t)(2 * ncell + 1) * 4 + (size_t)cap * 2 + 16;
 }
 // static LDS of the SRD kernel's largest variant (colloid lists, masks)
 size_t srd_static_lds() { return sizeof(TileColl) + sizeof(NearColl) + 4 * TCL_CAP + 2 * sizeof(AxisMask) + 512; }

@@ -79,8 +79,9 @@ def test_device_reflection_distributions(kbt, mass_fl):
     # |N(0, sigma^2)| has CDF erf(w / (sigma sqrt 2))
     from scipy.special import erf
     assert stats.kstest(wn, lambda w: erf(w / (sig * np.sqrt(2)))).pvalue > 1e-4
+    # the tangential part lies in the plane normal to rs (to rounding of |v|)
+    assert np.all(np.abs(wt @ nhat) <= 1e-13 * np.linalg.norm(vel, axis=1))
     d = wt / wn[:, None]
-    assert np.abs(d @ nhat).max() < 1e-12
     assert np.abs(d.mean(0)).max() < 5 / np.sqrt(n)
     # raw uniforms
     for k in range(4):
