@@ -120,7 +120,7 @@ class System:
     ENTRY = ["initialize", "initialize_colloid", "initialize_fluid", "create_box", "neighbour_list_mpcd",
              "neighbour_list_md", "compute_force_md", "tumble", "rotation_mpcd", "run", "update_pos_md",
              "update_pos_mpcd", "fluid_colloid_collision", "update_activity_direction",
-             "update_velocity_colloid", "bw_copy_old_force", "bw_startup"]
+             "update_velocity_colloid", "bw_copy_old_force", "bw_startup", "updown_velocity"]
 
     def __init__(self, replicas=None, **params):
         self.lib = lib()
@@ -295,6 +295,14 @@ class System:
         o = np.zeros(self.info[replica]["nf"], np.int32)
         self._chk(self.lib.bw_get_shifted_cells(self.h, replica, _ptr(o, _ip)))
         return o
+
+    def updown(self, replica=0) -> dict:
+        """Results of the last updown_velocity (cpp/run.cpp:28-50)."""
+        n = self.info[replica]["nc"]
+        cnt, up = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+        vel, upv = np.zeros((max(n, 1), 3)), np.zeros((max(n, 1), 3))
+        self._chk(self.lib.bw_get_updown(self.h, replica, _ptr(cnt, _ip), _ptr(up, _ip), _ptr(vel), _ptr(upv)))
+        return dict(cnt=cnt[:n], up_cnt=up[:n], vel=vel[:n], up_vel=upv[:n])
 
     def run_counts(self, replica=0):
         o = np.zeros(max(self.info[replica]["nc"], 1), np.int32)

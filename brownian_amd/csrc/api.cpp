@@ -44,6 +44,8 @@ void launch_debug_geometry(int n, const double *a, double L, const int *ia, int 
                            double *mod_out, int *wrap_out, int *div_out, hipStream_t st);
 void launch_copy(const void *, void *, long long, hipStream_t);
 void launch_debug_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st);
+void launch_updown(const Phys &ph, const Dev &d, int n_rep, const double *pos, const double *vel, double *sums,
+                   int *cnt, int *up_cnt, double *v, double *up, hipStream_t st);
 void launch_debug_reflect(const Phys &ph, const Dev &d, int n, long long step, int substep, double L,
                           const double *rf, const double *rs, double *vals6, double *vel3, hipStream_t st);
 void launch_features(const Phys &, const Dev &, const int *, hipStream_t);
@@ -108,6 +110,9 @@ struct bw_ctx {
     size_t srd_lds = 0;
     bool maps_alloc = false;
     double *stage_pos = nullptr, *stage_vel = nullptr;
+    // updown_velocity results: per-colloid sums, counts and velocities
+    double *ud_sums = nullptr, *ud_vel = nullptr, *ud_up = nullptr;
+    int *ud_cnt = nullptr, *ud_upcnt = nullptr;
     int *stage_cell = nullptr;
     bool inj_dirty = false;
     bool moments_ok = false;        // every replica has an even edge and even tile edges
@@ -1028,6 +1033,42 @@ BW_API int run(bw_ctx *c) {
         launch_run_count<float>(c->ph, d, c->fl[c->cur].f32, c->st);
         launch_stream<float>(c->ph, d, c->fl[c->cur].f32, true, false, false, false, c->st);
     }
+    return BW_OK;
+}
+
+// cpp/run.cpp:28-50 (the reference computes it and discards the results);
+// needs the neigh_fl lists of neighbour_list_mpcd
+BW_API int updown_velocity(bw_ctx *c) {
+    if (!c) return BW_ERR_ARG;
+    if (!c->maps_alloc) return fail(c, BW_ERR_STATE, "updown_velocity: call neighbour_list_mpcd first");
+    int rc;
+    const size_t NC = std::max(c->nc_total, 1);
+    if (!c->stage_pos && ((rc = dalloc(c, &c->stage_pos, 3 * (size_t)c->nf_total)) ||
+                          (rc = dalloc(c, &c->stage_vel, 3 * (size_t)c->nf_total)) ||
+                          (rc = dalloc(c, &c->stage_cell, c->nf_total))))
+        return rc;
+    if (!c->ud_sums && ((rc = dalloc(c, &c->ud_sums, 8 * NC)) || (rc = dalloc(c, &c->ud_vel, 3 * NC)) ||
+                        (rc = dalloc(c, &c->ud_up, 3 * NC)) || (rc = dalloc(c, &c->ud_cnt, NC)) ||
+                        (rc = dalloc(c, &c->ud_upcnt, NC))))
+        return rc;
+    Dev d = dev_for(c, c->cur);
+    if (c->fp64) launch_gather<double>(c->ph, d, c->fl[c->cur].f64, c->stage_pos, c->stage_vel, c->stage_cell, c->st);
+    else launch_gather<float>(c->ph, d, c->fl[c->cur].f32, c->stage_pos, c->stage_vel, c->stage_cell, c->st);
+    launch_updown(c->ph, d, c->n_rep, c->stage_pos, c->stage_vel, c->ud_sums, c->ud_cnt, c->ud_upcnt, c->ud_vel,
+                  c->ud_up, c->st);
+    return BW_OK;
+}
+
+BW_API int bw_get_updown(bw_ctx *c, int r, int *cnt, int *up_cnt, double *vel, double *up_vel) {
+    ReplicaHost *R = rep_of(c, r);
+    if (!R) return BW_ERR_ARG;
+    if (!c->ud_sums) return fail(c, BW_ERR_STATE, "bw_get_updown: updown_velocity not run");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const size_t g0 = (size_t)R->first_colloid, n = (size_t)R->nc;
+    if (cnt) HIPCHK(c, hipMemcpy(cnt, c->ud_cnt + g0, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (up_cnt) HIPCHK(c, hipMemcpy(up_cnt, c->ud_upcnt + g0, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (vel) HIPCHK(c, hipMemcpy(vel, c->ud_vel + 3 * g0, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    if (up_vel) HIPCHK(c, hipMemcpy(up_vel, c->ud_up + 3 * g0, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
     return BW_OK;
 }
 

@@ -351,15 +351,17 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     int *s_start = (int *)(s_v + 3 * cap);              // ncell + 1
     int *s_fill = s_start + ncell + 1;                  // ncell
     unsigned short *s_ord = (unsigned short *)(s_fill + ncell);  // cap: cell-sorted, id-ranked
-    // FAST carve-up (the same buffer): count, fixed-point sums (momentum 3,
-    // energy 1), the momentum change of the rounded collision (3) and the
-    // smallest particle id
-    int *f_cnt = (int *)smem;                                                      // ncell
-    int *f_minpid = f_cnt + ncell;                                                 // ncell
-    unsigned long long *f_sum = (unsigned long long *)(smem + 8 * (size_t)ncell); // 4 ncell
-    unsigned long long *f_dp = f_sum + 4 * ncell;                                  // 3 ncell
-    float *f_u = (float *)(f_dp + 3 * ncell);                                      // 3 ncell
-    float *f_m = f_u + 3 * ncell;                                                  // 9 ncell
+    // FAST carve-up (the same buffer): count, smallest particle id, scale
+    // exponent of the relative-energy sum; fixed-point sums (momentum 3,
+    // |v|^2 1, |v - u|^2 1) and the momentum change of the rounded collision
+    // (3); the mean (FP64) and the thermostat-scaled rotation (FP32)
+    int *f_cnt = (int *)smem;                                                       // ncell
+    int *f_minpid = f_cnt + ncell;                                                  // ncell
+    int *f_exp = f_minpid + ncell;                                                  // ncell (+1 pad)
+    unsigned long long *f_sum = (unsigned long long *)(smem + 8 * (((size_t)3 * ncell + 1) / 2));  // 5 ncell
+    unsigned long long *f_dp = f_sum + 5 * ncell;                                   // 3 ncell
+    double *f_u = (double *)(f_dp + 3 * ncell);                                     // 3 ncell
+    float *f_m = (float *)(f_u + 3 * ncell);                                        // 9 ncell
 
     const long long step = *d.step;
     const double sx = d.shift[3 * tg.r], sy = d.shift[3 * tg.r + 1], sz = d.shift[3 * tg.r + 2];
@@ -377,7 +379,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
             f_cnt[c] = 0;
             f_minpid[c] = 0x7fffffff;
-            f_sum[4 * c] = 0; f_sum[4 * c + 1] = 0; f_sum[4 * c + 2] = 0; f_sum[4 * c + 3] = 0;
+            for (int q = 0; q < 5; q++) f_sum[5 * c + q] = 0;
             f_dp[3 * c] = 0; f_dp[3 * c + 1] = 0; f_dp[3 * c + 2] = 0;
         }
     } else {
@@ -498,10 +500,10 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 const double e = (dx * dx + dy * dy) + dz * dz;
                 atomicAdd(&f_cnt[key], 1);
                 atomicMin(&f_minpid[key], rpid[k]);
-                atomicAdd(&f_sum[4 * key], (unsigned long long)__double2ll_rn(dx * 0x1p40));
-                atomicAdd(&f_sum[4 * key + 1], (unsigned long long)__double2ll_rn(dy * 0x1p40));
-                atomicAdd(&f_sum[4 * key + 2], (unsigned long long)__double2ll_rn(dz * 0x1p40));
-                atomicAdd(&f_sum[4 * key + 3], (unsigned long long)__double2ll_rn(e * 0x1p28));
+                atomicAdd(&f_sum[5 * key], (unsigned long long)__double2ll_rn(dx * 0x1p40));
+                atomicAdd(&f_sum[5 * key + 1], (unsigned long long)__double2ll_rn(dy * 0x1p40));
+                atomicAdd(&f_sum[5 * key + 2], (unsigned long long)__double2ll_rn(dz * 0x1p40));
+                atomicAdd(&f_sum[5 * key + 3], (unsigned long long)__double2ll_rn(e * 0x1p28));
             } else {
                 s_v[g] = rvx[k]; s_v[cap + g] = rvy[k]; s_v[2 * cap + g] = rvz[k];
                 atomicAdd(&s_start[key], 1);
@@ -512,45 +514,75 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     __syncthreads();
     bool zero_var = false;
     if constexpr (FAST) {
-        // per cell (cpp/rotation_mpcd.cpp:14-49): mean u = sum v / N and the
-        // relative kinetic energy sum |v - u|^2 = sum |v|^2 - N |u|^2 (which the
-        // rotation leaves unchanged) in FP64 from the exact sums; the random
-        // axis, the rotation by theta about it and the thermostat factor,
-        // stored as one FP32 matrix M = scale R with u
+        // per cell (cpp/rotation_mpcd.cpp:14-49), in three passes:
+        //  1. the mean u = sum v / N in FP64 from the exact sums; the random
+        //     axis and the rotation R by theta about it (FP32); the scale of
+        //     the relative-energy sum, 2^(60 - e) with sum |v|^2 < 2^e, which
+        //     bounds every term |v - u|^2 and their sum;
+        //  2. per particle |v - u|^2 in FP64, summed exactly in that fixed
+        //     point (no cancellation, whatever |u|; the rotation leaves the
+        //     sum unchanged);
+        //  3. the thermostat factor, folded into M = scale R.
         bool missing = false;
-        const float ct = (float)ph.ct, st = (float)ph.st, ict = (float)ph.ict;
         for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
             const int N = f_cnt[c];
             if (N < 2) continue;
-            const double sx = (double)(long long)f_sum[4 * c] * 0x1p-40;
-            const double sy = (double)(long long)f_sum[4 * c + 1] * 0x1p-40;
-            const double sz = (double)(long long)f_sum[4 * c + 2] * 0x1p-40;
-            const double E = (double)(long long)f_sum[4 * c + 3] * 0x1p-28;
-            const double ux = sx / N, uy = sy / N, uz = sz / N;
-            const double var = E - ((sx * ux + sy * uy) + sz * uz);
-            double scale = sqrt(3.0 * (N - 1) * ph.kbt / (ph.mass_fl * var));
-            if (!(var > 0)) {
-                zero_var = true;
-                scale = 1.0;
-            }
+            const double sx = (double)(long long)f_sum[5 * c] * 0x1p-40;
+            const double sy = (double)(long long)f_sum[5 * c + 1] * 0x1p-40;
+            const double sz = (double)(long long)f_sum[5 * c + 2] * 0x1p-40;
+            const double E = (double)(long long)f_sum[5 * c + 3] * 0x1p-28;
+            int ex;
+            frexp(E + 0x1p-16, &ex);  // sum |v|^2 < 2^ex (2^-16 covers <= 4096 roundings of 2^-29)
+            f_exp[c] = 60 - ex;
+            f_u[3 * c] = sx / N; f_u[3 * c + 1] = sy / N; f_u[3 * c + 2] = sz / N;
             const int kx = ox0 + c % B0, ky = oy0 + (c / B0) % B1, kz = oz0 + c / (B0 * B1);
             double u1, u2;
             if (!draw_axis_raw(d, ph, R, step, linear_cell(kx, ky, kz, L), u1, u2)) {
                 missing = true;
                 u1 = 0.5; u2 = 0.0;
             }
-            // cpp/rotation_mpcd.cpp:19-25: axis (rho = 2U - 1, phi = 2 pi U) and Rodrigues
-            const float rho = (float)(2.0 * u1 - 1.0);
-            float sp, cp;
-            sincosf((float)(6.283185307179586 * u2), &sp, &cp);
-            const float sq = sqrtf(fmaxf(0.0f, 1.0f - rho * rho));
-            const float r1 = cp * sq, r2 = sp * sq, r3 = rho;
-            const float ir1 = ict * r1, ir2 = ict * r2, ir3 = ict * r3, sc = (float)scale;
+            // cpp/rotation_mpcd.cpp:19-25: axis (rho = 2U - 1, phi = 2 pi U) and
+            // Rodrigues, in FP64 (one per cell; sqrt(1 - rho^2) cancels for |rho|
+            // near 1), rounded once to FP32
+            const double rho = 2.0 * u1 - 1.0;
+            const double phi = 6.283185307179586 * u2;
+            const double sq = sqrt(1.0 - rho * rho);
+            const double r1 = cos(phi) * sq, r2 = sin(phi) * sq, r3 = rho;
+            const double ir1 = ph.ict * r1, ir2 = ph.ict * r2, ir3 = ph.ict * r3;
+            const double rm[9] = {ir1 * r1 + ph.ct, ir1 * r2 - ph.st * r3, ir1 * r3 + ph.st * r2,
+                                  ir2 * r1 + ph.st * r3, ir2 * r2 + ph.ct, ir2 * r3 - ph.st * r1,
+                                  ir3 * r1 - ph.st * r2, ir3 * r2 + ph.st * r1, ir3 * r3 + ph.ct};
             float *m = f_m + 9 * c;
-            m[0] = sc * (ir1 * r1 + ct); m[1] = sc * (ir1 * r2 - st * r3); m[2] = sc * (ir1 * r3 + st * r2);
-            m[3] = sc * (ir2 * r1 + st * r3); m[4] = sc * (ir2 * r2 + ct); m[5] = sc * (ir2 * r3 - st * r1);
-            m[6] = sc * (ir3 * r1 - st * r2); m[7] = sc * (ir3 * r2 + st * r1); m[8] = sc * (ir3 * r3 + ct);
-            f_u[3 * c] = (float)ux; f_u[3 * c + 1] = (float)uy; f_u[3 * c + 2] = (float)uz;
+#pragma unroll
+            for (int q = 0; q < 9; q++) m[q] = (float)rm[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SRD_PT; k++) {
+            if (threadIdx.x + k * TH < n) {
+                const int c = rkey[k];
+                if (f_cnt[c] > 1) {
+                    const double ex = (double)rvx[k] - f_u[3 * c], ey = (double)rvy[k] - f_u[3 * c + 1],
+                                 ez = (double)rvz[k] - f_u[3 * c + 2];
+                    const double q = ldexp((ex * ex + ey * ey) + ez * ez, f_exp[c]);
+                    if (q < 0x1p62)  // false for NaN (flagged with the velocity above)
+                        atomicAdd(&f_sum[5 * c + 4], (unsigned long long)__double2ll_rn(q));
+                }
+            }
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+            const int N = f_cnt[c];
+            if (N < 2) continue;
+            const double var = ldexp((double)(long long)f_sum[5 * c + 4], -f_exp[c]);
+            double scale = sqrt(3.0 * (N - 1) * ph.kbt / (ph.mass_fl * var));
+            if (!(var > 0)) {
+                zero_var = true;
+                scale = 1.0;
+            }
+            float *m = f_m + 9 * c;
+#pragma unroll
+            for (int q = 0; q < 9; q++) m[q] = (float)((double)m[q] * scale);
         }
         if (missing) atomicOr(d.status, BW_FLAG_DRAW_MISSING);
     } else {
@@ -663,9 +695,9 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 if (N < 1) continue;
                 double ux, uy, uz;
                 if constexpr (FAST) {
-                    ux = (double)(long long)f_sum[4 * c] * 0x1p-40 / N;
-                    uy = (double)(long long)f_sum[4 * c + 1] * 0x1p-40 / N;
-                    uz = (double)(long long)f_sum[4 * c + 2] * 0x1p-40 / N;
+                    ux = (double)(long long)f_sum[5 * c] * 0x1p-40 / N;
+                    uy = (double)(long long)f_sum[5 * c + 1] * 0x1p-40 / N;
+                    uz = (double)(long long)f_sum[5 * c + 2] * 0x1p-40 / N;
                 } else {
                     ux = s_u[3 * c]; uy = s_u[3 * c + 1]; uz = s_u[3 * c + 2];
                 }
@@ -689,13 +721,15 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             int c = rkey[k];
             int N = FAST ? f_cnt[c] : s_start[c + 1] - s_start[c];
             if (FAST && N > 1) {
+                // v - u in FP64 (then FP32): a cell of small relative velocities
+                // multiplies it by a large thermostat factor
                 const float *m = f_m + 9 * c;
-                const float ux = f_u[3 * c], uy = f_u[3 * c + 1], uz = f_u[3 * c + 2];
+                const double ux = f_u[3 * c], uy = f_u[3 * c + 1], uz = f_u[3 * c + 2];
                 const float vx = (float)rvx[k], vy = (float)rvy[k], vz = (float)rvz[k];
-                const float dx = vx - ux, dy = vy - uy, dz = vz - uz;
-                const float nx = ux + ((m[0] * dx + m[1] * dy) + m[2] * dz);
-                const float ny = uy + ((m[3] * dx + m[4] * dy) + m[5] * dz);
-                const float nz = uz + ((m[6] * dx + m[7] * dy) + m[8] * dz);
+                const float dx = (float)((double)vx - ux), dy = (float)((double)vy - uy), dz = (float)((double)vz - uz);
+                const float nx = (float)ux + ((m[0] * dx + m[1] * dy) + m[2] * dz);
+                const float ny = (float)uy + ((m[3] * dx + m[4] * dy) + m[5] * dz);
+                const float nz = (float)uz + ((m[6] * dx + m[7] * dy) + m[8] * dz);
                 // the particle's exact momentum change; the cell's sum, which the
                 // collision makes zero up to the roundings, is taken back from
                 // the cell's smallest id below
@@ -1702,6 +1736,85 @@ __global__ void k_neigh_fl(Phys ph, Dev d, const int *colloid_rep) {
     }
 }
 
+// ------------------------------------------------------------ updown_velocity
+// cpp/run.cpp:28-50 on colloid g's neigh_fl list (built by neighbour_list_mpcd),
+// with positions and velocities gathered by id: the fluid behind the colloid
+// (r . v_c <= 0) within sigma/2 + 0.5 (rear shell) and sigma/2 + 0.1 (near
+// surface), compacted in list order, and their velocity sums taken in that
+// order.  out per colloid: [cnt, up_cnt, sum v rear (3), sum v near (3)].
+__global__ __launch_bounds__(256) void k_updown(Phys ph, Dev d, const double *pos, const double *vel, double *out) {
+    __shared__ int s_rear[1024], s_up[1024];
+    __shared__ int s_wtot[16];
+    __shared__ int s_nr, s_nu;
+    const int g = blockIdx.x;
+    const Rep R = d.rep[d.crep[g]];
+    const double Ld = (double)R.L;
+    const double lim_up = (0.5 * ph.sigma + 0.1) * (0.5 * ph.sigma + 0.1);
+    const double *pc = d.cpos + 3 * g, *vc = d.cvel + 3 * g;
+    const int n = d.neigh_cnt[g];
+    if (threadIdx.x == 0) { s_nr = 0; s_nu = 0; }
+    __syncthreads();
+    for (int base = 0; base < n; base += blockDim.x) {
+        const int i = base + threadIdx.x;
+        bool rear = false, up = false;
+        int id = 0;
+        if (i < n) {
+            id = d.neigh_list[(long long)g * d.neigh_cap + i];
+            const double *p = pos + 3 * (R.first_fluid + id);
+            const double ax = img1(p[0] - pc[0], Ld), ay = img1(p[1] - pc[1], Ld), az = img1(p[2] - pc[2], Ld);
+            const double d2 = (ax * ax + ay * ay) + az * az, dot = (ax * vc[0] + ay * vc[1]) + az * vc[2];
+            rear = d2 <= ph.lim_shell && dot <= 0.0;
+            up = d2 <= lim_up && dot <= 0.0;
+        }
+        int tr, tu;
+        const int kr = block_compact(rear, s_wtot, tr);
+        if (rear && s_nr + kr < 1024) s_rear[s_nr + kr] = id;
+        const int ku = block_compact(up, s_wtot, tu);
+        if (up && s_nu + ku < 1024) s_up[s_nu + ku] = id;
+        __syncthreads();
+        if (threadIdx.x == 0) { s_nr += tr; s_nu += tu; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (s_nr > 1024 || s_nu > 1024) atomicOr(d.status, BW_FLAG_NEIGH_OVERFLOW);
+        double vr[3] = {0, 0, 0}, vu[3] = {0, 0, 0};
+        for (int j = 0; j < min(s_nr, 1024); j++)
+            for (int k = 0; k < 3; k++) vr[k] += vel[3 * (R.first_fluid + s_rear[j]) + k];
+        for (int j = 0; j < min(s_nu, 1024); j++)
+            for (int k = 0; k < 3; k++) vu[k] += vel[3 * (R.first_fluid + s_up[j]) + k];
+        double *o = out + 8 * g;
+        o[0] = s_nr; o[1] = s_nu;
+        for (int k = 0; k < 3; k++) { o[2 + k] = vr[k]; o[5 + k] = vu[k]; }
+    }
+}
+// The reference's per-colloid results in colloid order (its up_vel is never
+// reset between colloids; both divisions are guarded by up_cnt > 0).
+__global__ void k_updown_finish(Dev d, const double *sums, int *cnt, int *up_cnt, double *vel, double *up_vel) {
+    const int r = blockIdx.x;  // one thread per replica
+    const Rep R = d.rep[r];
+    double up[3] = {0, 0, 0};
+    for (int j = 0; j < R.nc; j++) {
+        const int g = R.first_colloid + j;
+        const double *o = sums + 8 * g;
+        const int c = (int)o[0], cu = (int)o[1];
+        double v[3];
+        for (int k = 0; k < 3; k++) {
+            v[k] = o[2 + k];
+            up[k] = up[k] + o[5 + k];
+        }
+        for (int k = 0; k < 3; k++) {
+            if (cu > 0) {
+                up[k] = up[k] / cu - d.cvel[3 * g + k];
+                v[k] = v[k] / c - d.cvel[3 * g + k];
+            }
+            vel[3 * g + k] = v[k];
+            up_vel[3 * g + k] = up[k];
+        }
+        cnt[g] = c;
+        up_cnt[g] = cu;
+    }
+}
+
 // ------------------------------------------------------------------- audit
 template <class Real>
 __global__ void k_audit(Phys ph, Dev d, Fluid<Real> f) {
@@ -1926,8 +2039,8 @@ __global__ void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, lon
 #define LAUNCH(kern, grid, block, lds, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, st, __VA_ARGS__)
 
 size_t srd_lds_bytes(int ncell, int cap, bool fp64) {
-    if (!fp64)  // count, smallest id, 4 + 3 fixed-point sums, mean (3) and matrix (9) per cell
-        return (size_t)ncell * (8 + 56 + 48) + 16;
+    if (!fp64)  // count, smallest id, exponent; 5 + 3 fixed-point sums, mean (3 FP64), matrix (9 FP32)
+        return 8 * (((size_t)3 * ncell + 1) / 2) + (size_t)ncell * (64 + 24 + 36) + 16;
     return srd_cell_region(ncell, cap) + 3 * (size_t)cap * 8 + (size_t)(2 * ncell + 1) * 4 + (size_t)cap * 2 + 16;
 }
 // static LDS of the SRD kernel's largest variant (colloid lists, masks)
@@ -2072,6 +2185,11 @@ template void launch_mark_phantoms<double>(const Phys &, const Dev &, const Flui
 void launch_debug_geometry(int n, const double *a, double L, const int *ia, int Li, int b, double *img_out,
                            double *mod_out, int *wrap_out, int *div_out, hipStream_t st) {
     LAUNCH(k_debug_geometry, (n + 255) / 256, 256, 0, st, n, a, L, ia, Li, b, img_out, mod_out, wrap_out, div_out);
+}
+void launch_updown(const Phys &ph, const Dev &d, int n_rep, const double *pos, const double *vel, double *sums,
+                   int *cnt, int *up_cnt, double *v, double *up, hipStream_t st) {
+    if (ph.n_colloid_total > 0) LAUNCH(k_updown, ph.n_colloid_total, 256, 0, st, ph, d, pos, vel, sums);
+    LAUNCH(k_updown_finish, n_rep, 1, 0, st, d, sums, cnt, up_cnt, v, up);
 }
 void launch_debug_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st) {
     LAUNCH(k_debug_philox, (n + 255) / 256, 256, 0, st, n, ctr, key, out);

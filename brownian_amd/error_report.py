@@ -60,6 +60,66 @@ def compare(a, b, L) -> dict:
     return out
 
 
+def attribute(a, b, L, params, tol=1e-5) -> dict:
+    """Per-step errors of FP32 storage b against FP64 a, with discrete
+    collision decisions separated from the bulk.
+
+    A fluid particle whose velocity error exceeds tol (relative to the RMS
+    velocity) took a different collision decision near a colloid surface (or
+    followed one).  Each such particle is attributed to the colloid nearest to
+    it.  A colloid's velocity can then differ by at most the momentum those
+    particles exchanged differently, (m_f / m_c) sum |dv_i|, and its angular
+    velocity by (m_f / I) (R + 1) sum |dv_i| (collisions conserve momentum and
+    angular momentum, cpp/fluid_colloid_collision.cpp:34-47); a colloid with no
+    such particle must agree to tol.  Returns the bulk quantiles, how far from
+    a colloid the differing particles lie, the particles whose position
+    differs beyond tol without a differing velocity, and the colloids outside
+    their bound."""
+    pa, va, _ = a.download_fluid()
+    pb, vb, _ = b.download_fluid()
+    ca, cb = a.download_colloid(), b.download_colloid()
+    vrms = float(np.sqrt(np.mean(va ** 2)))
+    dv = np.abs(va - vb).max(axis=1) / vrms
+    dp = np.abs(_pdiff(pa, pb, L)).max(axis=1)
+    odd = np.flatnonzero(dv > tol)
+    nc = len(ca["pos"])
+    m_f, m_c = params["mass_fl"], params["mass_colloid"]
+    R = 0.5 * params["sigma"]
+    I = 0.1 * m_c * params["sigma"] ** 2
+    budget = np.zeros(nc)
+    surf = 0.0  # largest distance of a differing particle from its nearest colloid's centre
+    for i in odd:
+        r = _pdiff(pa[i][None, :], ca["pos"], L)
+        d2 = (r ** 2).sum(1)
+        j = int(np.argmin(d2))
+        surf = max(surf, float(np.sqrt(d2[j])))
+        budget[j] += np.abs(va[i] - vb[i]).sum()
+    pos_odd = np.flatnonzero(dp > tol)
+    vsc = float(np.sqrt(np.mean(ca["vel"] ** 2))) or 1.0
+    wsc = float(np.sqrt(np.mean(ca["omega"] ** 2))) or 1.0
+    dvc = np.abs(ca["vel"] - cb["vel"]).max(axis=1)
+    dwc = np.abs(ca["omega"] - cb["omega"]).max(axis=1)
+    dpc = np.abs(_pdiff(ca["pos"], cb["pos"], L)).max(axis=1)
+    lim_v = tol * vsc + (m_f / m_c) * budget
+    lim_w = tol * wsc + (m_f / I) * (R + 1.0) * budget
+    clean = budget == 0
+    return {
+        "fluid_vel_rel_q999999": float(np.quantile(dv, 1 - 1e-6)),
+        "fluid_pos_abs_q999999": float(np.quantile(dp, 1 - 1e-6)),
+        "fluid_vel_rel_q99999": float(np.quantile(dv, 1 - 1e-5)),
+        "fluid_pos_abs_q99999": float(np.quantile(dp, 1 - 1e-5)),
+        "fluid_vel_rel_max": float(dv.max()), "fluid_n_decisions": int(len(odd)),
+        "decisions_max_dist_from_centre": surf,
+        "fluid_pos_off_not_decisions": int(len(np.setdiff1d(pos_odd, odd))),
+        "colloids_with_decisions": int((~clean).sum()),
+        "clean_colloid_vel_rel": float(dvc[clean].max() / vsc) if clean.any() else 0.0,
+        "clean_colloid_omega_rel": float(dwc[clean].max() / wsc) if clean.any() else 0.0,
+        "colloid_pos_abs": float(dpc.max()),
+        "colloids_over_vel_bound": int((dvc > lim_v).sum()),
+        "colloids_over_omega_bound": int((dwc > lim_w).sum()),
+    }
+
+
 def report(L=128, ncol=2400, steps=3, seed=20240, n_sub=20, log=None) -> dict:
     prm = dict(L=L, density=10, n_colloid=ncol, n_sub=n_sub, ndt=0.1, kbt=1.0, mass_fl=1.0, mass_colloid=50.0,
                sig_colloid=4.2, eps=1.0, sigma=4.0, theta_deg=90.0, v0=0.0, seed=seed, draws=BW_DRAWS_DEVICE,
@@ -82,11 +142,12 @@ def report(L=128, ncol=2400, steps=3, seed=20240, n_sub=20, log=None) -> dict:
         for s in (s32, s64, f64):
             s.status()
         per_step.append(compare(s64, s32, L))
+        per_step[-1]["attribution"] = attribute(s64, s32, L, prm)
         drift.append(compare(f64, s32, L))
         if log:
             log(f"step {k + 1}: per-step fluid vel rel {per_step[-1]['fluid_vel_rel']:.2e}, "
                 f"pos abs {per_step[-1]['fluid_pos_abs']:.2e}")
-    worst = {key: max(e[key] for e in per_step) for key in per_step[0]}
+    worst = {key: max(e[key] for e in per_step) for key in per_step[0] if key != "attribution"}
     return {"workload": f"{L}^3 x 10, {ncol} colloids, {n_sub} substeps, Maxwell, v0~U[0,0.05]",
             "method": "per step: FP64 context loaded with the FP32 state before each step, same device draws; "
                       "drift: FP64 context from the common start, never reloaded",

@@ -121,6 +121,16 @@ BW_API int tumble(bw_ctx *ctx);
 BW_API int rotation_mpcd(bw_ctx *ctx);
 /* cpp/run.cpp:10-26 -- active kick with rear-shell momentum removal        */
 BW_API int run(bw_ctx *ctx);
+/* cpp/run.cpp:28-50 -- rear-shell and near-surface slip velocities behind
+ * each colloid, a diagnostic (the reference discards its results; the fused
+ * step, whose trajectory it does not change, leaves it out).  Needs the
+ * neigh_fl lists of neighbour_list_mpcd.  Results via bw_get_updown.      */
+BW_API int updown_velocity(bw_ctx *ctx);
+/* after updown_velocity, per colloid of replica r: rear-shell and
+ * near-surface counts (cnt, up_cnt) and the reference's vel and up_vel after
+ * that colloid (n_colloid x 3 each; up_vel carries over between colloids as
+ * in the reference) */
+BW_API int bw_get_updown(bw_ctx *ctx, int replica, int *cnt, int *up_cnt, double *vel, double *up_vel);
 /* cpp/update.cpp:48-54 -- colloid Verlet position                          */
 BW_API int update_pos_md(bw_ctx *ctx);
 /* cpp/update.cpp:56-60 -- fluid streaming                                  */

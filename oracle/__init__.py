@@ -82,7 +82,7 @@ def _load(so: str = ORACLE_SO):
                      "orc_update_activity_direction", "orc_update_velocity_colloid",
                      "orc_copy_old_force", "orc_startup", "orc_step", "orc_substep",
                      "orc_drawlog_clear", "orc_ran", "orc_potential", "orc_n_fluid", "orc_n_colloid",
-                     "orc_nbox", "orc_drawlog_count"]:
+                     "orc_nbox", "orc_drawlog_count", "orc_updown_velocity"]:
             getattr(lib, name).argtypes = [C.c_void_p]
         lib.orc_rng_reset.argtypes = [C.c_void_p, C.c_int]
         _libs[so] = lib
@@ -204,6 +204,20 @@ class Oracle:
     def run_lists(self):
         return self._lists(self.lib.orc_get_run_lists, self.nc * 343 * max(1, self.params["density"]) * 4)
 
+    def up_lists(self):
+        return self._lists(self.lib.orc_get_up_lists, self.nc * 343 * max(1, self.params["density"]) * 4)
+
+    def updown(self) -> dict:
+        """Results of the last updown_velocity: shell counts and, per colloid,
+        the reference's vel and up_vel after that colloid."""
+        cnt = np.zeros(max(self.nc, 1), np.int32)
+        up = np.zeros(max(self.nc, 1), np.int32)
+        vel = np.zeros((max(self.nc, 1), 3))
+        upv = np.zeros((max(self.nc, 1), 3))
+        self.lib.orc_get_updown(C.c_void_p(self.h), _ptr(cnt, _ip), _ptr(up, _ip), _ptr(vel), _ptr(upv))
+        n = self.nc
+        return dict(cnt=cnt[:n], up_cnt=up[:n], vel=vel[:n], up_vel=upv[:n])
+
     # draw log
     def drawlog(self):
         n = self.lib.orc_drawlog_count(self.h)
@@ -292,3 +306,6 @@ class Reference:
 
     def run_lists(self):
         return self._lists(self.lib.ref_get_run_lists, self.nc * 7001)
+
+    def up_lists(self):
+        return self._lists(self.lib.ref_get_up_lists, self.nc * 7001)

@@ -78,6 +78,10 @@ struct orc_state {
     /* run lists */
     int *cnt, *nbr_start, *nbr_list;
     size_t nbr_cap;
+    /* updown_velocity: up-shell lists and the per-colloid results */
+    int *up_cnt, *up_start, *up_list;
+    size_t up_cap;
+    v3 *ud_vel, *ud_up;
     /* draw log */
     drawrec *log;
     int nlog, caplog;
@@ -203,6 +207,12 @@ orc_state *orc_create(const orc_params *p) {
     s->nbr_start = (int *)calloc(nc + 1, sizeof(int));
     s->nbr_cap = 1024;
     s->nbr_list = (int *)calloc(s->nbr_cap, sizeof(int));
+    s->up_cnt = (int *)calloc(nc, sizeof(int));
+    s->up_start = (int *)calloc(nc + 1, sizeof(int));
+    s->up_cap = 1024;
+    s->up_list = (int *)calloc(s->up_cap, sizeof(int));
+    s->ud_vel = (v3 *)calloc(nc, sizeof(v3));
+    s->ud_up = (v3 *)calloc(nc, sizeof(v3));
     return s;
 }
 
@@ -215,6 +225,7 @@ void orc_destroy(orc_state *s) {
     free(s->box_neigh); free(s->no_neigh); free(s->neigh_start); free(s->neigh_list);
     free(s->n_neighbour); free(s->neighbour);
     free(s->cnt); free(s->nbr_start); free(s->nbr_list);
+    free(s->up_cnt); free(s->up_start); free(s->up_list); free(s->ud_vel); free(s->ud_up);
     free(s->log);
     free(s);
 }
@@ -561,6 +572,69 @@ void orc_run(orc_state *s) {
         }
     }
     s->nbr_start[s->nc] = (int)w;
+}
+
+/* cpp/run.cpp:28-50: rear-shell (|r| <= sigma/2 + 0.5) and near-surface
+ * (|r| <= sigma/2 + 0.1) fluid behind each colloid (r . v_c <= 0) in neigh_fl
+ * order, and their mean velocity relative to the colloid.  The reference
+ * never resets up_vel between colloids and guards both divisions with
+ * up_cnt > 0; kept as written.  ud_vel[i] / ud_up[i] are vel / up_vel after
+ * colloid i (the reference discards them). */
+void orc_updown_velocity(orc_state *s) {
+    v3 up_vel = V(0, 0, 0);
+    size_t w = 0, wu = 0;
+    double lim = pow(s->p.sigma * 0.5 + 0.5, 2), lim_up = pow(s->p.sigma * 0.5 + 0.1, 2);
+    for (int i = 0; i < s->nc; i++) {
+        v3 vel = V(0, 0, 0);
+        int c = 0, cu = 0;
+        s->nbr_start[i] = (int)w;
+        s->up_start[i] = (int)wu;
+        for (int jj = 0; jj < s->no_neigh[i]; jj++) {
+            int l = s->neigh_list[s->neigh_start[i] + jj];
+            v3 vector = vimg(vsub(s->pos_fl[l], s->pos_c[i]), s->len);
+            if (vsum(vmul(vector, vector)) <= lim && vsum(vmul(vector, s->vel_c[i])) <= 0.0) {
+                if (w + 1 > s->nbr_cap) {
+                    s->nbr_cap *= 2;
+                    s->nbr_list = (int *)realloc(s->nbr_list, sizeof(int) * s->nbr_cap);
+                }
+                s->nbr_list[w++] = l;
+                c++;
+            }
+            if (vsum(vmul(vector, vector)) <= lim_up && vsum(vmul(vector, s->vel_c[i])) <= 0.0) {
+                if (wu + 1 > s->up_cap) {
+                    s->up_cap *= 2;
+                    s->up_list = (int *)realloc(s->up_list, sizeof(int) * s->up_cap);
+                }
+                s->up_list[wu++] = l;
+                cu++;
+            }
+        }
+        s->cnt[i] = c;
+        s->up_cnt[i] = cu;
+        for (int jj = 0; jj < c; jj++) vel = vadd(vel, s->vel_fl[s->nbr_list[s->nbr_start[i] + jj]]);
+        for (int jj = 0; jj < cu; jj++) up_vel = vadd(up_vel, s->vel_fl[s->up_list[s->up_start[i] + jj]]);
+        up_vel = cu > 0 ? vsub(vdivs(up_vel, cu), s->vel_c[i]) : up_vel;
+        vel = cu > 0 ? vsub(vdivs(vel, c), s->vel_c[i]) : vel;
+        s->ud_vel[i] = vel;
+        s->ud_up[i] = up_vel;
+    }
+    s->nbr_start[s->nc] = (int)w;
+    s->up_start[s->nc] = (int)wu;
+}
+
+void orc_get_updown(const orc_state *s, int *cnt, int *up_cnt, double *vel, double *up_vel) {
+    for (int i = 0; i < s->nc; i++) {
+        cnt[i] = s->cnt[i];
+        up_cnt[i] = s->up_cnt[i];
+    }
+    get3(s->ud_vel, s->nc, vel);
+    get3(s->ud_up, s->nc, up_vel);
+}
+int orc_get_up_lists(const orc_state *s, int *counts, int *out) {
+    for (int i = 0; i < s->nc; i++) counts[i] = s->up_cnt[i];
+    int w = s->up_start[s->nc];
+    memcpy(out, s->up_list, sizeof(int) * (size_t)w);
+    return w;
 }
 
 /* ------------------------------------------------------------- collisions */

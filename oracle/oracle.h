@@ -94,6 +94,13 @@ int orc_get_neigh_fl(const orc_state *s, int *counts, int *out);
 int orc_get_md_list(const orc_state *s, int *counts, int *out);
 int orc_get_run_lists(const orc_state *s, int *counts, int *out);
 
+/* cpp/run.cpp:28-50 (a diagnostic the reference computes and discards):
+ * rear/up shell counts, the up-shell lists, and per colloid the reference's
+ * vel and up_vel after that colloid (n x 3 each) */
+void orc_updown_velocity(orc_state *s);
+void orc_get_updown(const orc_state *s, int *cnt, int *up_cnt, double *vel, double *up_vel);
+int orc_get_up_lists(const orc_state *s, int *counts, int *out);
+
 /* draw log: every draw consumed since the last orc_drawlog_clear */
 void orc_drawlog_clear(orc_state *s);
 int orc_drawlog_count(const orc_state *s);

@@ -213,6 +213,17 @@ int ref_get_md_list(int *counts, int *out) {
     }
     return w;
 }
+// cpp/run.cpp:28-50 (rewrites cnt/nbr and up_cnt/up_nbr; its velocities are locals)
+void ref_updown_velocity(void) { updown_velocity(); }
+// updown_velocity's near-surface lists: up_cnt per colloid, then up_nbr entries
+int ref_get_up_lists(int *counts, int *out) {
+    int w = 0;
+    for (int i = 1; i <= no_of_colloid; i++) {
+        counts[i - 1] = up_cnt[i];
+        for (int j = 1; j <= up_cnt[i]; j++) out[w++] = up_nbr[j][i];
+    }
+    return w;
+}
 // run() rear-shell lists: cnt per colloid, then concatenated nbr entries
 int ref_get_run_lists(int *counts, int *out) {
     int w = 0;

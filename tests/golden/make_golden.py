@@ -135,6 +135,22 @@ def case_overlap(q):
     q.put(out)
 
 
+def case_updown(q):
+    # cpp/run.cpp:28-50 where the reference calls it (cpp/active_brownian.cpp:32):
+    # after neighbour_list_mpcd of a substep, here the first substep of step 2
+    r = O.Reference(**C0LIKE)
+    r.startup()
+    r.steps(1)
+    r.rotation_mpcd_bigstack()
+    r.run()
+    r.copy_old_force(); r.update_pos_md(); r.neighbour_list_md(); r.update_pos_mpcd(); r.neighbour_list_mpcd()
+    out = {"pre_" + k: v for k, v in _state(r).items()}
+    r.updown_velocity()
+    out["cnt"], out["nbr"] = r.run_lists()
+    out["up_cnt"], out["up_nbr"] = r.up_lists()
+    q.put(out)
+
+
 def _run(fn):
     ctx = mp.get_context("spawn")
     q = ctx.Queue()
@@ -150,6 +166,14 @@ def _run(fn):
 def main():
     if not os.path.exists(O.REF_SO):
         O.build(with_reference=True)
+    if "--only" in sys.argv:
+        which = sys.argv[sys.argv.index("--only") + 1]
+        fn, params = {"updown": (case_updown, C0LIKE)}[which]
+        out = _run(fn)
+        out["params_json"] = np.array(repr(sorted(O.reference_defaults(**params).items())))
+        np.savez_compressed(os.path.join(HERE, f"{which}.npz"), **out)
+        print(which, "ok", len(out), "arrays")
+        return
     rng = _run(case_rng)
     rng.update(_run(case_rng_neg))
     np.savez_compressed(os.path.join(HERE, "rng.npz"), **rng)

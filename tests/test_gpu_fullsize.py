@@ -62,3 +62,40 @@ def test_full_size_momentum_temperature_and_flags():
         assert drift < 1e-10, drift
     T = a["ke_fluid"] / (1.5 * s.info[0]["nf"])
     assert abs(T - 1.0) < 0.01, T
+
+
+@pytest.mark.timeout(600)
+def test_fp32_per_step_error_at_full_size():
+    """FP32 storage against an FP64 run loaded with the same state before each
+    step (same device draws), at the benchmarked size.  A particle within a
+    rounding distance of a colloid surface can reflect in one precision and
+    not the other; that colloid's velocity then differs and so does every
+    later reflection off it.  Required per step: every particle off by more
+    than 1e-5 (velocity relative to the RMS) lies at a colloid surface (within
+    R + 1.5 of a centre) and there are fewer than 1e-5 N of them, so the
+    1 - 1e-5 quantile of the fluid errors is within 1e-5 in velocity and in
+    position (cells); no position differs beyond 1e-5 without its velocity;
+    colloids with no such particle near them agree to 1e-5; every other
+    colloid stays within the momentum and angular momentum that the differing
+    particles exchanged (cpp/fluid_colloid_collision.cpp:34-47)."""
+    from brownian_amd.error_report import _copy_state, attribute
+    v0 = np.random.default_rng(7).uniform(0.0, 0.05, PRM["n_colloid"])
+    s32 = started(v0)
+    s64 = started(v0, precision=B.BW_FP64)
+    worst = {}
+    for step in range(2):
+        _copy_state(s32, s64)
+        s32.step(1)
+        s64.step(1)
+        assert s32.status() == 0 and s64.status() == 0
+        a = attribute(s64, s32, PRM["L"], PRM)
+        print(f"step {step}: {a}")
+        assert a["fluid_n_decisions"] < 1e-5 * 20971520, a
+        assert a["decisions_max_dist_from_centre"] <= 0.5 * PRM["sigma"] + 1.5, a
+        assert a["fluid_vel_rel_q99999"] <= 1e-5 and a["fluid_pos_abs_q99999"] <= 1e-5, a
+        assert a["fluid_pos_off_not_decisions"] == 0, a
+        assert a["clean_colloid_vel_rel"] <= 1e-5 and a["clean_colloid_omega_rel"] <= 1e-5, a
+        assert a["colloids_over_vel_bound"] == 0 and a["colloids_over_omega_bound"] == 0, a
+        for k, v in a.items():
+            worst[k] = max(worst.get(k, v), v)
+    print("worst over the steps:", worst)

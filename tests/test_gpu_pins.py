@@ -144,3 +144,53 @@ def test_driver_cli_reference_argv():
     # a bad argument is an error, not a crash
     p = subprocess.run([exe, "5", "2", "--L", "5"], capture_output=True, text=True, timeout=120)
     assert p.returncode == 1 and "bw_ctx_create" in p.stderr
+
+
+def test_updown_velocity_against_reference_lists_and_oracle():
+    """updown_velocity (cpp/run.cpp:28-50) on the state where the compiled
+    reference ran it: its shell counts bit for bit, and the slip velocities
+    against the oracle; then on a C0-sized state after a device-draw step."""
+    import ast
+    import oracle as O
+    g = dict(np.load(os.path.join(ROOT, "tests", "golden", "updown.npz")))
+    p = dict(ast.literal_eval(str(g.pop("params_json"))))
+    sp = {k: v for k, v in p.items() if k in ("L", "density", "n_colloid", "n_sub", "ndt", "kbt", "mass_fl",
+                                            "mass_colloid", "sig_colloid", "eps", "sigma", "theta_deg", "v0", "seed")}
+    s = B.System(precision=B.BW_FP64, draws=B.BW_DRAWS_DEVICE, **sp)
+    s.startup()
+    s.upload_fluid(g["pre_pos"], g["pre_vel"])
+    s.upload_colloid(**{k: g["pre_c_" + k] for k in ("pos", "vel", "omega", "f", "old_f", "ra")})
+    s.neighbour_list_mpcd()
+    s.updown_velocity()
+    assert s.status() == 0
+    u = s.updown()
+    np.testing.assert_array_equal(u["cnt"], g["cnt"])
+    np.testing.assert_array_equal(u["up_cnt"], g["up_cnt"])
+    o = O.Oracle(**p)
+    o.set_fluid(g["pre_pos"], g["pre_vel"])
+    o.set_colloid(**{k: g["pre_c_" + k] for k in ("pos", "vel", "omega", "f", "old_f", "ra")})
+    o.create_box(); o.neighbour_list_mpcd(); o.updown_velocity()
+    ou = o.updown()
+    np.testing.assert_allclose(u["vel"], ou["vel"], rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(u["up_vel"], ou["up_vel"], rtol=1e-12, atol=1e-13)
+    # a larger state: 20^3 x 10, 8 colloids after one device-draw step
+    prm = dict(L=20, density=10, n_colloid=8, sig_colloid=4.2, sigma=4.0, mass_colloid=50.0, n_sub=20,
+               theta_deg=90.0, v0=0.01, seed=31)
+    s = B.System(precision=B.BW_FP64, draws=B.BW_DRAWS_DEVICE, **prm)
+    s.startup()
+    s.step(1)
+    s.neighbour_list_mpcd()
+    s.updown_velocity()
+    assert s.status() == 0
+    u = s.updown()
+    o = O.Oracle(**prm)
+    pos, vel, _ = s.download_fluid()
+    o.set_fluid(pos, vel)
+    o.set_colloid(**s.download_colloid())
+    o.create_box(); o.neighbour_list_mpcd(); o.updown_velocity()
+    ou = o.updown()
+    np.testing.assert_array_equal(u["cnt"], ou["cnt"])
+    np.testing.assert_array_equal(u["up_cnt"], ou["up_cnt"])
+    np.testing.assert_allclose(u["vel"], ou["vel"], rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(u["up_vel"], ou["up_vel"], rtol=1e-12, atol=1e-13)
+    assert u["cnt"].sum() > 100

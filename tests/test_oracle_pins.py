@@ -218,3 +218,53 @@ def test_oracle_rotation_invariants(theta):
     m = k == O.DRAW_AXIS
     axes[keys[m, 0]] = vals[m, :2]
     assert rotation_axes_check(vel, vel1, cells, axes, theta) < 1e-9
+
+
+def updown_brute_force(pos, vel, cpos, cvel, L, sigma, neigh):
+    """cpp/run.cpp:28-50 evaluated directly: per colloid the fluid of its
+    neigh_fl list behind it (r . v_c <= 0) within sigma/2 + 0.5 (rear shell) and
+    sigma/2 + 0.1 (near surface), the mean velocity relative to the colloid,
+    with the reference's running (never reset) up_vel and its up_cnt guard on
+    both divisions."""
+    out_v, out_u, up = [], [], np.zeros(3)
+    for i, lst in enumerate(neigh):
+        r = pos[lst] - cpos[i]
+        r -= L * np.round(r / L)
+        d2 = (r ** 2).sum(1)
+        behind = r @ cvel[i] <= 0
+        rear = lst[(d2 <= (sigma / 2 + 0.5) ** 2) & behind]
+        near = lst[(d2 <= (sigma / 2 + 0.1) ** 2) & behind]
+        v = vel[rear].sum(0) if len(rear) else np.zeros(3)
+        up = up + (vel[near].sum(0) if len(near) else 0.0)
+        if len(near):
+            up = up / len(near) - cvel[i]
+            v = v / len(rear) - cvel[i]
+        out_v.append(v)
+        out_u.append(up.copy())
+    return np.array(out_v), np.array(out_u)
+
+
+def test_updown_velocity_against_reference_lists_and_brute_force():
+    import ast
+    g = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "updown.npz")))
+    p = dict(ast.literal_eval(str(g.pop("params_json"))))
+    o = O.Oracle(**p)
+    o.set_fluid(g["pre_pos"], g["pre_vel"])
+    o.set_colloid(**{k: g["pre_c_" + k] for k in ("pos", "vel", "omega", "f", "old_f", "ra")})
+    o.create_box()
+    o.neighbour_list_mpcd()
+    o.updown_velocity()
+    # the lists the compiled reference wrote (cnt/nbr and up_cnt/up_nbr), bit for bit
+    cnt, nbr = o.run_lists()
+    np.testing.assert_array_equal(cnt, g["cnt"])
+    np.testing.assert_array_equal(nbr, g["nbr"])
+    up_cnt, up_nbr = o.up_lists()
+    np.testing.assert_array_equal(up_cnt, g["up_cnt"])
+    np.testing.assert_array_equal(up_nbr, g["up_nbr"])
+    res = o.updown()
+    counts, flat = o.neigh_fl()
+    neigh = np.split(flat, np.cumsum(counts)[:-1])
+    v, u = updown_brute_force(g["pre_pos"], g["pre_vel"], g["pre_c_pos"], g["pre_c_vel"], p["L"], p["sigma"], neigh)
+    np.testing.assert_allclose(res["vel"], v, rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(res["up_vel"], u, rtol=1e-13, atol=1e-14)
+    assert np.all(res["up_cnt"] > 0)
