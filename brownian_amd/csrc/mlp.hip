@@ -17,6 +17,7 @@
 #include <cmath>
 #include <type_traits>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/brownian.h"
@@ -170,6 +171,22 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float
     if (wb) wb[i] = f2bf(wi);
 }
 
+// Activations of bw_mlp_forward_noalloc for up to NOALLOC_ROWS rows, in a
+// file-scope device array instead of allocations: input (BF16), three hidden
+// layers (pre-activation and activation in FP32, activation in BF16) and the
+// padded output.  Each GPU holds its own copy until the process ends.
+constexpr int NOALLOC_ROWS = 1024;
+constexpr size_t NOALLOC_ROW_BYTES = (size_t)MLP_INP * 2 + (size_t)MLP_OUTP * 4 + 3 * ((size_t)MLP_H * 10);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_ROWS * NOALLOC_ROW_BYTES];
+
+// rows [0, n) of the padded output (stride MLP_OUTP) -> n x MLP_OUT
+__global__ void k_mlp_rows(const float *__restrict__ y, int n, float *__restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * MLP_OUT) return;
+    const int i = e / MLP_OUT, k = e - i * MLP_OUT;
+    out[e] = y[(size_t)i * MLP_OUTP + k];
+}
+
 }  // namespace bw
 
 using namespace bw;
@@ -188,6 +205,7 @@ struct bw_mlp {
     float *xmean = nullptr, *xistd = nullptr, *tmean = nullptr, *tistd = nullptr;
     long long adam_t = 0;
     MlpBuf tb;  // training / host-call activations
+    unsigned char *scratch = nullptr;  // device address of the scratch array on the network's GPU
     // data set on the device
     float *dX = nullptr, *dT = nullptr;
     int *didx = nullptr;
@@ -247,6 +265,44 @@ int malloc0(bw_mlp *m, T **p, size_t n) {
     m->allocs.push_back(q);
     *p = (T *)q;
     return BW_OK;
+}
+
+// The scratch array's device address on the current GPU, looked up once per
+// device (the array's host-side name is not a device address).
+int scratch_address(bw_mlp *m, unsigned char **out) {
+    static std::mutex mu;
+    static unsigned char *addr[256] = {};
+    int dev = 0;
+    MCHK(m, hipGetDevice(&dev));
+    if (dev < 0 || dev >= 256) return mfail(m, BW_ERR_HIP, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        MCHK(m, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        addr[dev] = (unsigned char *)p;
+    }
+    *out = addr[dev];
+    return BW_OK;
+}
+
+// MlpBuf of n_pad rows carved from the scratch array (forward buffers only)
+MlpBuf scratch_buf(unsigned char *base, int n_pad) {
+    MlpBuf b;
+    b.cap = n_pad;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char *p = base + o;
+        o += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    b.Xb = (unsigned short *)take((size_t)n_pad * MLP_INP * 2);
+    b.Y = (float *)take((size_t)n_pad * MLP_OUTP * 4);
+    for (int l = 0; l < 3; l++) {
+        b.Z[l] = (float *)take((size_t)n_pad * MLP_H * 4);
+        b.H[l] = (float *)take((size_t)n_pad * MLP_H * 4);
+        b.Hb[l] = (unsigned short *)take((size_t)n_pad * MLP_H * 2);
+    }
+    return b;
 }
 
 unsigned short host_bf16(float f) {
@@ -353,6 +409,7 @@ BW_API int bw_mlp_create(int seed, bw_mlp **out) {
     *out = m;
     MCHK(m, hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
     int rc;
+    if ((rc = scratch_address(m, &m->scratch))) return rc;
     for (int l = 0; l < MLP_L; l++) {
         size_t n = (size_t)m->K[l] * m->N[l];
         if ((rc = malloc0(m, &m->W[l], n)) || (rc = malloc0(m, &m->gW[l], n)) || (rc = malloc0(m, &m->mW[l], n)) ||
@@ -450,6 +507,24 @@ BW_API int bw_mlp_forward(bw_mlp *m, const float *x, int n, float *y) {
     hipFree(dx);
     for (int i = 0; i < n; i++)
         for (int k = 0; k < MLP_OUT; k++) y[(size_t)i * MLP_OUT + k] = yy[(size_t)i * MLP_OUTP + k];
+    return BW_OK;
+}
+
+// bw_mlp_forward on device buffers with no allocation: the activations live
+// in the scratch array (n <= NOALLOC_ROWS).  Every buffer it reads is written
+// first in the same call, so nothing needs clearing; calls that overlap on one
+// GPU share the array and race.
+BW_API int bw_mlp_forward_noalloc(bw_mlp *m, const float *x, int n, float *y) {
+    if (!m || !x || !y || n < 1) return BW_ERR_ARG;
+    if (n > NOALLOC_ROWS)
+        return mfail(m, BW_ERR_ARG, "bw_mlp_forward_noalloc: more rows than the scratch array holds (1024)");
+    if (!m->scratch) return mfail(m, BW_ERR_STATE, "bw_mlp_forward_noalloc: no scratch address");
+    const int n_pad = (n + 31) / 32 * 32;
+    MlpBuf B = scratch_buf(m->scratch, n_pad);
+    forward(m, B, x, nullptr, n, n_pad, m->st);
+    hipLaunchKernelGGL(k_mlp_rows, dim3((n * MLP_OUT + 255) / 256), dim3(256), 0, m->st, B.Y, n, y);
+    MCHK(m, hipGetLastError());
+    MCHK(m, hipStreamSynchronize(m->st));
     return BW_OK;
 }
 

@@ -281,6 +281,15 @@ BW_API int bw_mlp_loss_grad(bw_mlp *m, const float *x, const float *t, int n, do
 /* Adam over shuffled mini-batches; loss_out[epoch] = mean loss of the epoch */
 BW_API int bw_mlp_train(bw_mlp *m, const float *x, const float *t, int n, int epochs, int batch, float lr,
                         int seed, double *loss_out);
+/* No counterpart in the reference: bw_mlp_forward for callers that cannot
+ * allocate (tight serving loops, harnesses that forbid hipMalloc).  x (n x 144
+ * raw features) and y (n x 12, the bw_mlp_forward outputs) are DEVICE
+ * pointers on the network's GPU; the activations live in a fixed device
+ * array of the library (at most 1024 rows; larger n is BW_ERR_ARG, about
+ * 7.9 MiB held by each GPU until the process ends), so the call allocates
+ * nothing.  It synchronises the network's stream before returning.  Calls
+ * that overlap on one GPU share that array and race. */
+BW_API int bw_mlp_forward_noalloc(bw_mlp *m, const float *x, int n, float *y);
 /* the rotation pass also writes coarse-cell moments (needs even edges) */
 BW_API int bw_set_moments(bw_ctx *ctx, int on);
 /* drive colloids by the network's sampled impulses wherever every feature lies
