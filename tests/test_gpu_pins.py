@@ -194,3 +194,38 @@ def test_updown_velocity_against_reference_lists_and_oracle():
     np.testing.assert_allclose(u["vel"], ou["vel"], rtol=1e-12, atol=1e-13)
     np.testing.assert_allclose(u["up_vel"], ou["up_vel"], rtol=1e-12, atol=1e-13)
     assert u["cnt"].sum() > 100
+
+
+def test_driver_observable_series_and_trajectory(tmp_path):
+    """The driver's output files: one JSON line of energies, momentum and
+    observables per audit (energy_colloid = potential + kinetic + rotational,
+    cpp/active_brownian.cpp:51), and per audit each colloid's position,
+    velocity and angular velocity in point::print's format
+    (cpp/parameters.hpp:60) agreeing with the JSON energies."""
+    import json
+    import re
+    exe = os.path.join(ROOT, "brownian_amd", "bin", "active_brownian")
+    obs, traj = tmp_path / "obs.jsonl", tmp_path / "traj.txt"
+    p = subprocess.run([exe, "10", "3", "--L", "12", "--niter", "6", "--nsub", "10", "--theta", "90", "--v0", "0.02",
+                        "--mass", "50", "--sig", "2.5", "--sigma", "2", "--audit-every", "2", "--observables", str(obs),
+                        "--trajectory", str(traj)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    rows = [json.loads(l) for l in obs.read_text().splitlines()]
+    assert [r["step"] for r in rows] == [2, 4, 6]
+    for r in rows:
+        assert r["energy_colloid"] == pytest.approx(r["potential"] + r["ke_colloid"] + r["ang_ke_colloid"], rel=1e-15)
+        assert 0.5 < r["t_fluid"] < 1.5 and r["msd"] > 0 and -1 <= r["orient_corr"] <= 1
+    assert rows[-1]["msd"] > rows[0]["msd"]
+    lines = traj.read_text().splitlines()
+    num = r"-?\d+\.\d{32}"
+    pat = re.compile(rf"^ *{num} +{num} +{num}$")
+    blocks = [i for i, l in enumerate(lines) if l.startswith("# step ")]
+    assert [lines[i] for i in blocks] == ["# step 2", "# step 4", "# step 6"]
+    for b in blocks:
+        body = lines[b + 1:b + 1 + 9]
+        assert all(pat.match(l) for l in body), body
+        vals = np.array([[float(x) for x in l.split()] for l in body]).reshape(3, 3, 3)  # colloid, (pos, vel, w), xyz
+        assert np.all((vals[:, 0] >= 0) & (vals[:, 0] <= 12))
+        step = int(lines[b].split()[-1])
+        r = next(r for r in rows if r["step"] == step)
+        assert 0.5 * 50 * (vals[:, 1] ** 2).sum() == pytest.approx(r["ke_colloid"], rel=1e-12)
