@@ -200,7 +200,7 @@ def run_ensemble(B, n_rep, steps, warmup):
     reps = ensemble_params(n_rep)
     t0 = time.time()
     s = B.System(replicas=[dict(r, precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
-                                orient_law=B.BW_ORIENT_SPHERE) for r in reps])
+                                orient_law=B.BW_ORIENT_SPHERE, fluid_init=B.BW_INIT_DEVICE) for r in reps])
     s.startup()
     s.sync()
     t_setup = time.time() - t0
@@ -227,7 +227,7 @@ def run_surrogate(B, n_rep, collect_steps, epochs, steps):
     network is trained on pairs from an ensemble with disjoint seeds."""
     from brownian_amd import surrogate as S
     common = dict(precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
-                  orient_law=B.BW_ORIENT_SPHERE)
+                  orient_law=B.BW_ORIENT_SPHERE, fluid_init=B.BW_INIT_DEVICE)
     t0 = time.time()
     tr = B.System(replicas=[dict(r, **common) for r in ensemble_params(n_rep, seed0=5000)])
     tr.startup()
@@ -314,7 +314,7 @@ def main():
     t0 = time.time()
     mode = B.BW_STREAM_BLOCKED if args.streaming == "blocked" else B.BW_STREAM_PER_SUBSTEP
     sprm = dict(precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
-                orient_law=B.BW_ORIENT_SPHERE, streaming=mode, **prm)
+                orient_law=B.BW_ORIENT_SPHERE, streaming=mode, fluid_init=B.BW_INIT_DEVICE, **prm)
     rng = np.random.default_rng(args.seed)
     v0 = rng.uniform(0.0, 0.05, args.ncol)
     tune = None

@@ -26,6 +26,7 @@ BW_VEL_REFERENCE, BW_VEL_MAXWELL = 0, 1
 BW_ORIENT_REFERENCE, BW_ORIENT_SPHERE = 0, 1
 BW_DRAWS_DEVICE, BW_DRAWS_INJECTED = 0, 1
 BW_STREAM_BLOCKED, BW_STREAM_PER_SUBSTEP = 0, 1
+BW_INIT_HOST, BW_INIT_DEVICE = 0, 1
 KERNEL_FAMILIES = ["stream", "collide", "depart", "srd", "colloid", "other"]
 
 ERRORS = {-1: "BW_ERR_ARG", -2: "BW_ERR_HIP", -3: "BW_ERR_CAPACITY", -4: "BW_ERR_DRAWS",
@@ -49,7 +50,7 @@ class Params(C.Structure):
         ("sig_colloid", C.c_double), ("eps", C.c_double), ("sigma", C.c_double), ("theta_deg", C.c_double),
         ("v0", C.c_double), ("seed", C.c_int), ("precision", C.c_int), ("vel_law", C.c_int),
         ("orient_law", C.c_int), ("draws", C.c_int), ("tile", C.c_int * 3), ("streaming", C.c_int),
-        ("reserved", C.c_int * 7),
+        ("fluid_init", C.c_int), ("reserved", C.c_int * 6),
     ]
 
 

@@ -44,6 +44,10 @@ void launch_debug_geometry(int n, const double *a, double L, const int *ia, int 
                            double *mod_out, int *wrap_out, int *div_out, hipStream_t st);
 void launch_copy(const void *, void *, long long, hipStream_t);
 void launch_debug_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st);
+template <class Real>
+void launch_init_fluid(const Phys &ph, const Dev &d, const InitBins &b, int vel_law, long long n_total, double *pos,
+                       double *vel, double *mean, const Fluid<Real> &scratch, const Fluid<Real> &out, int *fill,
+                       int *count, hipStream_t st);
 void launch_updown(const Phys &ph, const Dev &d, int n_rep, const double *pos, const double *vel, double *sums,
                    int *cnt, int *up_cnt, double *v, double *up, hipStream_t st);
 void launch_debug_reflect(const Phys &ph, const Dev &d, int n, long long step, int substep, double L,
@@ -450,7 +454,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         if (p.n_sub != p0.n_sub || p.ndt != p0.ndt || p.kbt != p0.kbt || p.mass_fl != p0.mass_fl ||
             p.mass_colloid != p0.mass_colloid || p.sig_colloid != p0.sig_colloid || p.eps != p0.eps ||
             p.sigma != p0.sigma || p.theta_deg != p0.theta_deg || p.precision != p0.precision ||
-            p.draws != p0.draws || p.streaming != p0.streaming) {
+            p.draws != p0.draws || p.streaming != p0.streaming || p.fluid_init != p0.fluid_init) {
             delete c;
             return BW_ERR_ARG;
         }
@@ -922,9 +926,81 @@ BW_API int initialize_colloid(bw_ctx *c) {
     return BW_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// initialize_fluid on the GPU (BW_INIT_DEVICE): coarse colloid bins built on
+// the host, then placement, mean removal and tile binning on the device
+template <class Real>
+int init_fluid_device(bw_ctx *c) {
+    int rc;
+    if (!c->stage_pos && ((rc = dalloc(c, &c->stage_pos, 3 * (size_t)c->nf_total)) ||
+                          (rc = dalloc(c, &c->stage_vel, 3 * (size_t)c->nf_total)) ||
+                          (rc = dalloc(c, &c->stage_cell, c->nf_total))))
+        return rc;
+    std::vector<int> G(c->n_rep), start(1, 0), list;
+    std::vector<long long> first(c->n_rep);
+    std::vector<double> cpos(3 * (size_t)std::max(c->nc_total, 1));
+    if (c->nc_total)
+        HIPCHK(c, hipMemcpy(cpos.data(), c->cposb[c->cb], sizeof(double) * 3 * c->nc_total, hipMemcpyDeviceToHost));
+    for (int r = 0; r < c->n_rep; r++) {
+        const ReplicaHost &R = c->reps[r];
+        const double rad = 0.5 * R.p.sigma;
+        int g = (int)floor(R.L / std::max(rad, 1.0));
+        g = std::min(g, 32);
+        if (g < 3) g = 1;  // edge >= sigma/2; one bin when fewer than 3 fit
+        G[r] = g;
+        first[r] = (long long)start.size() - 1;
+        const double h = (double)R.L / g;
+        std::vector<std::vector<int>> bins((size_t)g * g * g);
+        for (int j = 0; j < R.nc; j++) {
+            const int gj = R.first_colloid + j;
+            int b3[3];
+            for (int k = 0; k < 3; k++) b3[k] = std::min((int)(cpos[3 * gj + k] / h), g - 1);
+            bins[(size_t)b3[0] + g * ((size_t)b3[1] + (size_t)g * b3[2])].push_back(gj);
+        }
+        for (auto &bl : bins) {
+            list.insert(list.end(), bl.begin(), bl.end());
+            start.push_back((int)list.size());
+        }
+    }
+    // temporaries of the start-up, in one allocation freed before returning
+    const size_t n_list = std::max<size_t>(list.size(), 1);
+    const size_t bytes = sizeof(double) * 3 * c->n_rep + sizeof(long long) * first.size() +
+                         sizeof(int) * (G.size() + start.size() + n_list + (size_t)c->ph.n_tiles) + 64;
+    char *tmp = nullptr;
+    HIPCHK(c, hipMalloc((void **)&tmp, bytes));
+    double *mean = (double *)tmp;
+    long long *dfirst = (long long *)(mean + 3 * c->n_rep);
+    int *dG = (int *)(dfirst + first.size());
+    int *dstart = dG + G.size(), *dlist = dstart + start.size(), *fill = dlist + n_list;
+    HIPCHK(c, hipMemsetAsync(fill, 0, sizeof(int) * (size_t)c->ph.n_tiles, c->st));
+    HIPCHK(c, hipMemcpy(dG, G.data(), sizeof(int) * G.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dfirst, first.data(), sizeof(long long) * first.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dstart, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice));
+    if (!list.empty()) HIPCHK(c, hipMemcpy(dlist, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice));
+    InitBins b{dG, dfirst, dstart, dlist, c->cposb[c->cb]};
+    Dev d = dev_for(c, c->cur);
+    HIPCHK(c, hipMemsetAsync(c->dv.status, 0, sizeof(unsigned), c->st));
+    launch_init_fluid<Real>(c->ph, d, b, c->reps[0].p.vel_law, c->nf_total, c->stage_pos, c->stage_vel, mean,
+                            fluid_of<Real>(c, 1 - c->cur), fluid_of<Real>(c, c->cur), fill, c->counts[c->cur], c->st);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    hipFree(tmp);
+    unsigned flags = 0;
+    HIPCHK(c, hipMemcpy(&flags, c->dv.status, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (flags & BW_FLAG_TILE_OVERFLOW) return fail(c, BW_ERR_CAPACITY, "tile capacity exceeded in the device start-up");
+    if (flags) return fail(c, BW_ERR_PHYSICS, "device start-up could not place a particle outside the colloids");
+    return BW_OK;
+}
+}  // namespace
+
+extern "C" {
+
 BW_API int initialize_fluid(bw_ctx *c) {
     if (!c) return BW_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->st));
+    if (c->reps[0].p.fluid_init == BW_INIT_DEVICE)
+        return c->fp64 ? init_fluid_device<double>(c) : init_fluid_device<float>(c);
     for (int r = 0; r < c->n_rep; r++) {
         ReplicaHost &R = c->reps[r];
         std::vector<double> cpos(3 * std::max(R.nc, 1));

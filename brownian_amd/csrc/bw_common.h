@@ -153,4 +153,14 @@ struct Dev {
                                    // impulse range (6 each) at offset N_FEAT
 };
 
+// Coarse bins of the colloids for the device start-up's exclusion test: per
+// replica a G^3 grid of edge L/G >= sigma/2 (CSR lists of global colloid ids)
+struct InitBins {
+    const int *G;              // [n_rep]
+    const long long *first;    // [n_rep] first bin of the replica
+    const int *start;          // [bins + 1]
+    const int *list;           // colloid ids
+    const double *cpos;        // colloid positions (global)
+};
+
 }  // namespace bw

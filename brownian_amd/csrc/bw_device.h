@@ -56,7 +56,7 @@ __device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
     return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
 }
 
-enum : uint32_t { KIND_SHIFT = 1, KIND_AXIS = 2, KIND_REFL = 3, KIND_SUR = 5 };
+enum : uint32_t { KIND_SHIFT = 1, KIND_AXIS = 2, KIND_REFL = 3, KIND_SUR = 5, KIND_INIT_POS = 6, KIND_INIT_VEL = 7 };
 
 __device__ __forceinline__ uint2 rep_key(const Rep &R) { return make_uint2(R.seed_lo, R.seed_hi); }
 

@@ -1736,6 +1736,149 @@ __global__ void k_neigh_fl(Phys ph, Dev d, const int *colloid_rep) {
     }
 }
 
+// ------------------------------------------------------------ device start-up
+// cpp/initialize.cpp:62-83 with counter-based draws keyed by (replica seed,
+// particle id): particle i draws uniform positions in the box (attempt a in
+// the counter) until none of its replica's colloids lies closer than sigma/2
+// (the colloids of the 27 coarse bins around it: a skipped colloid is farther
+// than a bin edge >= sigma/2); then its velocity, uniform in +-sqrt(12 kT/m)/2
+// per component (the reference's law) or Maxwell-Boltzmann (Box-Muller).
+__device__ __forceinline__ int rep_of_particle(const Dev &d, int n_rep, long long i) {
+    int lo = 0, hi = n_rep - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d.rep[mid].first_fluid <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+__global__ void k_init_place(Phys ph, Dev d, InitBins b, int vel_law, long long n_total, double *pos, double *vel) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_total) return;
+    const int r = rep_of_particle(d, ph.n_rep, i);
+    const Rep R = d.rep[r];
+    const long long id = i - R.first_fluid;
+    const double L = R.L, rad = 0.5 * ph.sigma;
+    const int G = b.G[r];  // 1 (every colloid in one bin) or >= 3
+    const double h = L / G;
+    double t[3] = {0, 0, 0};
+    bool placed = false;
+    for (uint32_t a = 0; a < 100000u && !placed; a++) {
+        const uint4 u = philox(make_uint4(KIND_INIT_POS | (a << 8), (uint32_t)id, (uint32_t)(id >> 32), 0u), rep_key(R));
+        const uint4 w = philox(make_uint4(KIND_INIT_POS | (a << 8), (uint32_t)id, (uint32_t)(id >> 32), 1u), rep_key(R));
+        t[0] = u01(u.x, u.y) * L; t[1] = u01(u.z, u.w) * L; t[2] = u01(w.x, w.y) * L;
+        placed = true;
+        if (R.nc > 0) {
+            int bx = min((int)(t[0] / h), G - 1), by = min((int)(t[1] / h), G - 1), bz = min((int)(t[2] / h), G - 1);
+            const int span = G < 3 ? 0 : 1;
+            for (int dz = -span; dz <= span && placed; dz++)
+                for (int dy = -span; dy <= span && placed; dy++)
+                    for (int dx = -span; dx <= span && placed; dx++) {
+                        const int X = G < 3 ? 0 : (bx + dx + G) % G, Y = G < 3 ? 0 : (by + dy + G) % G,
+                                  Z = G < 3 ? 0 : (bz + dz + G) % G;
+                        const long long bin = b.first[r] + X + (long long)G * (Y + (long long)G * Z);
+                        for (int q = b.start[bin]; q < b.start[bin + 1]; q++) {
+                            const double *c = b.cpos + 3 * b.list[q];
+                            const double ax = img1(t[0] - c[0], L), ay = img1(t[1] - c[1], L), az = img1(t[2] - c[2], L);
+                            if (sqrt((ax * ax + ay * ay) + az * az) < rad) { placed = false; break; }
+                        }
+                    }
+        }
+    }
+    if (!placed) atomicOr(d.status, BW_FLAG_DRIFT);
+    const uint4 u = philox(make_uint4(KIND_INIT_VEL, (uint32_t)id, (uint32_t)(id >> 32), 0u), rep_key(R));
+    const uint4 w = philox(make_uint4(KIND_INIT_VEL, (uint32_t)id, (uint32_t)(id >> 32), 1u), rep_key(R));
+    const double q[4] = {u01(u.x, u.y), u01(u.z, u.w), u01(w.x, w.y), u01(w.z, w.w)};
+    double v[3];
+    if (vel_law == BW_VEL_MAXWELL) {
+        const uint4 e = philox(make_uint4(KIND_INIT_VEL, (uint32_t)id, (uint32_t)(id >> 32), 2u), rep_key(R));
+        const double q2[2] = {u01(e.x, e.y), u01(e.z, e.w)};
+        const double s = sqrt(ph.kbt / ph.mass_fl), tp = 6.283185307179586;
+        const double r1 = sqrt(-2.0 * log(1.0 - q[0])), r2 = sqrt(-2.0 * log(1.0 - q[2]));
+        v[0] = s * r1 * cos(tp * q[1]);
+        v[1] = s * r1 * sin(tp * q[1]);
+        v[2] = s * r2 * cos(tp * q2[0]);
+    } else {
+        const double vs = sqrt(12.0 * ph.kbt / ph.mass_fl);
+        for (int k = 0; k < 3; k++) v[k] = (q[k] - 0.5) * vs;
+    }
+    for (int k = 0; k < 3; k++) {
+        pos[3 * i + k] = t[k];
+        vel[3 * i + k] = v[k];
+    }
+}
+// per replica (one workgroup each): mean velocity, summed in a fixed order
+__global__ __launch_bounds__(256) void k_init_mean(Dev d, const double *vel, double *mean) {
+    __shared__ double s[256][3];
+    const Rep R = d.rep[blockIdx.x];
+    double a[3] = {0, 0, 0};
+    for (long long i = threadIdx.x; i < R.nf; i += 256)
+        for (int k = 0; k < 3; k++) a[k] += vel[3 * (R.first_fluid + i) + k];
+    for (int k = 0; k < 3; k++) s[threadIdx.x][k] = a[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int k = 0; k < 3; k++) s[threadIdx.x][k] += s[threadIdx.x + h][k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) mean[3 * blockIdx.x + k] = s[0][k] / (double)R.nf;
+}
+// mean removed; cell + offset; appended to its tile of buffer `f` (any order)
+template <class Real>
+__global__ void k_init_bin(Phys ph, Dev d, long long n_total, const double *pos, const double *vel,
+                           const double *mean, Fluid<Real> f, int *fill) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_total) return;
+    const int r = rep_of_particle(d, ph.n_rep, i);
+    const Rep R = d.rep[r];
+    int ic[3];
+    Real o[3];
+    for (int k = 0; k < 3; k++) {
+        const double x = pos[3 * i + k];
+        long long q = (long long)floor(x);
+        Real ro = (Real)(x - (double)q);
+        if (ro >= Real(1)) { ro = Real(0); q += 1; }
+        q %= R.L;
+        if (q < 0) q += R.L;
+        ic[k] = (int)q;
+        o[k] = ro;
+    }
+    const int T = R.first_tile + ic[0] / R.B[0] + R.nt[0] * (ic[1] / R.B[1] + R.nt[1] * (ic[2] / R.B[2]));
+    const int e = atomicAdd(&fill[T], 1);
+    if (e >= ph.cap) { atomicOr(d.status, BW_FLAG_TILE_OVERFLOW); return; }
+    const size_t s = (size_t)T * ph.cap + e;
+    f.cell[s] = pack_cell(ic[0], ic[1], ic[2]);
+    f.ox[s] = o[0]; f.oy[s] = o[1]; f.oz[s] = o[2];
+    f.vx[s] = (Real)(vel[3 * i] - mean[3 * r]);
+    f.vy[s] = (Real)(vel[3 * i + 1] - mean[3 * r + 1]);
+    f.vz[s] = (Real)(vel[3 * i + 2] - mean[3 * r + 2]);
+    f.pid[s] = (int)(i - R.first_fluid);
+}
+// each tile's slots in ascending particle id (the upload's layout), from
+// buffer `a` into buffer `b`
+template <class Real>
+__global__ __launch_bounds__(512) void k_init_sort(Phys ph, Dev d, const int *fill, Fluid<Real> a, Fluid<Real> b,
+                                                   int *count) {
+    __shared__ int s_pid[MAX_CAP];
+    const int T = blockIdx.x;
+    const int n = min(fill[T], ph.cap);
+    const size_t base = (size_t)T * ph.cap;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s_pid[i] = a.pid[base + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int p = s_pid[i];
+        int rank = 0;
+        for (int j = 0; j < n; j++) rank += s_pid[j] < p ? 1 : 0;
+        const size_t src = base + i, dst = base + rank;
+        b.cell[dst] = a.cell[src];
+        b.ox[dst] = a.ox[src]; b.oy[dst] = a.oy[src]; b.oz[dst] = a.oz[src];
+        b.vx[dst] = a.vx[src]; b.vy[dst] = a.vy[src]; b.vz[dst] = a.vz[src];
+        b.pid[dst] = p;
+    }
+    if (threadIdx.x == 0) count[T] = n;
+}
+
 // ------------------------------------------------------------ updown_velocity
 // cpp/run.cpp:28-50 on colloid g's neigh_fl list (built by neighbour_list_mpcd),
 // with positions and velocities gathered by id: the fluid behind the colloid
@@ -2191,6 +2334,21 @@ void launch_updown(const Phys &ph, const Dev &d, int n_rep, const double *pos, c
     if (ph.n_colloid_total > 0) LAUNCH(k_updown, ph.n_colloid_total, 256, 0, st, ph, d, pos, vel, sums);
     LAUNCH(k_updown_finish, n_rep, 1, 0, st, d, sums, cnt, up_cnt, v, up);
 }
+template <class Real>
+void launch_init_fluid(const Phys &ph, const Dev &d, const InitBins &b, int vel_law, long long n_total, double *pos,
+                       double *vel, double *mean, const Fluid<Real> &scratch, const Fluid<Real> &out, int *fill,
+                       int *count, hipStream_t st) {
+    const unsigned g = (unsigned)((n_total + 255) / 256);
+    LAUNCH(k_init_place, g, 256, 0, st, ph, d, b, vel_law, n_total, pos, vel);
+    LAUNCH(k_init_mean, ph.n_rep, 256, 0, st, d, vel, mean);
+    LAUNCH(k_init_bin<Real>, g, 256, 0, st, ph, d, n_total, pos, vel, mean, scratch, fill);
+    LAUNCH(k_init_sort<Real>, ph.n_tiles, 512, 0, st, ph, d, fill, scratch, out, count);
+}
+template void launch_init_fluid<float>(const Phys &, const Dev &, const InitBins &, int, long long, double *, double *,
+                                       double *, const Fluid<float> &, const Fluid<float> &, int *, int *, hipStream_t);
+template void launch_init_fluid<double>(const Phys &, const Dev &, const InitBins &, int, long long, double *,
+                                        double *, double *, const Fluid<double> &, const Fluid<double> &, int *, int *,
+                                        hipStream_t);
 void launch_debug_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st) {
     LAUNCH(k_debug_philox, (n + 255) / 256, 256, 0, st, n, ctr, key, out);
 }

@@ -53,6 +53,12 @@ enum bw_draw_source { BW_DRAWS_DEVICE = 0, BW_DRAWS_INJECTED = 1 };
  * the step are streamed through all substeps by the rotation pass (same
  * arithmetic, same results); PER_SUBSTEP: every particle every substep. */
 enum bw_streaming { BW_STREAM_BLOCKED = 0, BW_STREAM_PER_SUBSTEP = 1 };
+/* initialize_fluid.  HOST: the reference's generator on the host, bit-exact
+ * with cpp/initialize.cpp:62-83 (and the draws of everything after it);
+ * DEVICE: the same laws with counter-based draws on the GPU (keyed by seed and
+ * particle id), the state binned into tiles on the GPU -- for large systems
+ * and ensembles, where the host start-up dominates. */
+enum bw_fluid_init { BW_INIT_HOST = 0, BW_INIT_DEVICE = 1 };
 
 /* Run parameters.  Mirrors the reference's global defaults
  * (cpp/active_brownian.cpp:3-7) with every value exposed; see
@@ -78,7 +84,8 @@ typedef struct bw_params {
     int draws;           /* BW_DRAWS_DEVICE (counter-based) or BW_DRAWS_INJECTED     */
     int tile[3];         /* tile shape in cells; 0 = choose                          */
     int streaming;       /* bw_streaming of the fused step                           */
-    int reserved[7];
+    int fluid_init;      /* bw_fluid_init of initialize_fluid                        */
+    int reserved[6];
 } bw_params;
 
 typedef struct bw_ctx bw_ctx;
