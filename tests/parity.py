@@ -68,3 +68,26 @@ def compare_state(sysm, o, L, pos_tol, vel_tol, col_tol, replica=0, label=""):
     for k in c:
         assert out["c_" + k] <= col_tol, f"{label} colloid {k} differs by {out['c_' + k]}"
     return out
+
+
+def blocked_equals_per_substep(a, b, L, pos_tol, replica=0):
+    """a (blocked streaming) and b (per-substep streaming) holding the same
+    start-up state take one step: velocities and every colloid quantity equal
+    bit for bit, positions within pos_tol (FP32 storage flies far particles
+    through the step in one rounding, the per-substep kernel in n_sub).  One
+    step from a start-up only: a reflection can leave an offset outside
+    [0,1), and reloading a state re-normalises such a (cell, offset) pair, so
+    a reloaded context would round that particle's next flight differently.
+    Returns the largest position difference."""
+    a.step(1)
+    b.step(1)
+    assert a.status() == 0 and b.status() == 0
+    pa, va, _ = a.download_fluid(replica)
+    pb, vb, _ = b.download_fluid(replica)
+    np.testing.assert_array_equal(va, vb)
+    dp = np.abs(pdiff(pa, pb, L)).max() if len(pa) else 0.0
+    assert dp <= pos_tol, dp
+    da, db = a.download_colloid(replica), b.download_colloid(replica)
+    for k in da:
+        np.testing.assert_array_equal(da[k], db[k], err_msg=k)
+    return dp

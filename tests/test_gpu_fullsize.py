@@ -3,9 +3,10 @@ do not depend on size: 128^3 cells x 10 fluid particles per cell (20,971,520),
 2400 colloids (WCA sigma 4.2, mass 50), 20 substeps per MPCD step, FP32
 storage with FP64 cell sums, tile (8,4,4), device draws.
 
-- Blocked streaming (far particles streamed through the whole step by the
-  rotation pass) equals per-substep streaming of every particle, bit for bit,
-  with the benchmark's activities v0 ~ U[0, 0.05].
+- Blocked streaming (far particles flown through the whole step by the
+  rotation pass) equals per-substep streaming of every particle over a step
+  (velocities and colloids bit for bit, positions to FP32 rounding), with the
+  benchmark's activities v0 ~ U[0, 0.05].
 - Without activity the total momentum (fluid + colloids, FP64 sums) is
   conserved to 1e-10 of its scale, the thermostat holds the fluid at kT
   (1 %), and no status flag (overflow, drift, margin) is raised."""
@@ -31,23 +32,15 @@ def started(v0=None, **over):
 
 
 def test_full_size_blocked_streaming_equals_per_substep():
+    """Blocked streaming (far particles flown through the step in one
+    rounding by the rotation pass) against every particle streamed every
+    substep, one step from a common start-up: velocities and colloids bit for
+    bit, positions within 2e-6 cells."""
+    from tests.parity import blocked_equals_per_substep
     v0 = np.random.default_rng(PRM["seed"]).uniform(0.0, 0.05, PRM["n_colloid"])
     a = started(v0, streaming=B.BW_STREAM_BLOCKED)
-    a.step(2)
-    assert a.status() == 0
-    pa, va, ca = a.download_fluid()
-    da = a.download_colloid()
-    del a
     b = started(v0, streaming=B.BW_STREAM_PER_SUBSTEP)
-    b.step(2)
-    assert b.status() == 0
-    pb, vb, cb = b.download_fluid()
-    np.testing.assert_array_equal(ca, cb)
-    np.testing.assert_array_equal(pa, pb)
-    np.testing.assert_array_equal(va, vb)
-    db = b.download_colloid()
-    for k in da:
-        np.testing.assert_array_equal(da[k], db[k], err_msg=k)
+    blocked_equals_per_substep(a, b, PRM["L"], 2e-6)
 
 
 def test_full_size_momentum_temperature_and_flags():
