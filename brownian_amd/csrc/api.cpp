@@ -54,6 +54,7 @@ void launch_debug_reflect(const Phys &ph, const Dev &d, int n, long long step, i
                           const double *rf, const double *rs, double *vals6, double *vel3, hipStream_t st);
 void launch_features(const Phys &, const Dev &, const int *, hipStream_t);
 template <class Real> void launch_mark_phantoms(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
+template <class Real> void launch_sur_shell_kick(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
 hipError_t prepare_kernels(Phys &ph, bool fp64);
 void launch_sur_sample(const Phys &, const Dev &, const int *, hipStream_t);
 }  // namespace bw
@@ -345,6 +346,7 @@ int enqueue_step(bw_ctx *c, int cur, int cb, bool final, Prof *prof) {
     if (prof) prof->mark(st, BW_KERNEL_DEPART);
     launch_srd<Real>(ph, d0, src, dst, col, false, c->block, c->srd_lds, st);
     if (prof) prof->mark(st, BW_KERNEL_SRD);
+    if (col && ph.surrogate) launch_sur_shell_kick<Real>(ph, dev_at(c, 1 - cur, cb), dst, st);
     for (int l = 0; l < ph.n_sub; l++) {
         Dev d1 = dev_at(c, 1 - cur, cb);
         // blocked without colloids: the rotation pass did all the streaming
@@ -641,6 +643,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         (rc = dalloc(c, &d.ev_total, n_rep)) ||
         (rc = dalloc(c, &d.cmom, c->moments_ok ? (size_t)cells / 8 * 5 : 1)) || (rc = dalloc(c, &d.mom_valid, 1)) ||
         (rc = dalloc(c, &d.cmode, NC)) || (rc = dalloc(c, &d.cmode_prev, NC)) || (rc = dalloc(c, &d.csur, 6 * NC)) || (rc = dalloc(c, &d.cimp, 6 * NC)) ||
+        (rc = dalloc(c, &d.sshell, NC)) || (rc = dalloc(c, &d.smir, 3 * NC)) ||
         (rc = dalloc(c, &d.feat, NC * N_FEAT)) || (rc = dalloc(c, &d.sur_out, NC * 12)) ||
         (rc = dalloc(c, &d.cvel, 3 * NC)) || (rc = dalloc(c, &d.comega, 3 * NC)) ||
         (rc = dalloc(c, &d.cf, 3 * NC)) || (rc = dalloc(c, &d.coldf, 3 * NC)) || (rc = dalloc(c, &d.cra, 3 * NC)) ||
@@ -904,6 +907,8 @@ BW_API int initialize(bw_ctx *c) {
     HIPCHK(c, hipMemsetAsync(c->dv.cimp, 0, sizeof(double) * 6 * NC, c->st));
     HIPCHK(c, hipMemsetAsync(c->dv.cmode, 0, sizeof(int) * NC, c->st));
     HIPCHK(c, hipMemsetAsync(c->dv.cmode_prev, 0, sizeof(int) * NC, c->st));
+    HIPCHK(c, hipMemsetAsync(c->dv.sshell, 0, sizeof(int) * NC, c->st));
+    HIPCHK(c, hipMemsetAsync(c->dv.smir, 0, sizeof(unsigned long long) * 3 * NC, c->st));
     HIPCHK(c, hipMemsetAsync(c->dv.mom_valid, 0, sizeof(int), c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->cb = 0;

@@ -147,6 +147,13 @@ struct Dev {
     int *cmode_prev;           // per colloid: the mode of the previous step
     double *csur;              // per colloid: surrogate impulse per substep (6, cdv/cdw units)
     double *cimp;              // per colloid: collision impulse accumulated since the features (6)
+    // per surrogate-driven colloid and step, formed by the rotation pass: the
+    // fluid in its shell (distance <= sigma/2 + 0.5 from the centre), which
+    // pays its sampled impulse, its run() kick and the mirrored fluid's
+    // momentum change, and that momentum change (3, fixed point 2^-40, in
+    // units of the fluid mass)
+    int *sshell;
+    unsigned long long *smir;
     float *feat;               // per colloid: surrogate features (144)
     float *sur_out;            // per colloid: impulse mean (6) and log-variance (6)
     const float *sur_lo, *sur_hi;  // feature range of the training data (144 each), then

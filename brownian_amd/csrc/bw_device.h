@@ -694,6 +694,10 @@ __device__ inline void colloid_load(const Phys &ph, const Dev &d, int g, unsigne
         in.sv[k] = ((flags & PH_APPLY) && ph.surrogate) ? d.csur[6 * g + k] : 0.0;
         in.imp[k] = (flags & PH_APPLY) ? d.cimp[6 * g + k] : 0.0;
     }
+    // a surrogate-driven colloid whose shell held no fluid cannot pass its
+    // impulse on: it takes back the mirrored fluid's momentum change instead
+    if (in.sur && d.sshell[g] == 0)
+        for (int k = 0; k < 3; k++) in.sv[k] = -((double)(long long)d.smir[3 * g + k] * 0x1p-40) / ph.n_sub;
     in.v0 = (flags & PH_KICK) ? d.cv0[g] : 0.0;
 }
 // The per-colloid phases of one substep, in PH_* order, by one thread, on the
@@ -757,6 +761,8 @@ __device__ inline void colloid_update(const Phys &ph, const Dev &d, int L, int r
             d.ccell_run[3 * g + k] = in.cell[k];
         }
         d.ccnt[g] = 0;
+        d.sshell[g] = 0;
+        d.smir[3 * g] = 0; d.smir[3 * g + 1] = 0; d.smir[3 * g + 2] = 0;
     }
     if (flags & PH_OLDF)
         for (int k = 0; k < 3; k++) of[k] = f[k];

@@ -307,6 +307,22 @@ __device__ __forceinline__ unsigned long long fixq_diff(float a, float b) {
     const long long qb = fabsf(b) < 1024.0f ? __double2ll_rn((double)b * 0x1p40) : 0;
     return (unsigned long long)(qa - qb);
 }
+__device__ __forceinline__ unsigned long long fixq_diff(double a, double b) {
+    const long long qa = fabs(a) < 1024.0 ? __double2ll_rn(a * 0x1p40) : 0;
+    const long long qb = fabs(b) < 1024.0 ? __double2ll_rn(b * 0x1p40) : 0;
+    return (unsigned long long)(qa - qb);
+}
+// a tile's surrogate shell counts and mirrored momentum into the per-colloid
+// totals (integer sums: the same in any order)
+__device__ inline void flush_surrogate_shells(const Dev &d, int ns, const int *sg, const int *scnt,
+                                              const unsigned long long (*smir)[3]) {
+    for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+        const int g = sg[q];
+        if (scnt[q]) atomicAdd(&d.sshell[g], scnt[q]);
+        for (int k = 0; k < 3; k++)
+            if (smir[q][k]) atomicAdd(&d.smir[3 * g + k], smir[q][k]);
+    }
+}
 // bytes of the SRD pass's per-cell region (7 doubles per cell), which also
 // holds the cap ids of the sort
 __host__ __device__ inline size_t srd_cell_region(int ncell, int cap) {
@@ -331,6 +347,8 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     __shared__ AxisMask s_am[COUNT_SHELL ? 2 : 1];  // [0] run shells, [1] near test
     __shared__ int s_cc[SRD_PT][TH / 64];   // near particles per chunk and wave
     __shared__ double s_sp[COUNT_SHELL ? TCL_CAP : 1][6];  // surrogate colloids: position, velocity
+    __shared__ int s_sg[COUNT_SHELL ? TCL_CAP : 1], s_scnt[COUNT_SHELL ? TCL_CAP : 1];  // id, shell count
+    __shared__ unsigned long long s_smir[COUNT_SHELL ? TCL_CAP : 1][3];  // mirrored momentum (2^-40)
     __shared__ int s_sn;
     const int T = blockIdx.x;
     Rep R;
@@ -386,7 +404,11 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         for (int c = threadIdx.x; c < ncell + 1; c += blockDim.x) s_start[c] = 0;
     }
     if (COUNT_SHELL)
-        for (int k = threadIdx.x; k < TCL_CAP; k += blockDim.x) s_shell[k] = 0;
+        for (int k = threadIdx.x; k < TCL_CAP; k += blockDim.x) {
+            s_shell[k] = 0;
+            s_scnt[k] = 0;
+            s_smir[k][0] = 0; s_smir[k][1] = 0; s_smir[k][2] = 0;
+        }
     // the tile's colloid list, requested now so that it arrives with the
     // particles (the step's list pass ran before this kernel; a list made
     // stale since is rebuilt below)
@@ -440,6 +462,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         const int ns = min(d.tsl_cnt[T], TCL_CAP);
         for (int q = threadIdx.x; q < ns; q += blockDim.x) {
             const int g = d.tsl_list[T * TCL_CAP + q];
+            s_sg[q] = g;
             for (int k = 0; k < 3; k++) {
                 s_sp[q][k] = d.cpos[3 * g + k];
                 s_sp[q][3 + k] = d.cvel[3 * g + k];
@@ -753,16 +776,64 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 rvy[k] = (Real)(uy + (v1y - uy) * sc);
                 rvz[k] = (Real)(uz + (v1z - uz) * sc);
             }
-            bool near = !BLOCK, moved = false;
+            bool near = !BLOCK, moved = false, shell = false;
             if (COUNT_SHELL) {
+                if (ph.surrogate && s_sn) {
+                    // fluid that entered a surrogate-driven colloid during the last
+                    // step is mirrored back out through its surface, its velocity
+                    // reversed relative to the colloid's; the momentum change is
+                    // summed per colloid, and the fluid in the colloid's shell (on
+                    // the stored position, the test k_sur_shell_kick repeats) is
+                    // counted: it pays the change back with the colloid's sampled
+                    // impulse and run() kick
+                    int ix, iy, iz;
+                    unpack_cell(rcell[k], ix, iy, iz);
+                    const double Ld = (double)L, R2 = 2.0 * sqrt(ph.lim_coll);  // colloid diameter
+                    const double x = (double)ix + (double)rox[k], y = (double)iy + (double)roy[k],
+                                 z = (double)iz + (double)roz[k];
+                    for (int q = 0; q < s_sn; q++) {
+                        double ax = img1(x - s_sp[q][0], Ld), ay = img1(y - s_sp[q][1], Ld), az = img1(z - s_sp[q][2], Ld);
+                        double r2 = (ax * ax + ay * ay) + az * az;
+                        if (r2 >= ph.lim_coll) continue;
+                        double rr = sqrt(r2);
+                        double sc = rr > 1e-12 ? (R2 - rr) / rr : 0.0;  // radius rr -> 2R - rr
+                        double nx = s_sp[q][0] + (rr > 1e-12 ? ax * sc : 0.5 * R2 + 1e-6);
+                        double ny = s_sp[q][1] + ay * sc, nz = s_sp[q][2] + az * sc;
+                        nx = mod1(nx, Ld); ny = mod1(ny, Ld); nz = mod1(nz, Ld);
+                        int jx = coord_cell(nx, L), jy = coord_cell(ny, L), jz = coord_cell(nz, L);
+                        rcell[k] = pack_cell(jx, jy, jz) | (rcell[k] & PHANTOM);
+                        rox[k] = (Real)(nx - jx); roy[k] = (Real)(ny - jy); roz[k] = (Real)(nz - jz);
+                        if (!(rox[k] < Real(1))) rox[k] = Real(0);
+                        if (!(roy[k] < Real(1))) roy[k] = Real(0);
+                        if (!(roz[k] < Real(1))) roz[k] = Real(0);
+                        const Real wx = (Real)(2.0 * s_sp[q][3] - (double)rvx[k]);
+                        const Real wy = (Real)(2.0 * s_sp[q][4] - (double)rvy[k]);
+                        const Real wz = (Real)(2.0 * s_sp[q][5] - (double)rvz[k]);
+                        atomicAdd(&s_smir[q][0], fixq_diff(wx, rvx[k]));
+                        atomicAdd(&s_smir[q][1], fixq_diff(wy, rvy[k]));
+                        atomicAdd(&s_smir[q][2], fixq_diff(wz, rvz[k]));
+                        rvx[k] = wx; rvy[k] = wy; rvz[k] = wz;
+                        moved = true;  // streamed per substep
+                        break;
+                    }
+                    unpack_cell(rcell[k], ix, iy, iz);
+                    const double x1 = (double)ix + (double)rox[k], y1 = (double)iy + (double)roy[k],
+                                 z1 = (double)iz + (double)roz[k];
+                    for (int q = 0; q < s_sn; q++) {
+                        double ax = img1(x1 - s_sp[q][0], Ld), ay = img1(y1 - s_sp[q][1], Ld), az = img1(z1 - s_sp[q][2], Ld);
+                        if ((ax * ax + ay * ay) + az * az <= ph.lim_shell) {
+                            atomicAdd(&s_scnt[q], 1);
+                            shell = true;  // kicked in the first substep
+                        }
+                    }
+                }
                 int ix, iy, iz;
                 unpack_cell(rcell[k], ix, iy, iz);
                 const double x = (double)ix + (double)rox[k], y = (double)iy + (double)roy[k],
                              z = (double)iz + (double)roz[k];
                 unsigned m = cell_mask(s_am[0], rg, ix, iy, iz, L);
                 if (NEAR_TEST) {
-                    // conservative test in FP32, tile-local coordinates (2e-2 cell slack),
-                    // on the position before any mirroring below
+                    // conservative test in FP32, tile-local coordinates (2e-2 cell slack)
                     const float Lf = (float)L, iLf = 1.0f / Lf;
                     const float px = (float)pdiff(ix, ox0, L) + (float)rox[k];
                     const float py = (float)pdiff(iy, oy0, L) + (float)roy[k];
@@ -801,33 +872,6 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                         if (q < tc.n && shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
                     });
                 }
-                if (ph.surrogate) {
-                    // fluid that entered a surrogate-driven colloid during the last
-                    // step is mirrored back out through its surface, its velocity
-                    // reversed relative to the colloid's (no impulse on the colloid)
-                    const double Ld = (double)L, R2 = 2.0 * sqrt(ph.lim_coll);  // colloid diameter
-                    for (int q = 0; q < s_sn; q++) {
-                        double ax = img1(x - s_sp[q][0], Ld), ay = img1(y - s_sp[q][1], Ld), az = img1(z - s_sp[q][2], Ld);
-                        double r2 = (ax * ax + ay * ay) + az * az;
-                        if (r2 >= ph.lim_coll) continue;
-                        double rr = sqrt(r2);
-                        double sc = rr > 1e-12 ? (R2 - rr) / rr : 0.0;  // radius rr -> 2R - rr
-                        double nx = s_sp[q][0] + (rr > 1e-12 ? ax * sc : 0.5 * R2 + 1e-6);
-                        double ny = s_sp[q][1] + ay * sc, nz = s_sp[q][2] + az * sc;
-                        nx = mod1(nx, Ld); ny = mod1(ny, Ld); nz = mod1(nz, Ld);
-                        int jx = coord_cell(nx, L), jy = coord_cell(ny, L), jz = coord_cell(nz, L);
-                        rcell[k] = pack_cell(jx, jy, jz) | (rcell[k] & PHANTOM);
-                        rox[k] = (Real)(nx - jx); roy[k] = (Real)(ny - jy); roz[k] = (Real)(nz - jz);
-                        if (!(rox[k] < Real(1))) rox[k] = Real(0);
-                        if (!(roy[k] < Real(1))) roy[k] = Real(0);
-                        if (!(roz[k] < Real(1))) roz[k] = Real(0);
-                        rvx[k] = (Real)(2.0 * s_sp[q][3] - (double)rvx[k]);
-                        rvy[k] = (Real)(2.0 * s_sp[q][4] - (double)rvy[k]);
-                        rvz[k] = (Real)(2.0 * s_sp[q][5] - (double)rvz[k]);
-                        moved = true;  // streamed per substep (its near test saw the old position)
-                        break;
-                    }
-                }
                 if (ph.surrogate) {  // inside a colloid that was a phantom in the last step
                     const double Ld = (double)L;
                     for_candidates(tc, m, [&](int q) {
@@ -837,7 +881,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                     });
                 }
             }
-            if (near || moved) nearbits |= 1u << k;
+            if (near || moved || shell) nearbits |= 1u << k;
         }
     }
     // output slots: near particles first, each group in gather order
@@ -853,9 +897,11 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             if (lane == 0) s_cc[k][w] = __popcll(bal[k]);
         }
         __syncthreads();
-        if (COUNT_SHELL)  // the apply phase's shell counts are complete
+        if (COUNT_SHELL) {  // the apply phase's shell counts are complete
             for (int q = threadIdx.x; q < tc.n; q += blockDim.x)
                 if (s_shell[q]) atomicAdd(&d.ccnt[tc.g[q]], s_shell[q]);
+            if (ph.surrogate) flush_surrogate_shells(d, s_sn, s_sg, s_scnt, s_smir);
+        }
         int base = 0;
 #pragma unroll
         for (int k = 0; k < SRD_PT; k++) {
@@ -973,6 +1019,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         __syncthreads();
         for (int q = threadIdx.x; q < tc.n; q += blockDim.x)
             if (s_shell[q]) atomicAdd(&d.ccnt[tc.g[q]], s_shell[q]);
+        if (ph.surrogate) flush_surrogate_shells(d, s_sn, s_sg, s_scnt, s_smir);
     }
 }
 
@@ -2102,6 +2149,61 @@ __global__ void k_sur_sample(Phys ph, Dev d, const int *colloid_rep) {
     }
 }
 
+// First substep, surrogate-driven colloids: the fluid in a colloid's shell
+// (distance from its centre <= sigma/2 + 0.5, counted by the rotation pass on
+// the same stored positions) pays the colloid's momentum debt of the step --
+// its sampled impulse, its run() kick (cpp/run.cpp:14, whose rear shell the
+// explicit path kicks) and the momentum change of the fluid mirrored out of
+// it -- in equal shares, so that the step conserves momentum as the explicit
+// collisions do (cpp/fluid_colloid_collision.cpp:46-47).  Shell particles were
+// made near, so they are among the first near_cnt of their tile; kicks of
+// several colloids are summed in FP64 and rounded once.  A colloid with an
+// empty shell receives the mirrored momentum's negative instead of its
+// impulse (colloid_load).  One workgroup per tile.
+template <class Real>
+__global__ __launch_bounds__(256) void k_sur_shell_kick(Phys ph, Dev d, Fluid<Real> f) {
+    __shared__ double s_k[TCL_CAP][6];  // position, velocity decrement
+    const int T = blockIdx.x;
+    const int ns = min(d.tsl_cnt[T], TCL_CAP);
+    if (ns == 0) return;
+    const int L = d.rep[d.tile_rep[T]].L;
+    for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+        const int g = d.tsl_list[T * TCL_CAP + q];
+        const int cnt = d.sshell[g];
+        const double ratio = ph.mass_colloid / ph.mass_fl, v0 = d.cv0[g];
+        for (int k = 0; k < 3; k++) {
+            s_k[q][k] = d.cpos[3 * g + k];
+            const double debt = ((double)ph.n_sub * d.csur[6 * g + k] + (double)(long long)d.smir[3 * g + k] * 0x1p-40) +
+                                ratio * (d.cra[3 * g + k] * v0);
+            s_k[q][3 + k] = cnt > 0 ? debt / cnt : 0.0;
+        }
+    }
+    __syncthreads();
+    const int n = min(d.near_cnt[T], ph.cap);
+    const double Ld = (double)L;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int slot = T * ph.cap + i;
+        int ix, iy, iz;
+        unpack_cell(f.cell[slot], ix, iy, iz);
+        const double x = (double)ix + (double)f.ox[slot], y = (double)iy + (double)f.oy[slot],
+                     z = (double)iz + (double)f.oz[slot];
+        double vx = f.vx[slot], vy = f.vy[slot], vz = f.vz[slot];
+        bool hit = false;
+        for (int q = 0; q < ns; q++) {
+            double ax = img1(x - s_k[q][0], Ld), ay = img1(y - s_k[q][1], Ld), az = img1(z - s_k[q][2], Ld);
+            if ((ax * ax + ay * ay) + az * az <= ph.lim_shell) {
+                vx = vx - s_k[q][3];
+                vy = vy - s_k[q][4];
+                vz = vz - s_k[q][5];
+                hit = true;
+            }
+        }
+        if (hit) {
+            f.vx[slot] = (Real)vx; f.vy[slot] = (Real)vy; f.vz[slot] = (Real)vz;
+        }
+    }
+}
+
 // When the surrogate is detached: every particle inside a colloid that is a
 // phantom now is marked, per tile (the tile's phantom colloids are listed as
 // for the tile colloid lists)
@@ -2337,6 +2439,12 @@ void launch_mark_phantoms(const Phys &ph, const Dev &d, const Fluid<Real> &f, hi
 }
 template void launch_mark_phantoms<float>(const Phys &, const Dev &, const Fluid<float> &, hipStream_t);
 template void launch_mark_phantoms<double>(const Phys &, const Dev &, const Fluid<double> &, hipStream_t);
+template <class Real>
+void launch_sur_shell_kick(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream_t st) {
+    LAUNCH(k_sur_shell_kick<Real>, ph.n_tiles, 256, 0, st, ph, d, f);
+}
+template void launch_sur_shell_kick<float>(const Phys &, const Dev &, const Fluid<float> &, hipStream_t);
+template void launch_sur_shell_kick<double>(const Phys &, const Dev &, const Fluid<double> &, hipStream_t);
 void launch_debug_geometry(int n, const double *a, double L, const int *ia, int Li, int b, double *img_out,
                            double *mod_out, int *wrap_out, int *div_out, hipStream_t st) {
     LAUNCH(k_debug_geometry, (n + 255) / 256, 256, 0, st, n, a, L, ia, Li, b, img_out, mod_out, wrap_out, div_out);

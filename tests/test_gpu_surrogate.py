@@ -124,3 +124,36 @@ def test_surrogate_drives_colloids_inside_the_training_range():
     assert s.status() == 0
     _, _, mode = s.surrogate_data()
     assert not mode.any()
+
+
+@pytest.mark.parametrize("precision", [B.BW_FP64, B.BW_FP32], ids=["fp64", "fp32"])
+def test_surrogate_closure_conserves_momentum(precision):
+    """With the surrogate attached, a network-driven colloid's sampled impulse,
+    its run() kick and the momentum of the fluid mirrored out of it are paid
+    by the fluid in its shell: total momentum is conserved step by step as in
+    the explicit method (cpp/fluid_colloid_collision.cpp:46-47, cpp/run.cpp:14-24).
+    FP64 storage: to 1e-10 of the momentum scale.  FP32 storage: to the
+    random walk of its roundings, a few 2^-24 |v| per particle and step."""
+    s = B.System(**dict(PRM, precision=precision))
+    s.startup()
+    X, T = S.collect(s, 100)
+    net, info = S.fit(X, T, epochs=10, batch=64, lr=1e-3, seed=2)
+    S.attach(s, net, info)
+    s.step(1)
+    assert s.status() == 0
+    p0 = s.audit()
+    nf = 16 ** 3 * 10
+    driven = 0
+    for k in range(1, 9):
+        s.step(1)
+        assert s.status() == 0
+        _, _, mode = s.surrogate_data()
+        driven += int(mode.sum())
+        a = s.audit()
+        drift = np.abs(a["momentum"] - p0["momentum"]).max() / a["momentum_scale"]
+        if precision == B.BW_FP64:
+            bound = 1e-10
+        else:
+            bound = 4 * 2.0 ** -24 * np.sqrt(3 * nf * 3 * k) * 3.0 / a["momentum_scale"]
+        assert drift <= bound, (k, drift, bound)
+    assert driven > 0  # the network drove colloids in these steps
