@@ -18,6 +18,7 @@ namespace bw {
 // kernel launchers (kernels.hip)
 size_t srd_lds_bytes(int ncell, int cap, bool fp64);
 size_t srd_static_lds();
+size_t srd_sur_lds();
 bool srd_needs_wrap(int L, int bmax);
 template <class Real> void launch_depart(const Phys &, const Dev &, const Fluid<Real> &, hipStream_t);
 template <class Real>
@@ -344,7 +345,7 @@ int enqueue_step(bw_ctx *c, int cur, int cb, bool final, Prof *prof) {
     if (prof) prof->mark(st, BW_KERNEL_COLLOID);
     launch_depart<Real>(ph, d0, src, st);
     if (prof) prof->mark(st, BW_KERNEL_DEPART);
-    launch_srd<Real>(ph, d0, src, dst, col, false, c->block, c->srd_lds, st);
+    launch_srd<Real>(ph, d0, src, dst, col, false, c->block, c->srd_lds + (ph.surrogate ? srd_sur_lds() : 0), st);
     if (prof) prof->mark(st, BW_KERNEL_SRD);
     if (col && ph.surrogate) launch_sur_shell_kick<Real>(ph, dev_at(c, 1 - cur, cb), dst, st);
     for (int l = 0; l < ph.n_sub; l++) {
@@ -643,7 +644,7 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         (rc = dalloc(c, &d.ev_total, n_rep)) ||
         (rc = dalloc(c, &d.cmom, c->moments_ok ? (size_t)cells / 8 * 5 : 1)) || (rc = dalloc(c, &d.mom_valid, 1)) ||
         (rc = dalloc(c, &d.cmode, NC)) || (rc = dalloc(c, &d.cmode_prev, NC)) || (rc = dalloc(c, &d.csur, 6 * NC)) || (rc = dalloc(c, &d.cimp, 6 * NC)) ||
-        (rc = dalloc(c, &d.sshell, NC)) || (rc = dalloc(c, &d.smir, 3 * NC)) ||
+        (rc = dalloc(c, &d.sshell, NC)) || (rc = dalloc(c, &d.sacc, SUR_ACC * NC)) ||
         (rc = dalloc(c, &d.feat, NC * N_FEAT)) || (rc = dalloc(c, &d.sur_out, NC * 12)) ||
         (rc = dalloc(c, &d.cvel, 3 * NC)) || (rc = dalloc(c, &d.comega, 3 * NC)) ||
         (rc = dalloc(c, &d.cf, 3 * NC)) || (rc = dalloc(c, &d.coldf, 3 * NC)) || (rc = dalloc(c, &d.cra, 3 * NC)) ||
@@ -908,7 +909,7 @@ BW_API int initialize(bw_ctx *c) {
     HIPCHK(c, hipMemsetAsync(c->dv.cmode, 0, sizeof(int) * NC, c->st));
     HIPCHK(c, hipMemsetAsync(c->dv.cmode_prev, 0, sizeof(int) * NC, c->st));
     HIPCHK(c, hipMemsetAsync(c->dv.sshell, 0, sizeof(int) * NC, c->st));
-    HIPCHK(c, hipMemsetAsync(c->dv.smir, 0, sizeof(unsigned long long) * 3 * NC, c->st));
+    HIPCHK(c, hipMemsetAsync(c->dv.sacc, 0, sizeof(unsigned long long) * SUR_ACC * NC, c->st));
     HIPCHK(c, hipMemsetAsync(c->dv.mom_valid, 0, sizeof(int), c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->cb = 0;
@@ -1566,6 +1567,8 @@ BW_API int bw_surrogate_attach(bw_ctx *c, bw_mlp *m, const float *lo, const floa
     }
     if (!lo || !hi || !t_lo || !t_hi) return BW_ERR_ARG;
     if (!c->moments_ok) return fail(c, BW_ERR_ARG, "the surrogate needs even box and tile edges");
+    if (c->srd_lds + srd_sur_lds() > 160 * 1024 - srd_static_lds())
+        return fail(c, BW_ERR_ARG, "tiles too large for the rotation pass's surrogate data in LDS");
     if (c->nc_total == 0) return fail(c, BW_ERR_ARG, "no colloids");
     if (mlp_buf_alloc(c->mbuf, c->nc_total)) return fail(c, BW_ERR_HIP, "hipMalloc (surrogate activations)");
     for (int k = 0; k < 2; k++) {

@@ -18,6 +18,7 @@ constexpr int CHAIN_CAP = 1024;     // serially replayed particles per replica p
 constexpr int CONT_CAP = 512;       // continuations per replica per substep
 constexpr int SERIAL_CAP = 2048;    // serial list incl. entries of overflowed buckets
 constexpr int TCL_CAP = 64;         // colloids listed for one tile
+constexpr int SUR_ACC = 15;         // fixed-point sums per surrogate-driven colloid and step
 constexpr int MASK_BITS = 32;       // listed colloids covered by the LDS stencil masks
 constexpr int AXIS_MAX = 64;        // cells per axis of a tile's region (box grown by TILE_REACH)
 constexpr int CELL_BITS = 10;       // packed cell coordinates: L <= 1023
@@ -149,11 +150,13 @@ struct Dev {
     double *cimp;              // per colloid: collision impulse accumulated since the features (6)
     // per surrogate-driven colloid and step, formed by the rotation pass: the
     // fluid in its shell (distance <= sigma/2 + 0.5 from the centre), which
-    // pays its sampled impulse, its run() kick and the mirrored fluid's
-    // momentum change, and that momentum change (3, fixed point 2^-40, in
-    // units of the fluid mass)
+    // pays the colloid's sampled impulse and torque, its run() kick and the
+    // mirrored fluid's momentum changes; and 15 fixed-point (2^-40) sums in
+    // units of the fluid mass: the mirrored fluid's momentum change (3) and
+    // angular momentum change about the centre (3), and the shell's first (3)
+    // and second (xx yy zz xy xz yz) moments of position about the centre
     int *sshell;
-    unsigned long long *smir;
+    unsigned long long *sacc;
     float *feat;               // per colloid: surrogate features (144)
     float *sur_out;            // per colloid: impulse mean (6) and log-variance (6)
     const float *sur_lo, *sur_hi;  // feature range of the training data (144 each), then

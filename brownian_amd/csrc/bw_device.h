@@ -413,6 +413,15 @@ __device__ __forceinline__ void wave_sync() {
 
 // build_tile_list (explicit colloids) by one wave; the list also goes to
 // lds_g.  Returns the list length.
+// The calling wave sorts list[0..n) (n <= 64, distinct values) ascending in
+// place: each lane's rank is the number of smaller values.
+__device__ inline void wave_sort_list(int *list, int n) {
+    const int lane = threadIdx.x & 63;
+    const int v = lane < n ? list[lane] : 0x7fffffff;
+    int rank = 0;
+    for (int j = 0; j < n; j++) rank += lane_value(v, j) < v ? 1 : 0;
+    if (lane < n) list[rank] = v;
+}
 __device__ inline int build_tile_list_wave(const Phys &ph, const Dev &d, const Rep &R, int T, int *lds_g) {
     const int lane = threadIdx.x & 63;
     int lo[3], ln[3];
@@ -446,12 +455,15 @@ struct TileCore {
     double px[TCL_CAP], py[TCL_CAP], pz[TCL_CAP];
     int cx[TCL_CAP], cy[TCL_CAP], cz[TCL_CAP];
 };
-// ... and their run() data (the passes that count shells or kick)
-struct TileColl : TileCore {
+// ... and their run() data (the passes that count shells)
+struct TileRun : TileCore {
     double rx[TCL_CAP], ry[TCL_CAP], rz[TCL_CAP];    // position at run() time
     int qx[TCL_CAP], qy[TCL_CAP], qz[TCL_CAP];       // its cell
     double vx[TCL_CAP], vy[TCL_CAP], vz[TCL_CAP];    // velocity after the kick
-    double kx[TCL_CAP], ky[TCL_CAP], kz[TCL_CAP];    // per-particle kick
+};
+// ... and the per-particle kick (the passes that kick)
+struct TileColl : TileRun {
+    double kx[TCL_CAP], ky[TCL_CAP], kz[TCL_CAP];
 };
 
 // listed colloids' current and run() positions relative to a tile's origin,
@@ -467,11 +479,15 @@ __device__ __forceinline__ void load_tile_slot(const Dev &d, const Phys &ph, int
     tc.g[slot] = g;
     tc.px[slot] = d.cpos[3 * g]; tc.py[slot] = d.cpos[3 * g + 1]; tc.pz[slot] = d.cpos[3 * g + 2];
     tc.cx[slot] = d.ccell[3 * g]; tc.cy[slot] = d.ccell[3 * g + 1]; tc.cz[slot] = d.ccell[3 * g + 2];
-    if constexpr (!std::is_same<TileCore, TC>::value) {
+    if constexpr (std::is_base_of<TileRun, TC>::value) {
         if (kick_data) {
             tc.rx[slot] = d.cpos_run[3 * g]; tc.ry[slot] = d.cpos_run[3 * g + 1]; tc.rz[slot] = d.cpos_run[3 * g + 2];
             tc.qx[slot] = d.ccell_run[3 * g]; tc.qy[slot] = d.ccell_run[3 * g + 1]; tc.qz[slot] = d.ccell_run[3 * g + 2];
             tc.vx[slot] = d.cvel[3 * g]; tc.vy[slot] = d.cvel[3 * g + 1]; tc.vz[slot] = d.cvel[3 * g + 2];
+        }
+    }
+    if constexpr (std::is_same<TileColl, TC>::value) {
+        if (kick_data) {
             // cpp/run.cpp:14,23: del*(mass_colloid/(mass_fl*cnt)) with del = ra*v0
             int c = d.ccnt[g];
             double fac = c > 0 ? ph.mass_colloid / (ph.mass_fl * c) : 0.0;
@@ -551,14 +567,14 @@ __device__ inline void build_axis_masks(int n, const int *cx, const int *cy, con
     }
     __syncthreads();
 }
-__device__ inline void build_axis_masks(const TileColl &tc, const Region &g, int L, bool run_cells, int reach,
+__device__ inline void build_axis_masks(const TileRun &tc, const Region &g, int L, bool run_cells, int reach,
                                         AxisMask &am) {
     if (run_cells) build_axis_masks(tc.n, tc.qx, tc.qy, tc.qz, g, L, reach, am);
     else build_axis_masks(tc.n, tc.cx, tc.cy, tc.cz, g, L, reach, am);
 }
 // both mask sets of a tile with one pair of barriers: run() cells within
 // reach0 into am0, current cells within reach1 into am1 (as two calls would)
-__device__ inline void build_axis_masks2(const TileColl &tc, const Region &g, int L, int reach0, AxisMask &am0,
+__device__ inline void build_axis_masks2(const TileRun &tc, const Region &g, int L, int reach0, AxisMask &am0,
                                          int reach1, AxisMask &am1) {
     for (int i = threadIdx.x; i < 3 * AXIS_MAX; i += blockDim.x) {
         (&am0.m[0][0])[i] = 0u;
@@ -583,7 +599,7 @@ __device__ inline void build_axis_masks2(const TileColl &tc, const Region &g, in
 
 // run() rear-shell test of listed colloid k for a particle at binned cell
 // (ix,iy,iz) and absolute position (x,y,z) (cpp/run.cpp:17-18)
-__device__ __forceinline__ bool shell_test(const Phys &ph, const TileColl &tc, int k, int L, int ix, int iy,
+__device__ __forceinline__ bool shell_test(const Phys &ph, const TileRun &tc, int k, int L, int ix, int iy,
                                            int iz, double x, double y, double z) {
     if (!in_stencil(ix, iy, iz, tc.qx[k], tc.qy[k], tc.qz[k], L)) return false;
     double Ld = (double)L;
@@ -669,6 +685,67 @@ __device__ inline void colloid_force_serial(const Phys &ph, const Dev &d, double
     }
     f[0] = f0; f[1] = f1; f[2] = f2;
 }
+// fixed-point (2^-40) value of a and back; |a| >= 2^22 counts as 0
+__device__ __forceinline__ unsigned long long fixq(double a) {
+    return fabs(a) < 0x1p22 ? (unsigned long long)__double2ll_rn(a * 0x1p40) : 0ull;
+}
+__device__ __forceinline__ double unfixq(unsigned long long q) { return (double)(long long)q * 0x1p-40; }
+
+// How the fluid in a surrogate-driven colloid's shell pays the colloid's
+// debt of the step: linear  D = n_sub * impulse + mirrored momentum change +
+// run() kick (cpp/run.cpp:14), angular  A = n_sub * torque + mirrored angular
+// momentum change.  Shell particle i (at r_i from the centre) gets
+// dv_i = -u + alpha x r_i, with u and alpha chosen so that the shell's
+// momentum changes by -D and its angular momentum about the centre by -A, as
+// the explicit collisions conserve both (cpp/fluid_colloid_collision.cpp:46-47):
+//   (tr C - C) alpha = -A + (S1 x D) / n,   u = (D + alpha x S1) / n,
+// C = S2 - S1 S1^T / n the shell's second moment about its centroid.  lin:
+// the shell holds fluid; ang: its inertia is invertible (else alpha = 0 and
+// the colloid takes the mirrored angular momentum's negative as its torque).
+struct ShellPay {
+    double u[3], a[3];
+    bool lin, ang;
+};
+__device__ inline ShellPay shell_pay(const Phys &ph, const Dev &d, int g) {
+    ShellPay p;
+    const int cnt = d.sshell[g];
+    const unsigned long long *q = d.sacc + (size_t)SUR_ACC * g;
+    const double ratio = ph.mass_colloid / ph.mass_fl, v0 = d.cv0[g], ns = (double)ph.n_sub;
+    double D[3], A[3], s1[3];
+    for (int k = 0; k < 3; k++) {
+        D[k] = (ns * d.csur[6 * g + k] + unfixq(q[k])) + ratio * (d.cra[3 * g + k] * v0);
+        A[k] = ns * d.csur[6 * g + 3 + k] + unfixq(q[3 + k]);
+        s1[k] = unfixq(q[6 + k]);
+        p.u[k] = 0.0;
+        p.a[k] = 0.0;
+    }
+    p.lin = cnt > 0;
+    p.ang = false;
+    if (!p.lin) return p;
+    const double n = (double)cnt;
+    const double cxx = unfixq(q[9]) - s1[0] * s1[0] / n, cyy = unfixq(q[10]) - s1[1] * s1[1] / n,
+                 czz = unfixq(q[11]) - s1[2] * s1[2] / n, cxy = unfixq(q[12]) - s1[0] * s1[1] / n,
+                 cxz = unfixq(q[13]) - s1[0] * s1[2] / n, cyz = unfixq(q[14]) - s1[1] * s1[2] / n;
+    const double tr = (cxx + cyy) + czz;
+    const double m00 = tr - cxx, m11 = tr - cyy, m22 = tr - czz, m01 = -cxy, m02 = -cxz, m12 = -cyz;
+    const double c00 = m11 * m22 - m12 * m12, c01 = m02 * m12 - m01 * m22, c02 = m01 * m12 - m02 * m11;
+    const double c11 = m00 * m22 - m02 * m02, c12 = m01 * m02 - m00 * m12, c22 = m00 * m11 - m01 * m01;
+    const double det = (m00 * c00 + m01 * c01) + m02 * c02;
+    p.ang = cnt >= 3 && tr > 0.0 && det > 1e-6 * tr * tr * tr;
+    if (p.ang) {
+        const double b0 = -A[0] + (s1[1] * D[2] - s1[2] * D[1]) / n;
+        const double b1 = -A[1] + (s1[2] * D[0] - s1[0] * D[2]) / n;
+        const double b2 = -A[2] + (s1[0] * D[1] - s1[1] * D[0]) / n;
+        p.a[0] = ((c00 * b0 + c01 * b1) + c02 * b2) / det;
+        p.a[1] = ((c01 * b0 + c11 * b1) + c12 * b2) / det;
+        p.a[2] = ((c02 * b0 + c12 * b1) + c22 * b2) / det;
+    }
+    p.u[0] = (D[0] + (p.a[1] * s1[2] - p.a[2] * s1[1])) / n;
+    p.u[1] = (D[1] + (p.a[2] * s1[0] - p.a[0] * s1[2])) / n;
+    p.u[2] = (D[2] + (p.a[0] * s1[1] - p.a[1] * s1[0])) / n;
+    return p;
+}
+
 struct ColIn {  // the inputs of colloid_update, loaded ahead of the force terms
     double v[3], w[3], ra[3], f[3], of[3], p0[3], dv[3], dw[3], sv[6], imp[6], disp[3], pmd[3], v0;
     int cell[3], cmd[3];
@@ -694,10 +771,17 @@ __device__ inline void colloid_load(const Phys &ph, const Dev &d, int g, unsigne
         in.sv[k] = ((flags & PH_APPLY) && ph.surrogate) ? d.csur[6 * g + k] : 0.0;
         in.imp[k] = (flags & PH_APPLY) ? d.cimp[6 * g + k] : 0.0;
     }
-    // a surrogate-driven colloid whose shell held no fluid cannot pass its
-    // impulse on: it takes back the mirrored fluid's momentum change instead
-    if (in.sur && d.sshell[g] == 0)
-        for (int k = 0; k < 3; k++) in.sv[k] = -((double)(long long)d.smir[3 * g + k] * 0x1p-40) / ph.n_sub;
+    // a surrogate-driven colloid whose shell cannot take its impulse (no
+    // fluid) or its torque (no invertible inertia) takes the mirrored fluid's
+    // momentum or angular momentum change back instead
+    if (in.sur) {
+        const ShellPay pay = shell_pay(ph, d, g);
+        const unsigned long long *q = d.sacc + (size_t)SUR_ACC * g;
+        for (int k = 0; k < 3; k++) {
+            if (!pay.lin) in.sv[k] = -unfixq(q[k]) / ph.n_sub;
+            if (!pay.ang) in.sv[3 + k] = -unfixq(q[3 + k]) / ph.n_sub;
+        }
+    }
     in.v0 = (flags & PH_KICK) ? d.cv0[g] : 0.0;
 }
 // The per-colloid phases of one substep, in PH_* order, by one thread, on the
@@ -762,7 +846,7 @@ __device__ inline void colloid_update(const Phys &ph, const Dev &d, int L, int r
         }
         d.ccnt[g] = 0;
         d.sshell[g] = 0;
-        d.smir[3 * g] = 0; d.smir[3 * g + 1] = 0; d.smir[3 * g + 2] = 0;
+        for (int k = 0; k < SUR_ACC; k++) d.sacc[(size_t)SUR_ACC * g + k] = 0;
     }
     if (flags & PH_OLDF)
         for (int k = 0; k < 3; k++) of[k] = f[k];

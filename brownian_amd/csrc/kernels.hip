@@ -192,6 +192,14 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
     const int per = ((n + NW - 1) / NW + 63) & ~63;
     const int lo = min(n, w * per), hi = min(n, lo + per);
     for (int t = lane; t < 27; t += 64) s_cnt[w][t] = 0;
+    if (w == NW - 1) {
+        // the step's tile colloid lists, appended in arbitrary order, sorted:
+        // every later pass meets a tile's colloids in ascending index, the
+        // reference's colloid order (a particle in two run() shells is kicked
+        // in that order, cpp/run.cpp:12-24)
+        wave_sort_list(d.tcl_list + (size_t)T * TCL_CAP, min(d.tcl_cnt[T], TCL_CAP));
+        if (ph.surrogate) wave_sort_list(d.tsl_list + (size_t)T * TCL_CAP, min(d.tsl_cnt[T], TCL_CAP));
+    }
     bool drift = false;
     // pass 1: the departure direction of each particle, counts per wave (the
     // particles that stay in the tile, most of them, are counted by ballot
@@ -312,15 +320,15 @@ __device__ __forceinline__ unsigned long long fixq_diff(double a, double b) {
     const long long qb = fabs(b) < 1024.0 ? __double2ll_rn(b * 0x1p40) : 0;
     return (unsigned long long)(qa - qb);
 }
-// a tile's surrogate shell counts and mirrored momentum into the per-colloid
+// a tile's surrogate shell counts and fixed-point sums into the per-colloid
 // totals (integer sums: the same in any order)
 __device__ inline void flush_surrogate_shells(const Dev &d, int ns, const int *sg, const int *scnt,
-                                              const unsigned long long (*smir)[3]) {
+                                              const unsigned long long (*sacc)[SUR_ACC]) {
     for (int q = threadIdx.x; q < ns; q += blockDim.x) {
         const int g = sg[q];
         if (scnt[q]) atomicAdd(&d.sshell[g], scnt[q]);
-        for (int k = 0; k < 3; k++)
-            if (smir[q][k]) atomicAdd(&d.smir[3 * g + k], smir[q][k]);
+        for (int k = 0; k < SUR_ACC; k++)
+            if (sacc[q][k]) atomicAdd(&d.sacc[(size_t)SUR_ACC * g + k], sacc[q][k]);
     }
 }
 // bytes of the SRD pass's per-cell region (7 doubles per cell), which also
@@ -329,6 +337,18 @@ __host__ __device__ inline size_t srd_cell_region(int ncell, int cap) {
     const size_t a = (size_t)ncell * 56, b = ((size_t)cap * 4 + 7) / 8 * 8;
     return a > b ? a : b;
 }
+
+// dynamic LDS of the SRD pass for tiles of ncell cells holding up to cap
+// particles (the surrogate's colloid data follows, 16-byte aligned)
+__host__ __device__ size_t srd_lds_bytes(int ncell, int cap, bool fp64) {
+    if (!fp64)  // count, smallest id, exponent; 5 + 3 fixed-point sums, mean (3 FP64), matrix (9 FP32)
+        return 8 * (((size_t)3 * ncell + 1) / 2) + (size_t)ncell * (64 + 24 + 36) + 16;
+    return srd_cell_region(ncell, cap) + 3 * (size_t)cap * 8 + (size_t)(2 * ncell + 1) * 4 + (size_t)cap * 2 + 16;
+}
+// surrogate-driven colloids of a tile: position and velocity (6 FP64), the
+// fixed-point sums of shell_pay (SUR_ACC), tile-local position (3 FP32), id,
+// shell count
+constexpr size_t SRD_SUR_LDS = (size_t)TCL_CAP * (48 + 8 * SUR_ACC + 12 + 4 + 4);
 
 // WRAP: the near test applies the periodic wrap to its FP32 differences
 // (srd_needs_wrap); otherwise they stay below L/2, where the wrap is the identity.
@@ -341,14 +361,11 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_pre[28], s_src[27], s_base[27];
     __shared__ int s_wtot[16];
-    __shared__ TileColl tc;
+    __shared__ TileRun tc;
     __shared__ NearColl nc[1];  // used by the near test only
     __shared__ int s_shell[COUNT_SHELL ? TCL_CAP : 1];
     __shared__ AxisMask s_am[COUNT_SHELL ? 2 : 1];  // [0] run shells, [1] near test
     __shared__ int s_cc[SRD_PT][TH / 64];   // near particles per chunk and wave
-    __shared__ double s_sp[COUNT_SHELL ? TCL_CAP : 1][6];  // surrogate colloids: position, velocity
-    __shared__ int s_sg[COUNT_SHELL ? TCL_CAP : 1], s_scnt[COUNT_SHELL ? TCL_CAP : 1];  // id, shell count
-    __shared__ unsigned long long s_smir[COUNT_SHELL ? TCL_CAP : 1][3];  // mirrored momentum (2^-40)
     __shared__ int s_sn;
     const int T = blockIdx.x;
     Rep R;
@@ -380,6 +397,15 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     unsigned long long *f_dp = f_sum + 5 * ncell;                                   // 3 ncell
     double *f_u = (double *)(f_dp + 3 * ncell);                                     // 3 ncell
     float *f_m = (float *)(f_u + 3 * ncell);                                        // 9 ncell
+    // surrogate-driven colloids (past the per-cell region, only while a
+    // surrogate is attached): position and velocity, the fixed-point sums of
+    // shell_pay, tile-local FP32 position, id, shell count
+    unsigned char *sur = smem + ((srd_lds_bytes(ncell, cap, !FAST) + 15) & ~(size_t)15);
+    double(*s_sp)[6] = (double(*)[6])sur;
+    unsigned long long(*s_sacc)[SUR_ACC] = (unsigned long long(*)[SUR_ACC])(sur + (size_t)TCL_CAP * 48);
+    float(*s_spf)[3] = (float(*)[3])(sur + (size_t)TCL_CAP * (48 + 8 * SUR_ACC));
+    int *s_sg = (int *)(sur + (size_t)TCL_CAP * (60 + 8 * SUR_ACC));
+    int *s_scnt = s_sg + TCL_CAP;
 
     const long long step = *d.step;
     const double sx = d.shift[3 * tg.r], sy = d.shift[3 * tg.r + 1], sz = d.shift[3 * tg.r + 2];
@@ -406,8 +432,10 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     if (COUNT_SHELL)
         for (int k = threadIdx.x; k < TCL_CAP; k += blockDim.x) {
             s_shell[k] = 0;
-            s_scnt[k] = 0;
-            s_smir[k][0] = 0; s_smir[k][1] = 0; s_smir[k][2] = 0;
+            if (ph.surrogate) {
+                s_scnt[k] = 0;
+                for (int j = 0; j < SUR_ACC; j++) s_sacc[k][j] = 0;
+            }
         }
     // the tile's colloid list, requested now so that it arrives with the
     // particles (the step's list pass ran before this kernel; a list made
@@ -467,6 +495,10 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 s_sp[q][k] = d.cpos[3 * g + k];
                 s_sp[q][3 + k] = d.cvel[3 * g + k];
             }
+            const double Ld = (double)L;
+            s_spf[q][0] = (float)img1(s_sp[q][0] - ox0, Ld);
+            s_spf[q][1] = (float)img1(s_sp[q][1] - oy0, Ld);
+            s_spf[q][2] = (float)img1(s_sp[q][2] - oz0, Ld);
         }
         if (threadIdx.x == 0) s_sn = ns;
     }
@@ -781,49 +813,94 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 if (ph.surrogate && s_sn) {
                     // fluid that entered a surrogate-driven colloid during the last
                     // step is mirrored back out through its surface, its velocity
-                    // reversed relative to the colloid's; the momentum change is
-                    // summed per colloid, and the fluid in the colloid's shell (on
-                    // the stored position, the test k_sur_shell_kick repeats) is
-                    // counted: it pays the change back with the colloid's sampled
-                    // impulse and run() kick
+                    // reversed relative to the colloid's; its momentum and angular
+                    // momentum changes are summed per colloid, and the fluid in the
+                    // colloid's shell (on the stored position, the test
+                    // k_sur_shell_kick repeats) is counted with its moments of
+                    // position: it pays them back with the colloid's sampled impulse
+                    // and torque and its run() kick (shell_pay).  An FP32 test in
+                    // tile-local coordinates (2e-2 slack) picks the colloids that get
+                    // the exact tests.
                     int ix, iy, iz;
                     unpack_cell(rcell[k], ix, iy, iz);
-                    const double Ld = (double)L, R2 = 2.0 * sqrt(ph.lim_coll);  // colloid diameter
-                    const double x = (double)ix + (double)rox[k], y = (double)iy + (double)roy[k],
-                                 z = (double)iz + (double)roz[k];
-                    for (int q = 0; q < s_sn; q++) {
-                        double ax = img1(x - s_sp[q][0], Ld), ay = img1(y - s_sp[q][1], Ld), az = img1(z - s_sp[q][2], Ld);
-                        double r2 = (ax * ax + ay * ay) + az * az;
-                        if (r2 >= ph.lim_coll) continue;
-                        double rr = sqrt(r2);
-                        double sc = rr > 1e-12 ? (R2 - rr) / rr : 0.0;  // radius rr -> 2R - rr
-                        double nx = s_sp[q][0] + (rr > 1e-12 ? ax * sc : 0.5 * R2 + 1e-6);
-                        double ny = s_sp[q][1] + ay * sc, nz = s_sp[q][2] + az * sc;
-                        nx = mod1(nx, Ld); ny = mod1(ny, Ld); nz = mod1(nz, Ld);
-                        int jx = coord_cell(nx, L), jy = coord_cell(ny, L), jz = coord_cell(nz, L);
-                        rcell[k] = pack_cell(jx, jy, jz) | (rcell[k] & PHANTOM);
-                        rox[k] = (Real)(nx - jx); roy[k] = (Real)(ny - jy); roz[k] = (Real)(nz - jz);
-                        if (!(rox[k] < Real(1))) rox[k] = Real(0);
-                        if (!(roy[k] < Real(1))) roy[k] = Real(0);
-                        if (!(roz[k] < Real(1))) roz[k] = Real(0);
-                        const Real wx = (Real)(2.0 * s_sp[q][3] - (double)rvx[k]);
-                        const Real wy = (Real)(2.0 * s_sp[q][4] - (double)rvy[k]);
-                        const Real wz = (Real)(2.0 * s_sp[q][5] - (double)rvz[k]);
-                        atomicAdd(&s_smir[q][0], fixq_diff(wx, rvx[k]));
-                        atomicAdd(&s_smir[q][1], fixq_diff(wy, rvy[k]));
-                        atomicAdd(&s_smir[q][2], fixq_diff(wz, rvz[k]));
-                        rvx[k] = wx; rvy[k] = wy; rvz[k] = wz;
-                        moved = true;  // streamed per substep
-                        break;
+                    unsigned long long cand = 0;
+                    {
+                        const float Lf = (float)L, iLf = 1.0f / Lf;
+                        const float px = (float)pdiff(ix, ox0, L) + (float)rox[k];
+                        const float py = (float)pdiff(iy, oy0, L) + (float)roy[k];
+                        const float pz = (float)pdiff(iz, oz0, L) + (float)roz[k];
+                        const float rs = sqrtf((float)ph.lim_shell) + 0.02f, rs2 = rs * rs;
+                        for (int q = 0; q < s_sn; q++) {
+                            float ax = px - s_spf[q][0], ay = py - s_spf[q][1], az = pz - s_spf[q][2];
+                            if constexpr (WRAP) {
+                                ax -= Lf * rintf(ax * iLf); ay -= Lf * rintf(ay * iLf); az -= Lf * rintf(az * iLf);
+                            }
+                            if ((ax * ax + ay * ay) + az * az <= rs2) cand |= 1ull << q;
+                        }
                     }
-                    unpack_cell(rcell[k], ix, iy, iz);
-                    const double x1 = (double)ix + (double)rox[k], y1 = (double)iy + (double)roy[k],
-                                 z1 = (double)iz + (double)roz[k];
-                    for (int q = 0; q < s_sn; q++) {
-                        double ax = img1(x1 - s_sp[q][0], Ld), ay = img1(y1 - s_sp[q][1], Ld), az = img1(z1 - s_sp[q][2], Ld);
-                        if ((ax * ax + ay * ay) + az * az <= ph.lim_shell) {
-                            atomicAdd(&s_scnt[q], 1);
-                            shell = true;  // kicked in the first substep
+                    if (cand) {
+                        const double Ld = (double)L, R2 = 2.0 * sqrt(ph.lim_coll);  // colloid diameter
+                        const double x = (double)ix + (double)rox[k], y = (double)iy + (double)roy[k],
+                                     z = (double)iz + (double)roz[k];
+                        for (unsigned long long m = cand; m; m &= m - 1) {
+                            const int q = __ffsll((long long)m) - 1;
+                            double ax = img1(x - s_sp[q][0], Ld), ay = img1(y - s_sp[q][1], Ld), az = img1(z - s_sp[q][2], Ld);
+                            double r2 = (ax * ax + ay * ay) + az * az;
+                            if (r2 >= ph.lim_coll) continue;
+                            // radius rr -> 2R - rr (reflected through the surface), at
+                            // most R + 0.5 (the shell's edge) for deep entries
+                            double rr = sqrt(r2);
+                            double sc = rr > 1e-12 ? (fmin(R2 - rr, 0.5 * R2 + 0.5) - rr) / rr : 0.0;
+                            double nx = s_sp[q][0] + (rr > 1e-12 ? ax * sc : 0.5 * R2 + 1e-6);
+                            double ny = s_sp[q][1] + ay * sc, nz = s_sp[q][2] + az * sc;
+                            nx = mod1(nx, Ld); ny = mod1(ny, Ld); nz = mod1(nz, Ld);
+                            int jx = coord_cell(nx, L), jy = coord_cell(ny, L), jz = coord_cell(nz, L);
+                            rcell[k] = pack_cell(jx, jy, jz) | (rcell[k] & PHANTOM);
+                            rox[k] = (Real)(nx - jx); roy[k] = (Real)(ny - jy); roz[k] = (Real)(nz - jz);
+                            if (!(rox[k] < Real(1))) rox[k] = Real(0);
+                            if (!(roy[k] < Real(1))) roy[k] = Real(0);
+                            if (!(roz[k] < Real(1))) roz[k] = Real(0);
+                            const Real wx = (Real)(2.0 * s_sp[q][3] - (double)rvx[k]);
+                            const Real wy = (Real)(2.0 * s_sp[q][4] - (double)rvy[k]);
+                            const Real wz = (Real)(2.0 * s_sp[q][5] - (double)rvz[k]);
+                            atomicAdd(&s_sacc[q][0], fixq_diff(wx, rvx[k]));
+                            atomicAdd(&s_sacc[q][1], fixq_diff(wy, rvy[k]));
+                            atomicAdd(&s_sacc[q][2], fixq_diff(wz, rvz[k]));
+                            {  // angular momentum change about the centre, at the new position
+                                const double dvx = (double)wx - (double)rvx[k], dvy = (double)wy - (double)rvy[k],
+                                             dvz = (double)wz - (double)rvz[k];
+                                const double px = img1(((double)jx + (double)rox[k]) - s_sp[q][0], Ld),
+                                             py = img1(((double)jy + (double)roy[k]) - s_sp[q][1], Ld),
+                                             pz = img1(((double)jz + (double)roz[k]) - s_sp[q][2], Ld);
+                                atomicAdd(&s_sacc[q][3], fixq(py * dvz - pz * dvy));
+                                atomicAdd(&s_sacc[q][4], fixq(pz * dvx - px * dvz));
+                                atomicAdd(&s_sacc[q][5], fixq(px * dvy - py * dvx));
+                            }
+                            rvx[k] = wx; rvy[k] = wy; rvz[k] = wz;
+                            moved = true;  // streamed per substep
+                            break;
+                        }
+                        // a mirrored particle is tested against every listed colloid
+                        if (moved) cand = s_sn >= 64 ? ~0ull : ((1ull << s_sn) - 1ull);
+                        unpack_cell(rcell[k], ix, iy, iz);
+                        const double x1 = (double)ix + (double)rox[k], y1 = (double)iy + (double)roy[k],
+                                     z1 = (double)iz + (double)roz[k];
+                        for (unsigned long long m = cand; m; m &= m - 1) {
+                            const int q = __ffsll((long long)m) - 1;
+                            double ax = img1(x1 - s_sp[q][0], Ld), ay = img1(y1 - s_sp[q][1], Ld), az = img1(z1 - s_sp[q][2], Ld);
+                            if ((ax * ax + ay * ay) + az * az <= ph.lim_shell) {
+                                atomicAdd(&s_scnt[q], 1);
+                                atomicAdd(&s_sacc[q][6], fixq(ax));
+                                atomicAdd(&s_sacc[q][7], fixq(ay));
+                                atomicAdd(&s_sacc[q][8], fixq(az));
+                                atomicAdd(&s_sacc[q][9], fixq(ax * ax));
+                                atomicAdd(&s_sacc[q][10], fixq(ay * ay));
+                                atomicAdd(&s_sacc[q][11], fixq(az * az));
+                                atomicAdd(&s_sacc[q][12], fixq(ax * ay));
+                                atomicAdd(&s_sacc[q][13], fixq(ax * az));
+                                atomicAdd(&s_sacc[q][14], fixq(ay * az));
+                                shell = true;  // kicked in the first substep
+                            }
                         }
                     }
                 }
@@ -900,7 +977,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         if (COUNT_SHELL) {  // the apply phase's shell counts are complete
             for (int q = threadIdx.x; q < tc.n; q += blockDim.x)
                 if (s_shell[q]) atomicAdd(&d.ccnt[tc.g[q]], s_shell[q]);
-            if (ph.surrogate) flush_surrogate_shells(d, s_sn, s_sg, s_scnt, s_smir);
+            if (ph.surrogate) flush_surrogate_shells(d, s_sn, s_sg, s_scnt, s_sacc);
         }
         int base = 0;
 #pragma unroll
@@ -1019,7 +1096,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         __syncthreads();
         for (int q = threadIdx.x; q < tc.n; q += blockDim.x)
             if (s_shell[q]) atomicAdd(&d.ccnt[tc.g[q]], s_shell[q]);
-        if (ph.surrogate) flush_surrogate_shells(d, s_sn, s_sg, s_scnt, s_smir);
+        if (ph.surrogate) flush_surrogate_shells(d, s_sn, s_sg, s_scnt, s_sacc);
     }
 }
 
@@ -2151,31 +2228,25 @@ __global__ void k_sur_sample(Phys ph, Dev d, const int *colloid_rep) {
 
 // First substep, surrogate-driven colloids: the fluid in a colloid's shell
 // (distance from its centre <= sigma/2 + 0.5, counted by the rotation pass on
-// the same stored positions) pays the colloid's momentum debt of the step --
-// its sampled impulse, its run() kick (cpp/run.cpp:14, whose rear shell the
-// explicit path kicks) and the momentum change of the fluid mirrored out of
-// it -- in equal shares, so that the step conserves momentum as the explicit
-// collisions do (cpp/fluid_colloid_collision.cpp:46-47).  Shell particles were
-// made near, so they are among the first near_cnt of their tile; kicks of
-// several colloids are summed in FP64 and rounded once.  A colloid with an
-// empty shell receives the mirrored momentum's negative instead of its
-// impulse (colloid_load).  One workgroup per tile.
+// the same stored positions) pays the colloid's momentum and angular momentum
+// debt of the step (shell_pay): particle i at r_i from the centre gets
+// dv_i = -u + alpha x r_i.  Shell particles were made near, so they are among
+// the first near_cnt of their tile; kicks of several colloids are summed in
+// FP64 and rounded once.  One workgroup per tile.
 template <class Real>
 __global__ __launch_bounds__(256) void k_sur_shell_kick(Phys ph, Dev d, Fluid<Real> f) {
-    __shared__ double s_k[TCL_CAP][6];  // position, velocity decrement
+    __shared__ double s_k[TCL_CAP][9];  // position, u, alpha
     const int T = blockIdx.x;
     const int ns = min(d.tsl_cnt[T], TCL_CAP);
     if (ns == 0) return;
     const int L = d.rep[d.tile_rep[T]].L;
     for (int q = threadIdx.x; q < ns; q += blockDim.x) {
         const int g = d.tsl_list[T * TCL_CAP + q];
-        const int cnt = d.sshell[g];
-        const double ratio = ph.mass_colloid / ph.mass_fl, v0 = d.cv0[g];
+        const ShellPay pay = shell_pay(ph, d, g);
         for (int k = 0; k < 3; k++) {
             s_k[q][k] = d.cpos[3 * g + k];
-            const double debt = ((double)ph.n_sub * d.csur[6 * g + k] + (double)(long long)d.smir[3 * g + k] * 0x1p-40) +
-                                ratio * (d.cra[3 * g + k] * v0);
-            s_k[q][3 + k] = cnt > 0 ? debt / cnt : 0.0;
+            s_k[q][3 + k] = pay.u[k];
+            s_k[q][6 + k] = pay.a[k];
         }
     }
     __syncthreads();
@@ -2190,11 +2261,12 @@ __global__ __launch_bounds__(256) void k_sur_shell_kick(Phys ph, Dev d, Fluid<Re
         double vx = f.vx[slot], vy = f.vy[slot], vz = f.vz[slot];
         bool hit = false;
         for (int q = 0; q < ns; q++) {
-            double ax = img1(x - s_k[q][0], Ld), ay = img1(y - s_k[q][1], Ld), az = img1(z - s_k[q][2], Ld);
+            const double ax = img1(x - s_k[q][0], Ld), ay = img1(y - s_k[q][1], Ld), az = img1(z - s_k[q][2], Ld);
             if ((ax * ax + ay * ay) + az * az <= ph.lim_shell) {
-                vx = vx - s_k[q][3];
-                vy = vy - s_k[q][4];
-                vz = vz - s_k[q][5];
+                const double *a = s_k[q] + 6;
+                vx = vx - s_k[q][3] + (a[1] * az - a[2] * ay);
+                vy = vy - s_k[q][4] + (a[2] * ax - a[0] * az);
+                vz = vz - s_k[q][5] + (a[0] * ay - a[1] * ax);
                 hit = true;
             }
         }
@@ -2295,13 +2367,10 @@ __global__ void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, lon
 // -------------------------------------------------------- launch wrappers
 #define LAUNCH(kern, grid, block, lds, st, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, st, __VA_ARGS__)
 
-size_t srd_lds_bytes(int ncell, int cap, bool fp64) {
-    if (!fp64)  // count, smallest id, exponent; 5 + 3 fixed-point sums, mean (3 FP64), matrix (9 FP32)
-        return 8 * (((size_t)3 * ncell + 1) / 2) + (size_t)ncell * (64 + 24 + 36) + 16;
-    return srd_cell_region(ncell, cap) + 3 * (size_t)cap * 8 + (size_t)(2 * ncell + 1) * 4 + (size_t)cap * 2 + 16;
-}
 // static LDS of the SRD kernel's largest variant (colloid lists, masks)
-size_t srd_static_lds() { return sizeof(TileColl) + sizeof(NearColl) + 4 * TCL_CAP + 2 * sizeof(AxisMask) + 512; }
+size_t srd_static_lds() { return sizeof(TileRun) + sizeof(NearColl) + 4 * TCL_CAP + 2 * sizeof(AxisMask) + 512; }
+// dynamic LDS the SRD kernel adds while a surrogate is attached
+size_t srd_sur_lds() { return SRD_SUR_LDS + 16; }
 
 template <class Real>
 void launch_depart(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream_t st) {
