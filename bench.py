@@ -17,6 +17,13 @@ built from oracle/oracle.c at -O2 and -O1) timed on one host core on a bounded
 sample of the same state (rotation + run() + the first substeps, extrapolated
 to one MPCD step).
 
+The batched ensemble (256 replicas, boxes 16-48, 1-64 colloids, v0 up to
+0.1) is timed with the tile shape its box edges share tuned on the whole
+ensemble, and compared with the surrogate closure trained on 2M pairs from
+8 single-system-style replicas (64^3, 300 colloids): colloid-steps/s, and
+from one start-up state the plain and surrogate runs' pooled MSD and
+orientation autocorrelation at 100..1000 steps with paired standard errors.
+
   python bench.py [--steps K --warmup W --L 128 --ncol 2400]
   python bench.py --pmc-summary FETCH_DIR WRITE_DIR --out FILE   (rocprofv3 --pmc outputs -> bytes per launch)
 """
@@ -196,11 +203,18 @@ def ensemble_params(n_rep, seed0=1000):
     return reps
 
 
-def run_ensemble(B, n_rep, steps, warmup):
-    reps = ensemble_params(n_rep)
+def ensemble_replicas(B, n_rep, tile=None, seed0=1000):
+    common = dict(precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
+                  orient_law=B.BW_ORIENT_SPHERE, fluid_init=B.BW_INIT_DEVICE)
+    if tile is not None:
+        common["tile"] = tuple(tile)
+    return [dict(r, **common) for r in ensemble_params(n_rep, seed0)]
+
+
+def run_ensemble(B, n_rep, steps, warmup, tile=None):
+    reps = ensemble_replicas(B, n_rep, tile)
     t0 = time.time()
-    s = B.System(replicas=[dict(r, precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
-                                orient_law=B.BW_ORIENT_SPHERE, fluid_init=B.BW_INIT_DEVICE) for r in reps])
+    s = B.System(replicas=reps)
     s.startup()
     s.sync()
     t_setup = time.time() - t0
@@ -212,59 +226,46 @@ def run_ensemble(B, n_rep, steps, warmup):
     ms = s.time_steps(steps)
     s.status()
     near = sum(s.near_particles(r) for r in range(n_rep))
+    prof = s.profile_steps(3)
+    s.status()
     step_s = ms / 1e3 / steps
     log(f"ensemble: {steps} timed steps: {ms / steps:.3f} ms/step")
     return {"workload": f"{n_rep} replicas, L~U{{16,24,32,40,48}} x 10/cell, colloids U{{1..cap(L)}} (4 % cap), "
                         f"v0~U[0,0.1], seeds 1000+r, 20 substeps, FP32, one set of launches per step",
             "replicas": n_rep, "n_fluid": nf, "n_colloid": nc, "steps": steps, "ms_per_step": ms / steps,
+            "tile": list(s.info[0]["tile"]),
             "fluid_particle_steps_per_s": nf / step_s, "colloid_steps_per_s": nc / step_s,
-            "near_fraction": near / nf, "setup_s": t_setup}
+            "near_fraction": near / nf, "setup_s": t_setup,
+            "kernels_ms_per_step": {k: v[0] / 3 for k, v in prof.items()}}
 
 
-def run_surrogate(B, n_rep, collect_steps, epochs, steps):
-    """Plain vs surrogate-driven ensemble from one start: colloid-steps/s and
-    the pooled colloid mean squared displacement after `steps` steps.  The
-    network is trained on pairs from an ensemble with disjoint seeds."""
+def run_surrogate(B, n_rep, n_pairs, epochs, lags, tile=None):
+    """The surrogate closure as specified: trained on n_pairs (features,
+    impulse) pairs from 8 single-system-style replicas (64^3 x 10, 300
+    colloids, v0 0.02, seeds 5000+r), run inside the batched ensemble (seeds
+    1000+r), against the plain ensemble from the same start: colloid-steps/s,
+    pooled MSD and orientation autocorrelation at each lag, diffusivity (MSD
+    slope) ratio, standard errors from the replicas as paired blocks."""
     from brownian_amd import surrogate as S
     common = dict(precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
                   orient_law=B.BW_ORIENT_SPHERE, fluid_init=B.BW_INIT_DEVICE)
     t0 = time.time()
-    tr = B.System(replicas=[dict(r, **common) for r in ensemble_params(n_rep, seed0=5000)])
-    tr.startup()
-    X, T = S.collect(tr, collect_steps)
-    del tr
+    X, T = S.collect_pairs(lambda: B.System(replicas=S.c1_replicas(8, **common)), n_pairs)
     log(f"surrogate: {len(X):,} training pairs collected ({time.time() - t0:.1f} s)")
     t1 = time.time()
     net, info = S.fit(X, T, epochs=epochs, batch=4096, lr=1e-3, seed=1)
-    log(f"surrogate: trained {epochs} epochs in {time.time() - t1:.1f} s, held-out NLL "
+    t_train = time.time() - t1
+    log(f"surrogate: trained {epochs} epochs in {t_train:.1f} s, held-out NLL "
         f"{info['val_loss_before']:.3f} -> {info['val_loss_after']:.3f}")
-    reps = [dict(r, **common) for r in ensemble_params(n_rep)]
-    out = {"training_pairs": int(len(X)), "epochs": epochs, "val_nll_before": info["val_loss_before"],
-           "val_nll_after": info["val_loss_after"], "train_nll": info["train_loss"]}
-    res = {}
-    for name in ("plain", "surrogate"):
-        s = B.System(replicas=reps)
-        s.startup()
-        if name == "surrogate":
-            S.attach(s, net, info)
-        s.step(3)
-        s.status()
-        s.observables_reset()
-        ms = s.time_steps(steps)
-        s.status()
-        nc = np.array([i["nc"] for i in s.info])
-        msd = np.array([s.observables(r)["msd"] for r in range(n_rep)])
-        frac = None
-        if name == "surrogate":
-            frac = float(np.mean(np.concatenate([s.surrogate_data(r)[2] for r in range(n_rep)])))
-        res[name] = {"ms_per_step": ms / steps, "colloid_steps_per_s": nc.sum() / (ms / 1e3 / steps),
-                     "msd": float((nc * msd).sum() / nc.sum()), "surrogate_fraction": frac}
-        log(f"surrogate: {name} {ms / steps:.3f} ms/step, pooled MSD {res[name]['msd']:.4f}")
-        del s
-    out.update(res)
-    out["speedup_colloid_steps"] = res["surrogate"]["colloid_steps_per_s"] / res["plain"]["colloid_steps_per_s"]
-    out["msd_ratio"] = res["surrogate"]["msd"] / res["plain"]["msd"]
-    out["msd_lag_steps"] = steps
+    reps = ensemble_replicas(B, n_rep, tile)
+    out = S.compare_ensemble(lambda: B.System(replicas=reps), net, info, None, lags, log=log)
+    out.update({"training_pairs": int(len(X)), "training_systems": "8 x (64^3 x 10, 300 colloids, v0 0.02), "
+                "seeds 5000-5007", "epochs": epochs, "train_s": t_train,
+                "val_nll_before": info["val_loss_before"], "val_nll_after": info["val_loss_after"]})
+    m = out["msd"][-1]
+    log(f"surrogate: MSD ratio at {lags[-1]} steps {m['ratio']:.4f} +- {m['ratio_se']:.4f}, diffusivity ratio "
+        f"{out['diffusivity']['ratio']:.4f} +- {out['diffusivity']['ratio_se']:.4f}, colloid-steps/s x"
+        f"{out['speedup_colloid_steps']:.3f}")
     return out
 
 
@@ -282,14 +283,15 @@ def main():
     ap.add_argument("--streaming", choices=["blocked", "per-substep"], default="blocked")
     ap.add_argument("--copy-reps", type=int, default=10)
     ap.add_argument("--ensemble", type=int, default=256, help="replicas of the batched run (0 = skip)")
-    ap.add_argument("--ensemble-steps", type=int, default=50)
+    ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True)
     ap.add_argument("--tile", type=int, nargs=3, metavar=("BX", "BY", "BZ"),
                     help="tile shape (skips the autotuner)")
     ap.add_argument("--surrogate", action=argparse.BooleanOptionalAction, default=True)
-    ap.add_argument("--surrogate-collect", type=int, default=40)
+    ap.add_argument("--surrogate-pairs", type=int, default=2_000_000)
     ap.add_argument("--surrogate-epochs", type=int, default=8)
-    ap.add_argument("--surrogate-steps", type=int, default=100)
+    ap.add_argument("--surrogate-lags", default="100,200,500,1000",
+                    help="steps at which the plain and surrogate ensembles are compared")
     ap.add_argument("--plain-steps", type=int, default=10, help="timed steps of per-substep streaming (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-ref", action="store_true", help="also time the compiled reference (oracle/_ref)")
@@ -427,14 +429,28 @@ def main():
                    "sample": "failed: " + str(r.get("error"))}
     ens = None
     if args.ensemble > 0:
+        ens_tile, ens_tune = None, None
         try:
-            ens = run_ensemble(B, args.ensemble, args.ensemble_steps, 5)
+            if args.autotune:
+                from brownian_amd.autotune import autotune_ensemble
+                log("autotuning the ensemble's tile shape")
+                ens_tune = autotune_ensemble(ensemble_replicas(B, args.ensemble), log=log)
+                ens_tile = ens_tune["best"]
+            ens = run_ensemble(B, args.ensemble, args.ensemble_steps, 5, ens_tile)
+            if ens_tune:
+                ens["autotune"] = {"best": list(ens_tune["best"]),
+                                   "ms_per_step": {str(k): v for k, v in ens_tune["ms_per_step"].items()}}
         except Exception as e:  # reported, the single-system figure stands
             ens = {"error": repr(e)}
         if args.surrogate:
             try:
-                ens["surrogate"] = run_surrogate(B, args.ensemble, args.surrogate_collect, args.surrogate_epochs,
-                                                 args.surrogate_steps)
+                lags = [int(x) for x in args.surrogate_lags.split(",")]
+                sur = run_surrogate(B, args.ensemble, args.surrogate_pairs, args.surrogate_epochs, lags, ens_tile)
+                # the plain ensemble's pooled statistics over the run (same start, replicas as blocks)
+                ens["pooled_plain"] = {"lags": sur["lags"],
+                                       "msd": [[m["plain"], m["plain_se"]] for m in sur["msd"]],
+                                       "orient_corr": [[m["plain"], m["plain_se"]] for m in sur["orient_corr"]]}
+                ens["surrogate"] = sur
             except Exception as e:
                 ens["surrogate"] = {"error": repr(e)}
     ach = kern[rk]["achieved_GBps"] * 1e9

@@ -348,6 +348,15 @@ class System:
         self._chk(self.lib.bw_get_surrogate_data(self.h, replica, f.ctypes.data_as(_fp), _ptr(imp), _ptr(mode, _ip)))
         return f, imp, mode
 
+    def surrogate_shells(self, replica=0):
+        """after a fused step: shell fluid count (n_colloid) and the 15 sums
+        its payment was formed from (n_colloid x 15)"""
+        nc = self.info[replica]["nc"]
+        cnt = np.zeros(nc, np.int32)
+        sums = np.zeros((nc, 15))
+        self._chk(self.lib.bw_get_surrogate_shells(self.h, replica, _ptr(cnt, _ip), _ptr(sums)))
+        return cnt, sums
+
     def near_particles(self, replica=0) -> int:
         n = C.c_longlong(0)
         self._chk(self.lib.bw_near_particles(self.h, replica, C.byref(n)))

@@ -87,3 +87,32 @@ def autotune(params: dict, steps: int = 4, warmup: int = 2, shapes=None, v0=None
             log(f"tile {shape}: {ms:.3f} ms/step ({time.time() - t0:.1f} s)")
     best = min(times, key=times.get)
     return {"best": best, "ms_per_step": times}
+
+
+def common_candidates(Ls, density: int = 10, fp64: bool = False) -> list[tuple[int, int, int]]:
+    """Tile shapes every box edge of an ensemble accepts (one slot capacity
+    and one SRD workgroup size serve all its replicas)."""
+    sets = [set(candidates(L, density, fp64)) for L in sorted(set(Ls))]
+    return sorted(set.intersection(*sets), key=lambda s: (s[0] * s[1] * s[2], s))
+
+
+def autotune_ensemble(replicas: list, steps: int = 5, warmup: int = 2, shapes=None, log=None) -> dict:
+    """Time `steps` fused steps of the whole ensemble (replica params as for
+    System(replicas=...)) for each tile shape its box edges all accept;
+    returns {"best": shape, "ms_per_step": {shape: ms}}."""
+    fp64 = replicas[0].get("precision", 0) == BW_FP64
+    shapes = shapes or common_candidates([r["L"] for r in replicas], replicas[0].get("density", 10), fp64)
+    times = {}
+    for shape in shapes:
+        t0 = time.time()
+        s = System(replicas=[dict(r, tile=shape) for r in replicas])
+        s.startup()
+        s.step(warmup)
+        s.status()
+        times[shape] = s.time_steps(steps) / steps
+        s.status()
+        s.close()
+        if log:
+            log(f"ensemble tile {shape}: {times[shape]:.3f} ms/step ({time.time() - t0:.1f} s)")
+    best = min(times, key=times.get)
+    return {"best": best, "ms_per_step": times}

@@ -1608,6 +1608,22 @@ BW_API int bw_get_surrogate_data(bw_ctx *c, int r, float *feat, double *imp, int
     return BW_OK;
 }
 
+BW_API int bw_get_surrogate_shells(bw_ctx *c, int r, int *count, double *sums) {
+    ReplicaHost *R = rep_of(c, r);
+    if (!R) return BW_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (!R->nc) return BW_OK;
+    size_t g0 = (size_t)R->first_colloid;
+    if (count) HIPCHK(c, hipMemcpy(count, c->dv.sshell + g0, sizeof(int) * R->nc, hipMemcpyDeviceToHost));
+    if (sums) {
+        std::vector<unsigned long long> q((size_t)SUR_ACC * R->nc);
+        HIPCHK(c, hipMemcpy(q.data(), c->dv.sacc + g0 * SUR_ACC, sizeof(unsigned long long) * q.size(),
+                            hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < q.size(); i++) sums[i] = (double)(long long)q[i] * 0x1p-40;
+    }
+    return BW_OK;
+}
+
 BW_API int bw_near_particles(bw_ctx *c, int r, long long *n_near) {
     ReplicaHost *R = rep_of(c, r);
     if (!R || !n_near) return BW_ERR_ARG;

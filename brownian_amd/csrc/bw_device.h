@@ -700,8 +700,14 @@ __device__ __forceinline__ double unfixq(unsigned long long q) { return (double)
 // the explicit collisions conserve both (cpp/fluid_colloid_collision.cpp:46-47):
 //   (tr C - C) alpha = -A + (S1 x D) / n,   u = (D + alpha x S1) / n,
 // C = S2 - S1 S1^T / n the shell's second moment about its centroid.  lin:
-// the shell holds fluid; ang: its inertia is invertible (else alpha = 0 and
-// the colloid takes the mirrored angular momentum's negative as its torque).
+// the shell holds at least SHELL_MIN_LIN particles and no share exceeds
+// SHELL_MAX_KICK; ang: besides, at least SHELL_MIN_ANG particles, an inertia
+// within a factor ~30 of isotropic and a rotation share below SHELL_MAX_KICK
+// at the shell's edge.  Otherwise the part is not paid by the shell (u or
+// alpha 0) and the colloid takes the mirrored fluid's momentum or angular
+// momentum change back instead of its impulse or torque.
+constexpr int SHELL_MIN_LIN = 8, SHELL_MIN_ANG = 16;
+constexpr double SHELL_MAX_KICK = 2.0;  // velocity units (the thermal speed is 1)
 struct ShellPay {
     double u[3], a[3];
     bool lin, ang;
@@ -719,7 +725,7 @@ __device__ inline ShellPay shell_pay(const Phys &ph, const Dev &d, int g) {
         p.u[k] = 0.0;
         p.a[k] = 0.0;
     }
-    p.lin = cnt > 0;
+    p.lin = cnt >= SHELL_MIN_LIN;
     p.ang = false;
     if (!p.lin) return p;
     const double n = (double)cnt;
@@ -731,7 +737,7 @@ __device__ inline ShellPay shell_pay(const Phys &ph, const Dev &d, int g) {
     const double c00 = m11 * m22 - m12 * m12, c01 = m02 * m12 - m01 * m22, c02 = m01 * m12 - m02 * m11;
     const double c11 = m00 * m22 - m02 * m02, c12 = m01 * m02 - m00 * m12, c22 = m00 * m11 - m01 * m01;
     const double det = (m00 * c00 + m01 * c01) + m02 * c02;
-    p.ang = cnt >= 3 && tr > 0.0 && det > 1e-6 * tr * tr * tr;
+    p.ang = cnt >= SHELL_MIN_ANG && tr > 0.0 && det > 0.01 * tr * tr * tr;
     if (p.ang) {
         const double b0 = -A[0] + (s1[1] * D[2] - s1[2] * D[1]) / n;
         const double b1 = -A[1] + (s1[2] * D[0] - s1[0] * D[2]) / n;
@@ -739,10 +745,23 @@ __device__ inline ShellPay shell_pay(const Phys &ph, const Dev &d, int g) {
         p.a[0] = ((c00 * b0 + c01 * b1) + c02 * b2) / det;
         p.a[1] = ((c01 * b0 + c11 * b1) + c12 * b2) / det;
         p.a[2] = ((c02 * b0 + c12 * b1) + c22 * b2) / det;
+        const double am = sqrt((p.a[0] * p.a[0] + p.a[1] * p.a[1]) + p.a[2] * p.a[2]);
+        if (!(am * sqrt(ph.lim_shell) <= SHELL_MAX_KICK)) {
+            p.ang = false;
+            p.a[0] = 0.0; p.a[1] = 0.0; p.a[2] = 0.0;
+        }
     }
     p.u[0] = (D[0] + (p.a[1] * s1[2] - p.a[2] * s1[1])) / n;
     p.u[1] = (D[1] + (p.a[2] * s1[0] - p.a[0] * s1[2])) / n;
     p.u[2] = (D[2] + (p.a[0] * s1[1] - p.a[1] * s1[0])) / n;
+    if (!(sqrt((p.u[0] * p.u[0] + p.u[1] * p.u[1]) + p.u[2] * p.u[2]) <= SHELL_MAX_KICK)) {
+        p.lin = false;
+        p.ang = false;
+        for (int k = 0; k < 3; k++) {
+            p.u[k] = 0.0;
+            p.a[k] = 0.0;
+        }
+    }
     return p;
 }
 

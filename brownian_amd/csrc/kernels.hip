@@ -345,6 +345,8 @@ __host__ __device__ size_t srd_lds_bytes(int ncell, int cap, bool fp64) {
         return 8 * (((size_t)3 * ncell + 1) / 2) + (size_t)ncell * (64 + 24 + 36) + 16;
     return srd_cell_region(ncell, cap) + 3 * (size_t)cap * 8 + (size_t)(2 * ncell + 1) * 4 + (size_t)cap * 2 + 16;
 }
+// deepest entry into a surrogate-driven colloid that is mirrored back out
+constexpr double MIRROR_DEPTH = 0.5;
 // surrogate-driven colloids of a tile: position and velocity (6 FP64), the
 // fixed-point sums of shell_pay (SUR_ACC), tile-local position (3 FP32), id,
 // shell count
@@ -847,11 +849,19 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                             double ax = img1(x - s_sp[q][0], Ld), ay = img1(y - s_sp[q][1], Ld), az = img1(z - s_sp[q][2], Ld);
                             double r2 = (ax * ax + ay * ay) + az * az;
                             if (r2 >= ph.lim_coll) continue;
-                            // radius rr -> 2R - rr (reflected through the surface), at
-                            // most R + 0.5 (the shell's edge) for deep entries
+                            // a phantom collides with nothing; a particle deeper than
+                            // MIRROR_DEPTH (one that crossed a colloid while it
+                            // collided explicitly, or came in faster than the step
+                            // allows) is left inside as a phantom; the others are
+                            // reflected through the surface, radius rr -> 2R - rr
+                            if (rcell[k] & PHANTOM) break;
                             double rr = sqrt(r2);
-                            double sc = rr > 1e-12 ? (fmin(R2 - rr, 0.5 * R2 + 0.5) - rr) / rr : 0.0;
-                            double nx = s_sp[q][0] + (rr > 1e-12 ? ax * sc : 0.5 * R2 + 1e-6);
+                            if (rr < 0.5 * R2 - MIRROR_DEPTH) {
+                                rcell[k] |= PHANTOM;
+                                break;
+                            }
+                            double sc = (R2 - rr) / rr;
+                            double nx = s_sp[q][0] + ax * sc;
                             double ny = s_sp[q][1] + ay * sc, nz = s_sp[q][2] + az * sc;
                             nx = mod1(nx, Ld); ny = mod1(ny, Ld); nz = mod1(nz, Ld);
                             int jx = coord_cell(nx, L), jy = coord_cell(ny, L), jz = coord_cell(nz, L);

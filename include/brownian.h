@@ -309,6 +309,13 @@ BW_API int bw_surrogate_attach(bw_ctx *ctx, bw_mlp *m, const float *lo, const fl
 /* after a fused step: features at its start (n_colloid x 144), collision
  * impulse over it (n_colloid x 6) and mode (0 explicit, 1 surrogate) */
 BW_API int bw_get_surrogate_data(bw_ctx *ctx, int replica, float *feat, double *imp, int *mode);
+/* after a fused step, per colloid of the replica that the network drove: the
+ * fluid particles in its shell, which paid its impulse (count, n_colloid), and
+ * the sums the payment was formed from (n_colloid x 15, fluid-mass units): the
+ * mirrored fluid's momentum change (3) and angular momentum change about the
+ * centre (3), the shell's first (3) and second (xx yy zz xy xz yz) moments of
+ * position about the centre.  Zero for explicitly colliding colloids. */
+BW_API int bw_get_surrogate_shells(bw_ctx *ctx, int replica, int *count, double *sums);
 
 /* Diagnostics (no counterpart in the reference): particles of replica r that
  * the last fused step's per-substep kernels touched ("near" a colloid under
