@@ -15,6 +15,7 @@ struct MlpBuf {
     int cap = 0;
     unsigned short *Xb = nullptr, *Hb[3] = {}, *dYb = nullptr, *dZb = nullptr;
     float *Z[3] = {}, *H[3] = {}, *Y = nullptr, *dY = nullptr, *dH = nullptr, *dZ = nullptr, *loss = nullptr;
+    float *part = nullptr;  // partial sums of the backward reductions (split-K products, column sums)
 };
 int mlp_buf_alloc(MlpBuf &b, int rows);  // hipMalloc; 0 on success
 void mlp_buf_free(MlpBuf &b);

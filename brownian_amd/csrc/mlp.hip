@@ -1,9 +1,9 @@
 // Surrogate closure network of libbrownian: a multilayer perceptron
 //   144 -> 256 -> 256 -> 256 -> 12, SiLU hidden activations,
 // run and trained on the GPU with its own kernels.  Matrix products use the
-// CDNA matrix cores (V_MFMA_F32_32X32X8_BF16: BF16 operands, FP32
-// accumulation); weights are kept in FP32 (Adam state too) with BF16 copies
-// for the products.  The output is the per-interval colloid impulse mean (6)
+// gfx950 matrix cores (V_MFMA_F32_32X32X16_BF16: BF16 operands, FP32
+// accumulation, operands staged through LDS); weights are kept in FP32 (Adam
+// state too) with BF16 copies for the products.  The output is the per-interval colloid impulse mean (6)
 // and log-variance (6) in standardised units; the loss is the Gaussian
 // negative log-likelihood.
 //
@@ -25,7 +25,6 @@
 
 namespace bw {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // FP32 -> BF16, round to nearest even (NaN kept quiet)
@@ -36,42 +35,84 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
     return (unsigned short)(u >> 16);
 }
 
-// C (M x N) = op(A) (M x K) . op(B) (K x N), FP32 accumulation on the matrix
-// cores; one wave per 32 x 32 tile of C, four waves per workgroup.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// C (M x N) = op(A) (M x K) . op(B) (K x N) on the gfx950 matrix cores
+// (V_MFMA_F32_32X32X16_BF16: BF16 operands, FP32 accumulation).  One
+// workgroup of four waves per 64 x 64 block of C, each wave a 32 x 32
+// quadrant; K in steps of 32 staged through LDS by 128-bit global loads (a
+// transposed operand is transposed on its way into LDS), A as As[m][k] and B
+// as Bs[n][k], so that a lane's 8 consecutive k are one 128-bit LDS read.
+// M, N and K are multiples of 32; rows or columns of a block past M or N are
+// zero-filled and not stored.
 //   TA = 0: A is M x K row-major (lda = row stride); TA = 1: A is stored K x M.
 //   TB = 0: B is K x N row-major;                    TB = 1: B is stored N x K.
 // EPI 0: C = acc.  EPI 1: Z = acc + bias, C = silu(Z), Cb = bf16(C).
 // EPI 2: C = acc + bias.
-// Operand layout of one 32x32x8 product: lane l supplies A[l%32][4(l/32)+j]
-// and B[4(l/32)+j][l%32], j = 0..3; accumulator element i of lane l is
-// C[8(i/4) + 4(l/32) + i%4][l%32].
+// Operand layout of one 32x32x16 product: lane l supplies A[l%32][k] and
+// B[k][l%32] for 8 values of k selected by l/32 (the same selection for A and
+// B, so that the sum over k does not depend on it); accumulator element i of
+// lane l is C[8(i/4) + 4(l/32) + i%4][l%32].
 template <int TA, int TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const unsigned short *__restrict__ A,
                                               const unsigned short *__restrict__ B, int M, int N, int K, int lda,
                                               int ldb, float *__restrict__ C, int ldc, const float *__restrict__ bias,
                                               float *__restrict__ Z, unsigned short *__restrict__ Cb) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int tn = N / 32;
-    const int t = blockIdx.x * 4 + w;
-    if (t >= (M / 32) * tn) return;
-    const int m0 = (t / tn) * 32, n0 = (t % tn) * 32;
-    const int r = lane & 31, kq = (lane >> 5) * 4;
+    constexpr int BM = 64, BN = 64, BK = 32, LD = BK + 8;  // LDS rows of 40 BF16 (80 bytes, 16-byte aligned)
+    __shared__ __align__(16) unsigned short As[BM * LD], Bs[BN * LD];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int nbn = (N + BN - 1) / BN;
+    const int m0 = (blockIdx.x / nbn) * BM, n0 = (blockIdx.x % nbn) * BN;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;  // the wave's quadrant
+    const int r = lane & 31, kh = (lane >> 5) * 8;
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; i++) acc[i] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += 8) {
-        bf16x4 a, b;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int k = k0 + kq + j;
-            a[j] = (short)(TA ? A[(size_t)k * lda + m0 + r] : A[(size_t)(m0 + r) * lda + k]);
-            b[j] = (short)(TB ? B[(size_t)(n0 + r) * ldb + k] : B[(size_t)k * ldb + n0 + r]);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        // 64 x 32 BF16 of each operand: 256 chunks of 8, one per thread
+        u32x4 va = {0u, 0u, 0u, 0u}, vb = {0u, 0u, 0u, 0u};
+        const int ri = tid >> 2, rk = (tid & 3) * 8;  // row-major chunk: row, first k
+        const int ck = tid >> 3, cm = (tid & 7) * 8;  // transposed chunk: k, first row
+        if (TA == 0) {
+            if (m0 + ri < M) va = *(const u32x4 *)(A + (size_t)(m0 + ri) * lda + k0 + rk);
+        } else {
+            if (m0 + cm < M) va = *(const u32x4 *)(A + (size_t)(k0 + ck) * lda + m0 + cm);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        if (TB == 1) {
+            if (n0 + ri < N) vb = *(const u32x4 *)(B + (size_t)(n0 + ri) * ldb + k0 + rk);
+        } else {
+            if (n0 + cm < N) vb = *(const u32x4 *)(B + (size_t)(k0 + ck) * ldb + n0 + cm);
+        }
+        if (TA == 0) {
+            *(u32x4 *)(As + ri * LD + rk) = va;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                As[(cm + j) * LD + ck] = (unsigned short)(va[j >> 1] >> (16 * (j & 1)));
+        }
+        if (TB == 1) {
+            *(u32x4 *)(Bs + ri * LD + rk) = vb;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                Bs[(cm + j) * LD + ck] = (unsigned short)(vb[j >> 1] >> (16 * (j & 1)));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            const u32x4 a = *(const u32x4 *)(As + (wm + r) * LD + kk + kh);
+            const u32x4 b = *(const u32x4 *)(Bs + (wn + r) * LD + kk + kh);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                          acc, 0, 0, 0);
+        }
+        __syncthreads();
     }
+    if (m0 + wm >= M || n0 + wn >= N) return;
+    const int kq = (lane >> 5) * 4;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const int m = m0 + (i >> 2) * 8 + kq + (i & 3), n = n0 + r;
+        const int m = m0 + wm + (i >> 2) * 8 + kq + (i & 3), n = n0 + wn + r;
         const size_t o = (size_t)m * ldc + n;
         const float v = acc[i];
         if (EPI == 0) {
@@ -146,12 +187,32 @@ __global__ void k_silu_back(const float *__restrict__ dh, const float *__restric
 }
 
 // column sums of a (rows x cols) matrix: bias gradients
-__global__ void k_colsum(const float *__restrict__ g, int rows, int cols, float *__restrict__ out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
-    float s = 0.0f;
-    for (int r = 0; r < rows; r++) s += g[(size_t)r * cols + c];
-    out[c] = s;
+// Column sums of g (rows x cols) in two fixed-order levels: workgroup (x, y)
+// sums rows [y COLSUM_ROWS, (y+1) COLSUM_ROWS) of 64 columns, four row
+// phases per column combined in order, into part[y][col]; k_sum_parts then
+// adds the row blocks in order.
+constexpr int COLSUM_ROWS = 128;
+constexpr int WGRAD_SPLITS = 8;  // chunks of the batch in a weight-gradient product
+__global__ __launch_bounds__(256) void k_colsum_part(const float *__restrict__ g, int rows, int cols,
+                                                     float *__restrict__ part) {
+    __shared__ float s[4][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63), ph = threadIdx.x >> 6;
+    const int r0 = blockIdx.y * COLSUM_ROWS, r1 = min(rows, r0 + COLSUM_ROWS);
+    float a = 0.0f;
+    if (c < cols)
+        for (int r = r0 + ph; r < r1; r += 4) a += g[(size_t)r * cols + c];
+    s[ph][threadIdx.x & 63] = a;
+    __syncthreads();
+    if (threadIdx.x < 64 && c < cols) part[(size_t)blockIdx.y * cols + c] = ((s[0][threadIdx.x] + s[1][threadIdx.x]) +
+                                                                           s[2][threadIdx.x]) + s[3][threadIdx.x];
+}
+// out[i] = sum over p < np of part[p * n + i], in order of p
+__global__ void k_sum_parts(const float *__restrict__ part, int np, long long n, float *__restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a = 0.0f;
+    for (int p = 0; p < np; p++) a += part[(size_t)p * n + i];
+    out[i] = a;
 }
 
 // Adam (Kingma & Ba, 2015) on FP32 master weights; refreshes the BF16 copy.
@@ -227,7 +288,8 @@ int mlp_buf_alloc(MlpBuf &b, int rows) {
     };
     int bad = al(&b.Xb, (size_t)cap * MLP_INP) | al(&b.Y, (size_t)cap * MLP_OUTP) | al(&b.dY, (size_t)cap * MLP_OUTP) |
               al(&b.dYb, (size_t)cap * MLP_OUTP) | al(&b.dH, (size_t)cap * MLP_H) | al(&b.dZ, (size_t)cap * MLP_H) |
-              al(&b.dZb, (size_t)cap * MLP_H) | al(&b.loss, (size_t)cap);
+              al(&b.dZb, (size_t)cap * MLP_H) | al(&b.loss, (size_t)cap) |
+              al(&b.part, std::max((size_t)WGRAD_SPLITS * MLP_H * MLP_H, ((size_t)cap / COLSUM_ROWS + 1) * MLP_H));
     for (int l = 0; l < 3; l++)
         bad |= al(&b.Z[l], (size_t)cap * MLP_H) | al(&b.H[l], (size_t)cap * MLP_H) | al(&b.Hb[l], (size_t)cap * MLP_H);
     hipDeviceSynchronize();
@@ -236,7 +298,7 @@ int mlp_buf_alloc(MlpBuf &b, int rows) {
 }
 void mlp_buf_free(MlpBuf &b) {
     void *ps[] = {b.Xb, b.Y, b.dY, b.dYb, b.dH, b.dZ, b.dZb, b.loss, b.Z[0], b.Z[1], b.Z[2], b.H[0], b.H[1], b.H[2],
-                  b.Hb[0], b.Hb[1], b.Hb[2]};
+                  b.Hb[0], b.Hb[1], b.Hb[2], b.part};
     if (b.cap) hipDeviceSynchronize();
     for (void *q : ps)
         if (q) hipFree(q);
@@ -332,9 +394,31 @@ int ensure_batch(bw_mlp *m, int rows) {
 template <int TA, int TB, int EPI>
 void gemm(bw_mlp *m, const unsigned short *A, const unsigned short *B, int M, int N, int K, int lda, int ldb,
           float *C, int ldc, const float *bias, float *Z, unsigned short *Cb, hipStream_t st) {
-    int tiles = (M / 32) * (N / 32);
-    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), dim3((tiles + 3) / 4), dim3(256), 0, st, A, B, M, N, K, lda, ldb, C,
-                       ldc, bias, Z, Cb);
+    const int blocks = ((M + 63) / 64) * ((N + 63) / 64);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), dim3(blocks), dim3(256), 0, st, A, B, M, N, K, lda, ldb, C, ldc, bias,
+                       Z, Cb);
+}
+
+// Weight gradient C (M x N, ldc = N) = A^T B with A stored K x M and B K x N,
+// K (the batch) long: split into up to WGRAD_SPLITS chunks of whole
+// multiples of 32 rows, one product each into `part`, added in chunk order.
+void gemm_wgrad(const unsigned short *A, const unsigned short *B, int M, int N, int K, float *C, float *part,
+                hipStream_t st) {
+    const int splits = std::max(1, std::min(WGRAD_SPLITS, K / 512));
+    const int kc = (K / 32 + splits - 1) / splits * 32;
+    const int blocks = ((M + 63) / 64) * ((N + 63) / 64);
+    int np = 0;
+    for (int k0 = 0; k0 < K; k0 += kc, np++)
+        hipLaunchKernelGGL((k_gemm<1, 0, 0>), dim3(blocks), dim3(256), 0, st, A + (size_t)k0 * M, B + (size_t)k0 * N,
+                           M, N, std::min(kc, K - k0), M, N, part + (size_t)np * M * N, N, (const float *)nullptr,
+                           (float *)nullptr, (unsigned short *)nullptr);
+    const long long n = (long long)M * N;
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, np, n, C);
+}
+void colsum(const float *g, int rows, int cols, float *out, float *part, hipStream_t st) {
+    const int nb = (rows + COLSUM_ROWS - 1) / COLSUM_ROWS;
+    hipLaunchKernelGGL(k_colsum_part, dim3((cols + 63) / 64, nb), dim3(256), 0, st, g, rows, cols, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3((cols + 255) / 256), dim3(256), 0, st, part, nb, (long long)cols, out);
 }
 
 // forward over rows [0, n) of device features x (n x 144, raw units; idx
@@ -360,9 +444,8 @@ void backward(bw_mlp *m, MlpBuf &B, int n_pad, hipStream_t st) {
     for (int l = MLP_L - 1; l >= 0; l--) {
         const unsigned short *hin = l == 0 ? B.Xb : B.Hb[l - 1];
         // gW_l (K x N) = H_{l-1}^T (K x n_pad) . dZ_l (n_pad x N)
-        gemm<1, 0, 0>(m, hin, dzb, m->K[l], m->N[l], n_pad, m->K[l], m->N[l], m->gW[l], m->N[l], nullptr, nullptr,
-                      nullptr, st);
-        hipLaunchKernelGGL(k_colsum, dim3((m->N[l] + 255) / 256), dim3(256), 0, st, dz, n_pad, m->N[l], m->gb[l]);
+        gemm_wgrad(hin, dzb, m->K[l], m->N[l], n_pad, m->gW[l], B.part, st);
+        colsum(dz, n_pad, m->N[l], m->gb[l], B.part, st);
         if (l == 0) break;
         // dH_{l-1} (n_pad x K) = dZ_l (n_pad x N) . W_l^T (N x K)
         gemm<0, 1, 0>(m, dzb, m->Wb[l], n_pad, m->K[l], m->N[l], m->N[l], m->N[l], B.dH, m->K[l], nullptr, nullptr,
