@@ -46,9 +46,9 @@ void launch_debug_geometry(int n, const double *a, double L, const int *ia, int 
 void launch_copy(const void *, void *, long long, hipStream_t);
 void launch_debug_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st);
 template <class Real>
-void launch_init_fluid(const Phys &ph, const Dev &d, const InitBins &b, int vel_law, long long n_total, double *pos,
-                       double *vel, double *mean, const Fluid<Real> &scratch, const Fluid<Real> &out, int *fill,
-                       int *count, hipStream_t st);
+void launch_init_fluid(const Phys &ph, const Dev &d, const InitBins &b, int vel_law, long long n_total,
+                       long long max_nf, double *pos, double *vel, double *mean, const Fluid<Real> &scratch,
+                       const Fluid<Real> &out, int *fill, int *count, hipStream_t st);
 void launch_updown(const Phys &ph, const Dev &d, int n_rep, const double *pos, const double *vel, double *sums,
                    int *cnt, int *up_cnt, double *v, double *up, hipStream_t st);
 void launch_debug_reflect(const Phys &ph, const Dev &d, int n, long long step, int substep, double L,
@@ -972,12 +972,15 @@ int init_fluid_device(bw_ctx *c) {
     }
     // temporaries of the start-up, in one allocation freed before returning
     const size_t n_list = std::max<size_t>(list.size(), 1);
-    const size_t bytes = sizeof(double) * 3 * c->n_rep + sizeof(long long) * first.size() +
+    long long max_nf = 1;
+    for (const auto &R : c->reps) max_nf = std::max(max_nf, (long long)R.nf);
+    const size_t n_mean = 3 * (size_t)c->n_rep * (1 + (size_t)init_mean_chunks(max_nf));  // means + chunk sums
+    const size_t bytes = sizeof(double) * n_mean + sizeof(long long) * first.size() +
                          sizeof(int) * (G.size() + start.size() + n_list + (size_t)c->ph.n_tiles) + 64;
     char *tmp = nullptr;
     HIPCHK(c, hipMalloc((void **)&tmp, bytes));
     double *mean = (double *)tmp;
-    long long *dfirst = (long long *)(mean + 3 * c->n_rep);
+    long long *dfirst = (long long *)(mean + n_mean);
     int *dG = (int *)(dfirst + first.size());
     int *dstart = dG + G.size(), *dlist = dstart + start.size(), *fill = dlist + n_list;
     HIPCHK(c, hipMemsetAsync(fill, 0, sizeof(int) * (size_t)c->ph.n_tiles, c->st));
@@ -988,7 +991,7 @@ int init_fluid_device(bw_ctx *c) {
     InitBins b{dG, dfirst, dstart, dlist, c->cposb[c->cb]};
     Dev d = dev_for(c, c->cur);
     HIPCHK(c, hipMemsetAsync(c->dv.status, 0, sizeof(unsigned), c->st));
-    launch_init_fluid<Real>(c->ph, d, b, c->reps[0].p.vel_law, c->nf_total, c->stage_pos, c->stage_vel, mean,
+    launch_init_fluid<Real>(c->ph, d, b, c->reps[0].p.vel_law, c->nf_total, max_nf, c->stage_pos, c->stage_vel, mean,
                             fluid_of<Real>(c, 1 - c->cur), fluid_of<Real>(c, c->cur), fill, c->counts[c->cur], c->st);
     HIPCHK(c, hipStreamSynchronize(c->st));
     hipFree(tmp);

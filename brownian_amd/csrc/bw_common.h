@@ -11,6 +11,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <algorithm>
+
 namespace bw {
 
 constexpr int EV_CAP = 64;          // reflection events per colloid per substep
@@ -37,6 +39,9 @@ constexpr double MD_SKIN = 2.0;     // candidate-list skin for the colloid pair 
 constexpr double STREAM_MARGIN = 0.25;  // colloid displacement bound within a step (blocked streaming)
 constexpr int N_FEAT = 144;         // surrogate features per colloid
 constexpr int COARSE_MAX = 64;      // coarse cells of one tile
+constexpr long long MEAN_CHUNK = 65536;  // particles per workgroup of the start-up's mean
+// workgroups per replica of the start-up's mean for replicas of up to max_nf particles
+inline int init_mean_chunks(long long max_nf) { return (int)std::max(1LL, (max_nf + MEAN_CHUNK - 1) / MEAN_CHUNK); }
 
 // colloid phase flags (applied in this order)
 enum : unsigned {

@@ -1954,11 +1954,16 @@ __global__ void k_init_place(Phys ph, Dev d, InitBins b, int vel_law, long long 
     }
 }
 // per replica (one workgroup each): mean velocity, summed in a fixed order
-__global__ __launch_bounds__(256) void k_init_mean(Dev d, const double *vel, double *mean) {
+// per replica mean velocity in two fixed-order levels: workgroup (p, r) sums
+// particles [p MEAN_CHUNK, (p+1) MEAN_CHUNK) of replica r (thread-strided, then
+// a tree), k_init_mean_final adds the chunks in order
+__global__ __launch_bounds__(256) void k_init_mean_part(Dev d, const double *vel, int nchunk, double *part) {
     __shared__ double s[256][3];
-    const Rep R = d.rep[blockIdx.x];
+    const int r = blockIdx.y, p = blockIdx.x;
+    const Rep R = d.rep[r];
+    const long long i0 = (long long)p * MEAN_CHUNK, i1 = min((long long)R.nf, i0 + MEAN_CHUNK);
     double a[3] = {0, 0, 0};
-    for (long long i = threadIdx.x; i < R.nf; i += 256)
+    for (long long i = i0 + threadIdx.x; i < i1; i += 256)
         for (int k = 0; k < 3; k++) a[k] += vel[3 * (R.first_fluid + i) + k];
     for (int k = 0; k < 3; k++) s[threadIdx.x][k] = a[k];
     __syncthreads();
@@ -1967,8 +1972,15 @@ __global__ __launch_bounds__(256) void k_init_mean(Dev d, const double *vel, dou
             for (int k = 0; k < 3; k++) s[threadIdx.x][k] += s[threadIdx.x + h][k];
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; k++) mean[3 * blockIdx.x + k] = s[0][k] / (double)R.nf;
+    if (threadIdx.x < 3) part[3 * ((long long)r * nchunk + p) + threadIdx.x] = s[0][threadIdx.x];
+}
+__global__ void k_init_mean_final(Dev d, const double *part, int nchunk, double *mean) {
+    const int r = blockIdx.x, k = threadIdx.x;
+    if (k >= 3) return;
+    const Rep R = d.rep[r];
+    double a = 0;
+    for (int p = 0; p < nchunk; p++) a += part[3 * ((long long)r * nchunk + p) + k];
+    mean[3 * r + k] = R.nf > 0 ? a / (double)R.nf : 0.0;
 }
 // mean removed; cell + offset; appended to its tile of buffer `f` (any order)
 template <class Real>
@@ -2534,20 +2546,24 @@ void launch_updown(const Phys &ph, const Dev &d, int n_rep, const double *pos, c
     LAUNCH(k_updown_finish, n_rep, 1, 0, st, d, sums, cnt, up_cnt, v, up);
 }
 template <class Real>
-void launch_init_fluid(const Phys &ph, const Dev &d, const InitBins &b, int vel_law, long long n_total, double *pos,
-                       double *vel, double *mean, const Fluid<Real> &scratch, const Fluid<Real> &out, int *fill,
-                       int *count, hipStream_t st) {
+void launch_init_fluid(const Phys &ph, const Dev &d, const InitBins &b, int vel_law, long long n_total,
+                       long long max_nf, double *pos, double *vel, double *mean, const Fluid<Real> &scratch,
+                       const Fluid<Real> &out, int *fill, int *count, hipStream_t st) {
     const unsigned g = (unsigned)((n_total + 255) / 256);
     LAUNCH(k_init_place, g, 256, 0, st, ph, d, b, vel_law, n_total, pos, vel);
-    LAUNCH(k_init_mean, ph.n_rep, 256, 0, st, d, vel, mean);
+    const int nchunk = init_mean_chunks(max_nf);
+    double *part = mean + 3 * ph.n_rep;
+    hipLaunchKernelGGL(k_init_mean_part, dim3(nchunk, ph.n_rep), dim3(256), 0, st, d, vel, nchunk, part);
+    LAUNCH(k_init_mean_final, ph.n_rep, 64, 0, st, d, part, nchunk, mean);
     LAUNCH(k_init_bin<Real>, g, 256, 0, st, ph, d, n_total, pos, vel, mean, scratch, fill);
     LAUNCH(k_init_sort<Real>, ph.n_tiles, 512, 0, st, ph, d, fill, scratch, out, count);
 }
-template void launch_init_fluid<float>(const Phys &, const Dev &, const InitBins &, int, long long, double *, double *,
-                                       double *, const Fluid<float> &, const Fluid<float> &, int *, int *, hipStream_t);
-template void launch_init_fluid<double>(const Phys &, const Dev &, const InitBins &, int, long long, double *,
-                                        double *, double *, const Fluid<double> &, const Fluid<double> &, int *, int *,
-                                        hipStream_t);
+template void launch_init_fluid<float>(const Phys &, const Dev &, const InitBins &, int, long long, long long,
+                                       double *, double *, double *, const Fluid<float> &, const Fluid<float> &, int *,
+                                       int *, hipStream_t);
+template void launch_init_fluid<double>(const Phys &, const Dev &, const InitBins &, int, long long, long long,
+                                        double *, double *, double *, const Fluid<double> &, const Fluid<double> &,
+                                        int *, int *, hipStream_t);
 void launch_debug_philox(int n, const uint4 *ctr, const uint2 *key, uint4 *out, hipStream_t st) {
     LAUNCH(k_debug_philox, (n + 255) / 256, 256, 0, st, n, ctr, key, out);
 }
