@@ -12,6 +12,9 @@
   steps with no status flag, replicas of edge 40 and 48 bit-identical to the
   same systems run alone, and with every v0 set to zero each replica's total
   momentum conserved to 1e-10 of its scale.
+* The single-system configuration (64^3 x 10, 300 colloids, v0 = 0.02):
+  blocked == per-substep streaming, and 100 steps with the thermostat and
+  momentum bookkeeping.
 """
 import sys
 import os
@@ -136,3 +139,36 @@ def test_c3_ensemble_256_replicas():
         if reps[r]["L"] >= 40:
             worst_large = max(worst_large, drift)
     assert worst_large < 1e-10, worst_large
+
+
+def test_c1_single_system_with_activity():
+    """The single-system configuration with its activity (64^3 x 10 =
+    2,621,440 fluid particles, 300 colloids of radius 2, v0 = 0.02, FP32
+    integer-cell storage, fluid placed on the GPU): one step with blocked
+    streaming equals per-substep streaming (velocities and every colloid
+    quantity bit for bit, positions to FP32 rounding); over 100 steps no
+    status flag, the fluid held at kT by the thermostat (1 %), and the total
+    momentum conserved -- run() removes each colloid's kick from the fluid in
+    its rear shell (cpp/run.cpp:14-24) -- to the random walk of the FP32
+    roundings."""
+    from tests.parity import blocked_equals_per_substep
+    prm = dict(L=64, density=10, n_colloid=300, sig_colloid=4.2, sigma=4.0, mass_colloid=50.0, n_sub=20, ndt=0.1,
+               theta_deg=90.0, v0=0.02, seed=6401, precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE,
+               vel_law=B.BW_VEL_MAXWELL, orient_law=B.BW_ORIENT_SPHERE, fluid_init=B.BW_INIT_DEVICE)
+    a = B.System(**prm)
+    b = B.System(**dict(prm, streaming=B.BW_STREAM_PER_SUBSTEP))
+    a.startup(); b.startup()
+    dp = blocked_equals_per_substep(a, b, 64, pos_tol=2e-6)
+    print(f"64^3 blocked vs per-substep: largest position difference {dp:.2e}")
+    b.close()
+    before = a.audit()
+    a.step(100)
+    assert a.status() == 0
+    after = a.audit()
+    nf = a.info[0]["nf"]
+    obs_t = after["ke_fluid"] * 2.0 / (3.0 * nf)
+    assert abs(obs_t - 1.0) < 1e-2, obs_t
+    drift = np.abs(after["momentum"] - before["momentum"]).max() / after["momentum_scale"]
+    bound = 4 * 2.0 ** -24 * np.sqrt(3 * nf * 100 * 3) / after["momentum_scale"]
+    print(f"64^3, 100 steps: T_fluid {obs_t:.5f}, momentum drift {drift:.2e} (bound {bound:.2e})")
+    assert drift <= bound, (drift, bound)
