@@ -549,49 +549,63 @@ __device__ __forceinline__ unsigned cell_mask(const AxisMask &am, const Region &
     if ((ux | uy | uz) < 0) return 0xffffffffu;
     return am.m[0][ux] & am.m[1][uy] & am.m[2][uz];
 }
-// run_cells: the colloids' cells at run() time, else their current cells
-// masks of the cells (cx, cy, cz)[k] (current or run-time cells of the listed
-// colloids)
-__device__ inline void build_axis_masks(int n, const int *cx, const int *cy, const int *cz, const Region &g, int L,
-                                        int reach, AxisMask &am) {
+__device__ inline void zero_axis_masks(AxisMask &am) {
     for (int i = threadIdx.x; i < 3 * AXIS_MAX; i += blockDim.x) (&am.m[0][0])[i] = 0u;
-    __syncthreads();
+}
+// masks of the cells (cx, cy, cz)[k] of the first n listed colloids within
+// `reach` into masks zeroed before the last barrier; ends with a barrier
+__device__ inline void fill_axis_masks(int n, const int *cx, const int *cy, const int *cz, const Region &g, int L,
+                                       int reach, AxisMask &am) {
+    // the region's bounds as six scalars picked by selects (an array indexed
+    // at run time would live in scratch memory, whose loads wait behind every
+    // outstanding global load of the wave)
+    const int lo0 = g.lo[0], lo1 = g.lo[1], lo2 = g.lo[2], len0 = g.len[0], len1 = g.len[1], len2 = g.len[2];
     const int nk = min(n, MASK_BITS);
     for (int w = threadIdx.x; w < nk * 3; w += blockDim.x) {
         const int k = w / 3, a = w - 3 * k;
         const int c = a == 0 ? cx[k] : a == 1 ? cy[k] : cz[k];
+        const int lo = a == 0 ? lo0 : a == 1 ? lo1 : lo2, len = a == 0 ? len0 : a == 1 ? len1 : len2;
         for (int dd = -reach; dd <= reach; dd++) {
-            int u = axis_index(g, a, wrapi(c + dd, L), L);
-            if (u >= 0) atomicOr(&am.m[a][u], 1u << k);
+            int u = wrapi(c + dd, L) - lo;
+            if (u < 0) u += L;
+            else if (u >= L) u -= L;
+            if (u < len) atomicOr(&am.m[a][u], 1u << k);
         }
     }
     __syncthreads();
 }
+__device__ inline void build_axis_masks(int n, const int *cx, const int *cy, const int *cz, const Region &g, int L,
+                                        int reach, AxisMask &am) {
+    zero_axis_masks(am);
+    __syncthreads();
+    fill_axis_masks(n, cx, cy, cz, g, L, reach, am);
+}
+// run_cells: the colloids' cells at run() time, else their current cells
 __device__ inline void build_axis_masks(const TileRun &tc, const Region &g, int L, bool run_cells, int reach,
                                         AxisMask &am) {
     if (run_cells) build_axis_masks(tc.n, tc.qx, tc.qy, tc.qz, g, L, reach, am);
     else build_axis_masks(tc.n, tc.cx, tc.cy, tc.cz, g, L, reach, am);
 }
-// both mask sets of a tile with one pair of barriers: run() cells within
-// reach0 into am0, current cells within reach1 into am1 (as two calls would)
-__device__ inline void build_axis_masks2(const TileRun &tc, const Region &g, int L, int reach0, AxisMask &am0,
-                                         int reach1, AxisMask &am1) {
-    for (int i = threadIdx.x; i < 3 * AXIS_MAX; i += blockDim.x) {
-        (&am0.m[0][0])[i] = 0u;
-        (&am1.m[0][0])[i] = 0u;
-    }
-    __syncthreads();
+// both mask sets of a tile in one pass: run() cells within reach0 into am0,
+// current cells within reach1 into am1 (as two fills would), into masks
+// zeroed before the last barrier; ends with a barrier
+__device__ inline void fill_axis_masks2(const TileRun &tc, const Region &g, int L, int reach0, AxisMask &am0,
+                                        int reach1, AxisMask &am1) {
+    const int lo0 = g.lo[0], lo1 = g.lo[1], lo2 = g.lo[2], len0 = g.len[0], len1 = g.len[1], len2 = g.len[2];
     const int nk = min(tc.n, MASK_BITS);
     for (int w = threadIdx.x; w < nk * 6; w += blockDim.x) {
         const bool cur = w >= nk * 3;
         const int v = cur ? w - nk * 3 : w, k = v / 3, a = v - 3 * k;
         const int c = cur ? (a == 0 ? tc.cx[k] : a == 1 ? tc.cy[k] : tc.cz[k])
                           : (a == 0 ? tc.qx[k] : a == 1 ? tc.qy[k] : tc.qz[k]);
+        const int lo = a == 0 ? lo0 : a == 1 ? lo1 : lo2, len = a == 0 ? len0 : a == 1 ? len1 : len2;
         const int reach = cur ? reach1 : reach0;
         AxisMask &am = cur ? am1 : am0;
         for (int dd = -reach; dd <= reach; dd++) {
-            int u = axis_index(g, a, wrapi(c + dd, L), L);
-            if (u >= 0) atomicOr(&am.m[a][u], 1u << k);
+            int u = wrapi(c + dd, L) - lo;
+            if (u < 0) u += L;
+            else if (u >= L) u -= L;
+            if (u < len) atomicOr(&am.m[a][u], 1u << k);
         }
     }
     __syncthreads();

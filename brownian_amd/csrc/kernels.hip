@@ -409,17 +409,33 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     int *s_sg = (int *)(sur + (size_t)TCL_CAP * (60 + 8 * SUR_ACC));
     int *s_scnt = s_sg + TCL_CAP;
 
-    const long long step = *d.step;
-    const double sx = d.shift[3 * tg.r], sy = d.shift[3 * tg.r + 1], sz = d.shift[3 * tg.r + 2];
+    // wave-uniform words through the scalar cache, so that no vector load
+    // waits on them
+    const long long step = ld_const(d.step);
+    const double sx = ld_const(d.shift + 3 * tg.r), sy = ld_const(d.shift + 3 * tg.r + 1),
+                 sz = ld_const(d.shift + 3 * tg.r + 2);
 
+    // Every global request of the tile's set-up goes out before the first
+    // barrier (the tile's colloid list and its staleness flag, the 27 source
+    // lists' offsets), and the listed colloids' data goes out together with
+    // the source-list indices below, so that their round trips overlap.  (The
+    // step's list pass ran before this kernel; a list made stale since is
+    // rebuilt below.)
+    int cl_n = 0, cl_g = 0;
+    bool cl_stale = false;
+    if (COUNT_SHELL) {
+        cl_stale = ld_const(d.tcl_stale + tg.r) != 0;
+        cl_n = min(ld_const(d.tcl_cnt + T), TCL_CAP);
+        if (threadIdx.x < cl_n) cl_g = d.tcl_list[T * TCL_CAP + threadIdx.x];
+    }
+    int dep_S = 0, dep_base = 0, dep_end = 0;
     if (threadIdx.x < 27) {
-        int t = threadIdx.x;
-        int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-        int ax = wrapi(tg.tx - dx, R.nt[0]), ay = wrapi(tg.ty - dy, R.nt[1]), az = wrapi(tg.tz - dz, R.nt[2]);
-        int S = R.first_tile + ax + R.nt[0] * (ay + R.nt[1] * az);
-        s_src[t] = S;
-        s_base[t] = d.dep_off[S * 28 + t];
-        s_pre[t] = d.dep_off[S * 28 + t + 1] - d.dep_off[S * 28 + t];
+        const int t = threadIdx.x;
+        const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+        const int ax = wrapi(tg.tx - dx, R.nt[0]), ay = wrapi(tg.ty - dy, R.nt[1]), az = wrapi(tg.tz - dz, R.nt[2]);
+        dep_S = R.first_tile + ax + R.nt[0] * (ay + R.nt[1] * az);
+        dep_base = d.dep_off[dep_S * 28 + t];
+        dep_end = d.dep_off[dep_S * 28 + t + 1];
     }
     if constexpr (FAST) {
         for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
@@ -431,7 +447,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     } else {
         for (int c = threadIdx.x; c < ncell + 1; c += blockDim.x) s_start[c] = 0;
     }
-    if (COUNT_SHELL)
+    if (COUNT_SHELL) {
         for (int k = threadIdx.x; k < TCL_CAP; k += blockDim.x) {
             s_shell[k] = 0;
             if (ph.surrogate) {
@@ -439,23 +455,25 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 for (int j = 0; j < SUR_ACC; j++) s_sacc[k][j] = 0;
             }
         }
-    // the tile's colloid list, requested now so that it arrives with the
-    // particles (the step's list pass ran before this kernel; a list made
-    // stale since is rebuilt below)
-    int cl_n = 0, cl_g = 0;
-    if (COUNT_SHELL) {
-        cl_n = min(d.tcl_cnt[T], TCL_CAP);
-        if (threadIdx.x < cl_n) cl_g = d.tcl_list[T * TCL_CAP + threadIdx.x];
+        zero_axis_masks(s_am[0]);
+        if (NEAR_TEST) zero_axis_masks(s_am[1]);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < 27; t++) {
-            int c = s_pre[t];
-            s_pre[t] = acc;
-            acc += c;
+    // gather offsets: an exclusive scan of the 27 list lengths across the
+    // lanes of the first wave (the lists in direction order)
+    if (threadIdx.x < 64) {
+        const int dep_cnt = dep_end - dep_base;
+        int incl = dep_cnt;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if ((int)threadIdx.x >= o) incl += y;
         }
-        s_pre[27] = acc;
+        if (threadIdx.x < 27) {
+            s_src[threadIdx.x] = dep_S;
+            s_base[threadIdx.x] = dep_base;
+            s_pre[threadIdx.x] = incl - dep_cnt;
+        }
+        if (threadIdx.x == 26) s_pre[27] = incl;
     }
     __syncthreads();
     int n = s_pre[27];
@@ -463,29 +481,72 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         if (threadIdx.x == 0) atomicOr(d.status, BW_FLAG_TILE_OVERFLOW);
         n = cap;
     }
-    // gather: the loads are issued here and consumed after the colloid set-up,
-    // which runs while they are in flight
-    int rcell[SRD_PT], rpid[SRD_PT], rkey[SRD_PT];
+    // listed colloid threadIdx.x: requested before the source-list indices,
+    // written to LDS once it is here (while those are in flight)
+    const bool cl_mine = COUNT_SHELL && !cl_stale && (int)threadIdx.x < cl_n;
+    double cpx = 0, cpy = 0, cpz = 0, crx = 0, cry = 0, crz = 0, cvx = 0, cvy = 0, cvz = 0;
+    int ccx = 0, ccy = 0, ccz = 0, cqx = 0, cqy = 0, cqz = 0;
+    if (cl_mine) {
+        const int g = cl_g;
+        cpx = d.cpos[3 * g]; cpy = d.cpos[3 * g + 1]; cpz = d.cpos[3 * g + 2];
+        ccx = d.ccell[3 * g]; ccy = d.ccell[3 * g + 1]; ccz = d.ccell[3 * g + 2];
+        crx = d.cpos_run[3 * g]; cry = d.cpos_run[3 * g + 1]; crz = d.cpos_run[3 * g + 2];
+        cqx = d.ccell_run[3 * g]; cqy = d.ccell_run[3 * g + 1]; cqz = d.ccell_run[3 * g + 2];
+        cvx = d.cvel[3 * g]; cvy = d.cvel[3 * g + 1]; cvz = d.cvel[3 * g + 2];
+    }
+    // gather: the source-list index of each slot (the list holding gather
+    // index g is the last t with s_pre[t] <= g, by five fixed halvings).  A
+    // slot past the tile's count reads the index and the state of the last
+    // particle (of slot 0 of an in-range index for an empty tile): the loads
+    // are straight runs, each with one wait after it, and the values are
+    // never used
+    int rcell[SRD_PT], rpid[SRD_PT], rkey[SRD_PT], rslot[SRD_PT];
     Real rox[SRD_PT], roy[SRD_PT], roz[SRD_PT], rvx[SRD_PT], rvy[SRD_PT], rvz[SRD_PT];
+    {
+        const int g_last = max(n - 1, 0), idx_last = ph.n_tiles * cap - 1;
 #pragma unroll
-    for (int k = 0; k < SRD_PT; k++) {
-        int g = threadIdx.x + k * TH;
-        rkey[k] = -1;
-        if (g < n) {
-            // the source list holding gather index g: the last t with
-            // s_pre[t] <= g, by five fixed halvings (the slots' searches interleave)
+        for (int k = 0; k < SRD_PT; k++) {
+            const int g = min((int)threadIdx.x + k * TH, g_last);
             int lo = 0;
 #pragma unroll
             for (int h = 16; h > 0; h >>= 1)
                 if (lo + h <= 26 && s_pre[lo + h] <= g) lo += h;
-            int S = s_src[lo];
-            int li = d.dep_idx[S * cap + s_base[lo] + (g - s_pre[lo])];
-            int slot = S * cap + li;
-            rcell[k] = src.cell[slot];
-            rox[k] = src.ox[slot]; roy[k] = src.oy[slot]; roz[k] = src.oz[slot];
-            rvx[k] = src.vx[slot]; rvy[k] = src.vy[slot]; rvz[k] = src.vz[slot];
-            rpid[k] = src.pid[slot];
+            rkey[k] = s_src[lo] * cap;  // the source tile's first slot, until the key is known
+            rslot[k] = min(rkey[k] + s_base[lo] + (g - s_pre[lo]), idx_last);
         }
+#pragma unroll
+        for (int k = 0; k < SRD_PT; k++) rslot[k] = d.dep_idx[rslot[k]];
+#pragma unroll
+        for (int k = 0; k < SRD_PT; k++) rslot[k] += rkey[k];
+    }
+#pragma unroll
+    for (int k = 0; k < SRD_PT; k++) rkey[k] = -1;
+    const double Ld = (double)L;
+    if (cl_mine) {
+        const int q = threadIdx.x;
+        tc.g[q] = cl_g;
+        tc.px[q] = cpx; tc.py[q] = cpy; tc.pz[q] = cpz;
+        tc.cx[q] = ccx; tc.cy[q] = ccy; tc.cz[q] = ccz;
+        tc.rx[q] = crx; tc.ry[q] = cry; tc.rz[q] = crz;
+        tc.qx[q] = cqx; tc.qy[q] = cqy; tc.qz[q] = cqz;
+        tc.vx[q] = cvx; tc.vy[q] = cvy; tc.vz[q] = cvz;
+        if (NEAR_TEST) {
+            nc[0].cx[q] = (float)img1(cpx - ox0, Ld);
+            nc[0].cy[q] = (float)img1(cpy - oy0, Ld);
+            nc[0].cz[q] = (float)img1(cpz - oz0, Ld);
+            nc[0].rx[q] = (float)img1(crx - ox0, Ld);
+            nc[0].ry[q] = (float)img1(cry - oy0, Ld);
+            nc[0].rz[q] = (float)img1(crz - oz0, Ld);
+        }
+    }
+    if (COUNT_SHELL && !cl_stale && threadIdx.x == 0) tc.n = cl_n;
+#pragma unroll
+    for (int k = 0; k < SRD_PT; k++) {
+        const int slot = rslot[k];
+        rcell[k] = src.cell[slot];
+        rox[k] = src.ox[slot]; roy[k] = src.oy[slot]; roz[k] = src.oz[slot];
+        rvx[k] = src.vx[slot]; rvy[k] = src.vy[slot]; rvz[k] = src.vz[slot];
+        rpid[k] = src.pid[slot];
     }
     Region rg = tile_region(R, tg.tx, tg.ty, tg.tz);
     if (COUNT_SHELL && ph.surrogate) {
@@ -497,7 +558,6 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 s_sp[q][k] = d.cpos[3 * g + k];
                 s_sp[q][3 + k] = d.cvel[3 * g + k];
             }
-            const double Ld = (double)L;
             s_spf[q][0] = (float)img1(s_sp[q][0] - ox0, Ld);
             s_spf[q][1] = (float)img1(s_sp[q][1] - oy0, Ld);
             s_spf[q][2] = (float)img1(s_sp[q][2] - oz0, Ld);
@@ -505,26 +565,23 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         if (threadIdx.x == 0) s_sn = ns;
     }
     if (COUNT_SHELL) {
-        if (d.tcl_stale[tg.r]) {
-            load_tile_colloids(d, ph, R, tg.r, T, true, tc);
+        if (cl_stale) {
+            load_tile_colloids(d, ph, R, tg.r, T, true, tc);  // ends with a barrier
+            if (NEAR_TEST)
+                for (int q = threadIdx.x; q < tc.n; q += blockDim.x) {
+                    nc[0].cx[q] = (float)img1(tc.px[q] - ox0, Ld);
+                    nc[0].cy[q] = (float)img1(tc.py[q] - oy0, Ld);
+                    nc[0].cz[q] = (float)img1(tc.pz[q] - oz0, Ld);
+                    nc[0].rx[q] = (float)img1(tc.rx[q] - ox0, Ld);
+                    nc[0].ry[q] = (float)img1(tc.ry[q] - oy0, Ld);
+                    nc[0].rz[q] = (float)img1(tc.rz[q] - oz0, Ld);
+                }
         } else {
-            if (threadIdx.x < cl_n) load_tile_slot(d, ph, cl_g, threadIdx.x, true, tc);
-            if (threadIdx.x == 0) tc.n = cl_n;
             __syncthreads();
         }
-        if (NEAR_TEST) build_axis_masks2(tc, rg, L, 3, s_am[0], ph.near_reach, s_am[COUNT_SHELL ? 1 : 0]);
-        else build_axis_masks(tc, rg, L, true, 3, s_am[0]);
-        if (NEAR_TEST) {
-            const double Ld = (double)L;
-            for (int q = threadIdx.x; q < tc.n; q += blockDim.x) {
-                nc[0].cx[q] = (float)img1(tc.px[q] - ox0, Ld);
-                nc[0].cy[q] = (float)img1(tc.py[q] - oy0, Ld);
-                nc[0].cz[q] = (float)img1(tc.pz[q] - oz0, Ld);
-                nc[0].rx[q] = (float)img1(tc.rx[q] - ox0, Ld);
-                nc[0].ry[q] = (float)img1(tc.ry[q] - oy0, Ld);
-                nc[0].rz[q] = (float)img1(tc.rz[q] - oz0, Ld);
-            }
-        }
+        // masks zeroed before the first barrier
+        if (NEAR_TEST) fill_axis_masks2(tc, rg, L, 3, s_am[0], ph.near_reach, s_am[COUNT_SHELL ? 1 : 0]);
+        else fill_axis_masks(tc.n, tc.qx, tc.qy, tc.qz, rg, L, 3, s_am[0]);
     }
     // shifted cells and the per-cell histogram
     bool drift = false;
