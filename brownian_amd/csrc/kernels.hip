@@ -308,16 +308,23 @@ __global__ __launch_bounds__(DEPART_THREADS) void k_depart(Phys ph, Dev d, Fluid
 // shell -- is streamed through all n_sub substeps here, in registers, with
 // the arithmetic of the streaming kernel.  The others ("near") are stored
 // first in the tile, and only they are touched by the per-substep kernels.
+// x rounded to the nearest integer (ties to even), as __double2ll_rn does,
+// for |x| < 2^51: x + 1.5 2^52 lies in [2^52, 2^53), where the doubles are
+// the integers, so the addition's rounding is the integer rounding and the
+// sum's bit pattern is that of 1.5 2^52 plus the integer
+__device__ __forceinline__ long long rint_small(double x) {
+    return __double_as_longlong(x + 0x1.8p52) - 0x4338000000000000LL;
+}
 // fixed-point (2^-40) difference a - b of two velocity components, exact for
 // |a|, |b| < 1024 (larger ones are flagged by the caller and count as 0)
 __device__ __forceinline__ unsigned long long fixq_diff(float a, float b) {
-    const long long qa = fabsf(a) < 1024.0f ? __double2ll_rn((double)a * 0x1p40) : 0;
-    const long long qb = fabsf(b) < 1024.0f ? __double2ll_rn((double)b * 0x1p40) : 0;
+    const long long qa = fabsf(a) < 1024.0f ? rint_small((double)a * 0x1p40) : 0;
+    const long long qb = fabsf(b) < 1024.0f ? rint_small((double)b * 0x1p40) : 0;
     return (unsigned long long)(qa - qb);
 }
 __device__ __forceinline__ unsigned long long fixq_diff(double a, double b) {
-    const long long qa = fabs(a) < 1024.0 ? __double2ll_rn(a * 0x1p40) : 0;
-    const long long qb = fabs(b) < 1024.0 ? __double2ll_rn(b * 0x1p40) : 0;
+    const long long qa = fabs(a) < 1024.0 ? rint_small(a * 0x1p40) : 0;
+    const long long qb = fabs(b) < 1024.0 ? rint_small(b * 0x1p40) : 0;
     return (unsigned long long)(qa - qb);
 }
 // a tile's surrogate shell counts and fixed-point sums into the per-colloid
@@ -614,10 +621,10 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 const double e = (dx * dx + dy * dy) + dz * dz;
                 atomicAdd(&f_cnt[key], 1);
                 atomicMin(&f_minpid[key], rpid[k]);
-                atomicAdd(&f_sum[5 * key], (unsigned long long)__double2ll_rn(dx * 0x1p40));
-                atomicAdd(&f_sum[5 * key + 1], (unsigned long long)__double2ll_rn(dy * 0x1p40));
-                atomicAdd(&f_sum[5 * key + 2], (unsigned long long)__double2ll_rn(dz * 0x1p40));
-                atomicAdd(&f_sum[5 * key + 3], (unsigned long long)__double2ll_rn(e * 0x1p28));
+                atomicAdd(&f_sum[5 * key], (unsigned long long)rint_small(dx * 0x1p40));
+                atomicAdd(&f_sum[5 * key + 1], (unsigned long long)rint_small(dy * 0x1p40));
+                atomicAdd(&f_sum[5 * key + 2], (unsigned long long)rint_small(dz * 0x1p40));
+                atomicAdd(&f_sum[5 * key + 3], (unsigned long long)rint_small(e * 0x1p28));
             } else {
                 s_v[g] = rvx[k]; s_v[cap + g] = rvy[k]; s_v[2 * cap + g] = rvz[k];
                 atomicAdd(&s_start[key], 1);
@@ -631,8 +638,9 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         // per cell (cpp/rotation_mpcd.cpp:14-49), in three passes:
         //  1. the mean u = sum v / N in FP64 from the exact sums; the random
         //     axis and the rotation R by theta about it (FP32); the scale of
-        //     the relative-energy sum, 2^(60 - e) with sum |v|^2 < 2^e, which
-        //     bounds every term |v - u|^2 and their sum;
+        //     the relative-energy sum, 2^(50 - e) with sum |v|^2 < 2^e, which
+        //     bounds every term |v - u|^2 and their sum (so each scaled term
+        //     is below 2^50 and rounds to an integer by rint_small);
         //  2. per particle |v - u|^2 in FP64, summed exactly in that fixed
         //     point (no cancellation, whatever |u|; the rotation leaves the
         //     sum unchanged);
@@ -647,8 +655,9 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             const double E = (double)(long long)f_sum[5 * c + 3] * 0x1p-28;
             int ex;
             frexp(E + 0x1p-16, &ex);  // sum |v|^2 < 2^ex (2^-16 covers <= 4096 roundings of 2^-29)
-            f_exp[c] = 60 - ex;
-            f_u[3 * c] = sx / N; f_u[3 * c + 1] = sy / N; f_u[3 * c + 2] = sz / N;
+            f_exp[c] = 50 - ex;
+            const double iN = 1.0 / N;  // the collision conserves the cell's momentum whatever u's rounding
+            f_u[3 * c] = sx * iN; f_u[3 * c + 1] = sy * iN; f_u[3 * c + 2] = sz * iN;
             const int kx = ox0 + c % B0, ky = oy0 + (c / B0) % B1, kz = oz0 + c / (B0 * B1);
             double u1, u2;
             if (!draw_axis_raw(d, ph, R, step, linear_cell(kx, ky, kz, L), u1, u2)) {
@@ -661,7 +670,9 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             const double rho = 2.0 * u1 - 1.0;
             const double phi = 6.283185307179586 * u2;
             const double sq = sqrt(1.0 - rho * rho);
-            const double r1 = cos(phi) * sq, r2 = sin(phi) * sq, r3 = rho;
+            double sphi, cphi;
+            sincos(phi, &sphi, &cphi);
+            const double r1 = cphi * sq, r2 = sphi * sq, r3 = rho;
             const double ir1 = ph.ict * r1, ir2 = ph.ict * r2, ir3 = ph.ict * r3;
             const double rm[9] = {ir1 * r1 + ph.ct, ir1 * r2 - ph.st * r3, ir1 * r3 + ph.st * r2,
                                   ir2 * r1 + ph.st * r3, ir2 * r2 + ph.ct, ir2 * r3 - ph.st * r1,
@@ -679,8 +690,8 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                     const double ex = (double)rvx[k] - f_u[3 * c], ey = (double)rvy[k] - f_u[3 * c + 1],
                                  ez = (double)rvz[k] - f_u[3 * c + 2];
                     const double q = ldexp((ex * ex + ey * ey) + ez * ez, f_exp[c]);
-                    if (q < 0x1p62)  // false for NaN (flagged with the velocity above)
-                        atomicAdd(&f_sum[5 * c + 4], (unsigned long long)__double2ll_rn(q));
+                    if (q < 0x1p51)  // false for NaN (flagged with the velocity above)
+                        atomicAdd(&f_sum[5 * c + 4], (unsigned long long)rint_small(q));
                 }
             }
         }
@@ -983,7 +994,8 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                     const float py = (float)pdiff(iy, oy0, L) + (float)roy[k];
                     const float pz = (float)pdiff(iz, oz0, L) + (float)roz[k];
                     const float vx = (float)rvx[k], vy = (float)rvy[k], vz = (float)rvz[k];
-                    const float tr = sqrtf((vx * vx + vy * vy) + vz * vz) * (float)ph.ndt;
+                    // the hardware square root (1 ulp): both uses below carry slack
+                    const float tr = __builtin_amdgcn_sqrtf((vx * vx + vy * vy) + vz * vz) * (float)ph.ndt;
                     const bool inside01 = rox[k] >= Real(0) && rox[k] < Real(1) && roy[k] >= Real(0) &&
                                           roy[k] < Real(1) && roz[k] >= Real(0) && roz[k] < Real(1);
                     near = !(tr <= (float)ph.vcap_ndt) || !inside01;
