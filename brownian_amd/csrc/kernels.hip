@@ -555,6 +555,37 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         rvx[k] = src.vx[slot]; rvy[k] = src.vy[slot]; rvz[k] = src.vz[slot];
         rpid[k] = src.pid[slot];
     }
+    if constexpr (FAST) {
+        // per cell, while the particles are on their way: the random axis
+        // (rho = 2U - 1, phi = 2 pi U) and the rotation by theta about it
+        // (cpp/rotation_mpcd.cpp:19-25), in FP64 (sqrt(1 - rho^2) cancels for
+        // |rho| near 1), rounded once to FP32; the thermostat factor is folded
+        // in once the relative energies are known.  A missing draw is marked in
+        // the cell's exponent word, which the cell's first pass reads (only a
+        // cell of two or more particles needs its draw)
+        for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+            const int kx = ox0 + c % B0, ky = oy0 + (c / B0) % B1, kz = oz0 + c / (B0 * B1);
+            double u1, u2;
+            f_exp[c] = 0;
+            if (!draw_axis_raw(d, ph, R, step, linear_cell(kx, ky, kz, L), u1, u2)) {
+                f_exp[c] = 1;
+                u1 = 0.5; u2 = 0.0;
+            }
+            const double rho = 2.0 * u1 - 1.0;
+            const double phi = 6.283185307179586 * u2;
+            const double sq = sqrt(1.0 - rho * rho);
+            double sphi, cphi;
+            sincos(phi, &sphi, &cphi);
+            const double r1 = cphi * sq, r2 = sphi * sq, r3 = rho;
+            const double ir1 = ph.ict * r1, ir2 = ph.ict * r2, ir3 = ph.ict * r3;
+            const double rm[9] = {ir1 * r1 + ph.ct, ir1 * r2 - ph.st * r3, ir1 * r3 + ph.st * r2,
+                                  ir2 * r1 + ph.st * r3, ir2 * r2 + ph.ct, ir2 * r3 - ph.st * r1,
+                                  ir3 * r1 - ph.st * r2, ir3 * r2 + ph.st * r1, ir3 * r3 + ph.ct};
+            float *m = f_m + 9 * c;
+#pragma unroll
+            for (int q = 0; q < 9; q++) m[q] = (float)rm[q];
+        }
+    }
     Region rg = tile_region(R, tg.tx, tg.ty, tg.tz);
     if (COUNT_SHELL && ph.surrogate) {
         const int ns = min(d.tsl_cnt[T], TCL_CAP);
@@ -636,8 +667,9 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     bool zero_var = false;
     if constexpr (FAST) {
         // per cell (cpp/rotation_mpcd.cpp:14-49), in three passes:
-        //  1. the mean u = sum v / N in FP64 from the exact sums; the random
-        //     axis and the rotation R by theta about it (FP32); the scale of
+        //  1. the mean u = sum v / N in FP64 from the exact sums (the random
+        //     axis and the rotation R by theta about it, FP32, were formed while
+        //     the particles were on their way); the scale of
         //     the relative-energy sum, 2^(50 - e) with sum |v|^2 < 2^e, which
         //     bounds every term |v - u|^2 and their sum (so each scaled term
         //     is below 2^50 and rounds to an integer by rint_small);
@@ -649,6 +681,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
             const int N = f_cnt[c];
             if (N < 2) continue;
+            if (f_exp[c]) missing = true;
             const double sx = (double)(long long)f_sum[5 * c] * 0x1p-40;
             const double sy = (double)(long long)f_sum[5 * c + 1] * 0x1p-40;
             const double sz = (double)(long long)f_sum[5 * c + 2] * 0x1p-40;
@@ -658,29 +691,8 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             f_exp[c] = 50 - ex;
             const double iN = 1.0 / N;  // the collision conserves the cell's momentum whatever u's rounding
             f_u[3 * c] = sx * iN; f_u[3 * c + 1] = sy * iN; f_u[3 * c + 2] = sz * iN;
-            const int kx = ox0 + c % B0, ky = oy0 + (c / B0) % B1, kz = oz0 + c / (B0 * B1);
-            double u1, u2;
-            if (!draw_axis_raw(d, ph, R, step, linear_cell(kx, ky, kz, L), u1, u2)) {
-                missing = true;
-                u1 = 0.5; u2 = 0.0;
-            }
-            // cpp/rotation_mpcd.cpp:19-25: axis (rho = 2U - 1, phi = 2 pi U) and
-            // Rodrigues, in FP64 (one per cell; sqrt(1 - rho^2) cancels for |rho|
-            // near 1), rounded once to FP32
-            const double rho = 2.0 * u1 - 1.0;
-            const double phi = 6.283185307179586 * u2;
-            const double sq = sqrt(1.0 - rho * rho);
-            double sphi, cphi;
-            sincos(phi, &sphi, &cphi);
-            const double r1 = cphi * sq, r2 = sphi * sq, r3 = rho;
-            const double ir1 = ph.ict * r1, ir2 = ph.ict * r2, ir3 = ph.ict * r3;
-            const double rm[9] = {ir1 * r1 + ph.ct, ir1 * r2 - ph.st * r3, ir1 * r3 + ph.st * r2,
-                                  ir2 * r1 + ph.st * r3, ir2 * r2 + ph.ct, ir2 * r3 - ph.st * r1,
-                                  ir3 * r1 - ph.st * r2, ir3 * r2 + ph.st * r1, ir3 * r3 + ph.ct};
-            float *m = f_m + 9 * c;
-#pragma unroll
-            for (int q = 0; q < 9; q++) m[q] = (float)rm[q];
         }
+        if (missing) atomicOr(d.status, BW_FLAG_DRAW_MISSING);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < SRD_PT; k++) {
@@ -709,7 +721,6 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
 #pragma unroll
             for (int q = 0; q < 9; q++) m[q] = (float)((double)m[q] * scale);
         }
-        if (missing) atomicOr(d.status, BW_FLAG_DRAW_MISSING);
     } else {
     block_exclusive_scan(s_start, ncell, s_wtot);
     if (threadIdx.x == 0) s_start[ncell] = n;
