@@ -995,8 +995,13 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 }
                 int ix, iy, iz;
                 unpack_cell(rcell[k], ix, iy, iz);
-                const double x = (double)ix + (double)rox[k], y = (double)iy + (double)roy[k],
-                             z = (double)iz + (double)roz[k];
+                // the FP64 position, formed only for a colloid that passes the
+                // mask and FP32 tests
+                const Real sox = rox[k], soy = roy[k], soz = roz[k];
+                auto shell_of = [&](int q) {
+                    return shell_test(ph, tc, q, L, ix, iy, iz, (double)ix + (double)sox, (double)iy + (double)soy,
+                                      (double)iz + (double)soz);
+                };
                 unsigned m = cell_mask(s_am[0], rg, ix, iy, iz, L);
                 if (NEAR_TEST) {
                     // conservative test in FP32, tile-local coordinates (2e-2 cell slack)
@@ -1007,8 +1012,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                     const float vx = (float)rvx[k], vy = (float)rvy[k], vz = (float)rvz[k];
                     // the hardware square root (1 ulp): both uses below carry slack
                     const float tr = __builtin_amdgcn_sqrtf((vx * vx + vy * vy) + vz * vz) * (float)ph.ndt;
-                    const bool inside01 = rox[k] >= Real(0) && rox[k] < Real(1) && roy[k] >= Real(0) &&
-                                          roy[k] < Real(1) && roz[k] >= Real(0) && roz[k] < Real(1);
+                    const bool inside01 = in_unit(rox[k]) && in_unit(roy[k]) && in_unit(roz[k]);
                     near = !(tr <= (float)ph.vcap_ndt) || !inside01;
                     const float rs = sqrtf((float)ph.near_shell2) + 0.02f, rs2 = rs * rs;
                     const float rc = (float)ph.near_base + tr + 0.02f, rc2 = rc * rc;
@@ -1025,7 +1029,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                     for_candidates(tc, m, [&](int q) {
                         if (q < tc.n && d2(nc[0].rx[q], nc[0].ry[q], nc[0].rz[q]) <= rs2) {
                             near = true;
-                            if (shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
+                            if (shell_of(q)) atomicAdd(&s_shell[q], 1);
                         }
                     });
                     if (!near) {  // within reach of a collision during the step
@@ -1036,13 +1040,14 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                     }
                 } else {
                     for_candidates(tc, m, [&](int q) {
-                        if (q < tc.n && shell_test(ph, tc, q, L, ix, iy, iz, x, y, z)) atomicAdd(&s_shell[q], 1);
+                        if (q < tc.n && shell_of(q)) atomicAdd(&s_shell[q], 1);
                     });
                 }
                 if (ph.surrogate) {  // inside a colloid that was a phantom in the last step
                     const double Ld = (double)L;
                     for_candidates(tc, m, [&](int q) {
                         if (q >= tc.n || !d.cmode_prev[tc.g[q]]) return;
+                        const double x = (double)ix + (double)sox, y = (double)iy + (double)soy, z = (double)iz + (double)soz;
                         double ax = img1(x - tc.px[q], Ld), ay = img1(y - tc.py[q], Ld), az = img1(z - tc.pz[q], Ld);
                         if ((ax * ax + ay * ay) + az * az <= ph.lim_coll) rcell[k] |= PHANTOM;
                     });
@@ -1207,6 +1212,22 @@ struct StreamOp {
     const AxisMask *mdet, *mkick;
     int L, r, lox, loy, loz, lnx, lny, lnz;
     bool drift, ch_over;
+    // DIRECT: the listed colloids' current and run() positions relative to
+    // the tile box's origin (bx0, by0, bz0), FP32, for a conservative
+    // pre-test (2e-2 cell slack) ahead of the exact one; wrap when the box is
+    // small enough for the periodic image to matter
+    const NearColl *nf = nullptr;
+    int bx0 = 0, by0 = 0, bz0 = 0;
+    bool wrap = true;
+
+    __device__ __forceinline__ float d2f(float px, float py, float pz, float cx, float cy, float cz) const {
+        float ax = px - cx, ay = py - cy, az = pz - cz;
+        if (wrap) {
+            const float Lf = (float)L, iLf = 1.0f / Lf;
+            ax -= Lf * rintf(ax * iLf); ay -= Lf * rintf(ay * iLf); az -= Lf * rintf(az * iLf);
+        }
+        return (ax * ax + ay * ay) + az * az;
+    }
 
     __device__ __forceinline__ unsigned all_listed() const {
         return tc.n >= MASK_BITS ? 0xffffffffu : ((1u << tc.n) - 1u);
@@ -1222,7 +1243,11 @@ struct StreamOp {
                 double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
                 double wx = vx, wy = vy, wz = vz;
                 bool hit = false;
+                const float kpx = (float)pdiff(ix, bx0, L) + (float)ox, kpy = (float)pdiff(iy, by0, L) + (float)oy,
+                            kpz = (float)pdiff(iz, bz0, L) + (float)oz;
+                const float rs = sqrtf((float)ph.lim_shell) + 0.02f, rs2 = rs * rs;
                 auto kick = [&](int k) {
+                    if (DIRECT && nf && k < tc.n && d2f(kpx, kpy, kpz, nf->rx[k], nf->ry[k], nf->rz[k]) > rs2) return;
                     if (k < tc.n && shell_test(ph, tc, k, L, ix, iy, iz, x, y, z)) {
                         wx = wx - tc.kx[k];
                         wy = wy - tc.ky[k];
@@ -1260,11 +1285,18 @@ struct StreamOp {
             double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
             int n_in = 0, gin = -1;
             bool inside_any = false;  // by distance alone (the stencil skips a colloid's own cell)
+            const float dpx = (float)pdiff(ix, bx0, L) + (float)ox, dpy = (float)pdiff(iy, by0, L) + (float)oy,
+                        dpz = (float)pdiff(iz, bz0, L) + (float)oz;
+            const float rr = sqrtf((float)ph.lim_coll) + 0.02f, rr2 = rr * rr;
             auto test = [&](int k) {
                 if (k >= tc.n) return;
-                if (DIRECT) {  // inside a colloid: within 3 cells of its cell on every axis
-                    const int ax = pdiff(ix, tc.cx[k], L), ay = pdiff(iy, tc.cy[k], L), az = pdiff(iz, tc.cz[k], L);
-                    if (ax < -3 || ax > 3 || ay < -3 || ay > 3 || az < -3 || az > 3) return;
+                if (DIRECT) {
+                    if (nf) {  // within the FP32 radius of the colloid (the exact test follows)
+                        if (d2f(dpx, dpy, dpz, nf->cx[k], nf->cy[k], nf->cz[k]) > rr2) return;
+                    } else {  // inside a colloid: within 3 cells of its cell on every axis
+                        const int ax = pdiff(ix, tc.cx[k], L), ay = pdiff(iy, tc.cy[k], L), az = pdiff(iz, tc.cz[k], L);
+                        if (ax < -3 || ax > 3 || ay < -3 || ay > 3 || az < -3 || az > 3) return;
+                    }
                 }
                 double rx = img1(tc.px[k] - x, Ld), ry = img1(tc.py[k] - y, Ld), rz = img1(tc.pz[k] - z, Ld);
                 if ((rx * rx + ry * ry) + rz * rz <= ph.lim_coll) {
@@ -1347,6 +1379,7 @@ __global__ __launch_bounds__(256, KICK ? 1 : 8) void k_stream_near(Phys ph, Dev 
     static_assert(TCL_CAP == 64, "one lane per listed colloid");
     using TC = typename std::conditional<KICK, TileColl, TileCore>::type;
     __shared__ TC s_tc[4];
+    __shared__ NearColl s_nf[4];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     TC &tc = s_tc[w];
     // the argument fields of the loop, fetched in one batch of scalar loads
@@ -1389,14 +1422,27 @@ __global__ __launch_bounds__(256, KICK ? 1 : 8) void k_stream_near(Phys ph, Dev 
             if (lane < cnt) load_tile_slot(d, ph, gl, lane, KICK, tc);
         }
         if (lane == 0) tc.n = cnt;
-        wave_sync();
         const int lt = T - R.first_tile;
         const int tx = lt % R.nt[0], ty = (lt / R.nt[0]) % R.nt[1], tz = lt / (R.nt[0] * R.nt[1]);
+        const int bx0 = tx * R.B[0], by0 = ty * R.B[1], bz0 = tz * R.B[2];
+        if (lane < cnt) {  // this lane's colloid, relative to the tile box's origin
+            const double Ld = (double)L;
+            s_nf[w].cx[lane] = (float)img1(tc.px[lane] - bx0, Ld);
+            s_nf[w].cy[lane] = (float)img1(tc.py[lane] - by0, Ld);
+            s_nf[w].cz[lane] = (float)img1(tc.pz[lane] - bz0, Ld);
+            if constexpr (KICK) {
+                s_nf[w].rx[lane] = (float)img1(tc.rx[lane] - bx0, Ld);
+                s_nf[w].ry[lane] = (float)img1(tc.ry[lane] - by0, Ld);
+                s_nf[w].rz[lane] = (float)img1(tc.rz[lane] - bz0, Ld);
+            }
+        }
+        wave_sync();
         const Region rg = tile_region(R, tx, ty, tz);
         StreamOp<Real, KICK, true, true, TC, true> op{ph, d, f, tc, rg, nullptr, nullptr, L, r,
-                                                      tx * R.B[0] - TILE_REACH, ty * R.B[1] - TILE_REACH,
-                                                      tz * R.B[2] - TILE_REACH, R.B[0] + 2 * TILE_REACH,
-                                                      R.B[1] + 2 * TILE_REACH, R.B[2] + 2 * TILE_REACH, false, false};
+                                                      bx0 - TILE_REACH, by0 - TILE_REACH,
+                                                      bz0 - TILE_REACH, R.B[0] + 2 * TILE_REACH,
+                                                      R.B[1] + 2 * TILE_REACH, R.B[2] + 2 * TILE_REACH, false, false,
+                                                      &s_nf[w], bx0, by0, bz0, ph.near_wrap != 0};
         if (i < n) op(slot, c0, ox, oy, oz, vx, vy, vz);
         drift = drift || op.drift;
         ch_over = ch_over || op.ch_over;
