@@ -1153,10 +1153,11 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 ix = wrap_once(ix, L); iy = wrap_once(iy, L); iz = wrap_once(iz, L);
                 rcell[k] = pack_cell(ix, iy, iz) | (rcell[k] & PHANTOM);
                 rox[k] = ox; roy[k] = oy; roz[k] = oz;
-                if (!(near_interval(ix, ox0 - TILE_REACH, B0 + 2 * TILE_REACH, L) &&
-                      near_interval(iy, oy0 - TILE_REACH, B1 + 2 * TILE_REACH, L) &&
-                      near_interval(iz, oz0 - TILE_REACH, B2 + 2 * TILE_REACH, L)))
-                    far_drift = true;
+                // no reach check: the particle's shifted cell lies in the tile box
+                // (binning outside it is flagged above) and differs from its cell
+                // by floor(o + s) in {-1, 0, 1} (offset in [0,1), shift in
+                // [-1/2, 1/2)); with less than a cell of travel it ends within two
+                // cells of the box, inside the TILE_REACH the next step needs
             }
         }
     }
