@@ -666,18 +666,20 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     __syncthreads();
     bool zero_var = false;
     if constexpr (FAST) {
-        // per cell (cpp/rotation_mpcd.cpp:14-49), in three passes:
-        //  1. the mean u = sum v / N in FP64 from the exact sums (the random
-        //     axis and the rotation R by theta about it, FP32, were formed while
-        //     the particles were on their way); the scale of
-        //     the relative-energy sum, 2^(50 - e) with sum |v|^2 < 2^e, which
-        //     bounds every term |v - u|^2 and their sum (so each scaled term
-        //     is below 2^50 and rounds to an integer by rint_small);
-        //  2. per particle |v - u|^2 in FP64, summed exactly in that fixed
-        //     point (no cancellation, whatever |u|; the rotation leaves the
-        //     sum unchanged);
-        //  3. the thermostat factor, folded into M = scale R.
-        bool missing = false;
+        // per cell (cpp/rotation_mpcd.cpp:14-49).  The mean u = sum v / N in
+        // FP64 from the exact sums (the random axis and the rotation R by theta
+        // about it, FP32, were formed while the particles were on their way).
+        // The relative energy sum |v - u|^2 (which the rotation leaves
+        // unchanged) is sum |v|^2 - |sum v|^2 / N from the same sums, wherever
+        // that difference is known to 2^-26 of itself: its error is bounded by
+        // the sums' roundings (N 2^-29 for the |v|^2 terms, 2^-41 per velocity
+        // term) and the FP64 operations.  A cell where it is not (a cold cell,
+        // or one moving fast against its spread) takes the exact second pass:
+        // per particle |v - u|^2 in FP64, summed in the fixed point 2^(50 - e)
+        // with sum |v|^2 < 2^e, which bounds every term and their sum (each
+        // scaled term below 2^50 rounds to an integer by rint_small).  The
+        // thermostat factor is folded into M = scale R.
+        bool missing = false, second = false;
         for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
             const int N = f_cnt[c];
             if (N < 2) continue;
@@ -686,40 +688,53 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             const double sy = (double)(long long)f_sum[5 * c + 1] * 0x1p-40;
             const double sz = (double)(long long)f_sum[5 * c + 2] * 0x1p-40;
             const double E = (double)(long long)f_sum[5 * c + 3] * 0x1p-28;
-            int ex;
-            frexp(E + 0x1p-16, &ex);  // sum |v|^2 < 2^ex (2^-16 covers <= 4096 roundings of 2^-29)
-            f_exp[c] = 50 - ex;
             const double iN = 1.0 / N;  // the collision conserves the cell's momentum whatever u's rounding
             f_u[3 * c] = sx * iN; f_u[3 * c + 1] = sy * iN; f_u[3 * c + 2] = sz * iN;
+            const double S2 = (sx * sx + sy * sy) + sz * sz;
+            const double var = E - S2 * iN;
+            const double err = N * 0x1p-29 + 0x1p-51 * E + 0x1p-39 * sqrt(S2) + 0x1p-50 * (S2 * iN);
+            if (var > err * 0x1p26) {
+                const double scale = sqrt(3.0 * (N - 1) * ph.kbt / (ph.mass_fl * var));
+                float *m = f_m + 9 * c;
+#pragma unroll
+                for (int q = 0; q < 9; q++) m[q] = (float)((double)m[q] * scale);
+                f_exp[c] = -1;  // done
+            } else {
+                int ex;
+                frexp(E + 0x1p-16, &ex);  // sum |v|^2 < 2^ex (2^-16 covers <= 4096 roundings of 2^-29)
+                f_exp[c] = 50 - ex;
+                second = true;
+            }
         }
         if (missing) atomicOr(d.status, BW_FLAG_DRAW_MISSING);
-        __syncthreads();
+        if (__syncthreads_or(second)) {  // some cell of the tile needs the exact pass
 #pragma unroll
-        for (int k = 0; k < SRD_PT; k++) {
-            if (threadIdx.x + k * TH < n) {
-                const int c = rkey[k];
-                if (f_cnt[c] > 1) {
-                    const double ex = (double)rvx[k] - f_u[3 * c], ey = (double)rvy[k] - f_u[3 * c + 1],
-                                 ez = (double)rvz[k] - f_u[3 * c + 2];
-                    const double q = ldexp((ex * ex + ey * ey) + ez * ez, f_exp[c]);
-                    if (q < 0x1p51)  // false for NaN (flagged with the velocity above)
-                        atomicAdd(&f_sum[5 * c + 4], (unsigned long long)rint_small(q));
+            for (int k = 0; k < SRD_PT; k++) {
+                if (threadIdx.x + k * TH < n) {
+                    const int c = rkey[k];
+                    if (f_cnt[c] > 1 && f_exp[c] >= 0) {
+                        const double ex = (double)rvx[k] - f_u[3 * c], ey = (double)rvy[k] - f_u[3 * c + 1],
+                                     ez = (double)rvz[k] - f_u[3 * c + 2];
+                        const double q = ldexp((ex * ex + ey * ey) + ez * ez, f_exp[c]);
+                        if (q < 0x1p51)  // false for NaN (flagged with the velocity above)
+                            atomicAdd(&f_sum[5 * c + 4], (unsigned long long)rint_small(q));
+                    }
                 }
             }
-        }
-        __syncthreads();
-        for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-            const int N = f_cnt[c];
-            if (N < 2) continue;
-            const double var = ldexp((double)(long long)f_sum[5 * c + 4], -f_exp[c]);
-            double scale = sqrt(3.0 * (N - 1) * ph.kbt / (ph.mass_fl * var));
-            if (!(var > 0)) {
-                zero_var = true;
-                scale = 1.0;
-            }
-            float *m = f_m + 9 * c;
+            __syncthreads();
+            for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+                const int N = f_cnt[c];
+                if (N < 2 || f_exp[c] < 0) continue;
+                const double var = ldexp((double)(long long)f_sum[5 * c + 4], -f_exp[c]);
+                double scale = sqrt(3.0 * (N - 1) * ph.kbt / (ph.mass_fl * var));
+                if (!(var > 0)) {
+                    zero_var = true;
+                    scale = 1.0;
+                }
+                float *m = f_m + 9 * c;
 #pragma unroll
-            for (int q = 0; q < 9; q++) m[q] = (float)((double)m[q] * scale);
+                for (int q = 0; q < 9; q++) m[q] = (float)((double)m[q] * scale);
+            }
         }
     } else {
     block_exclusive_scan(s_start, ncell, s_wtot);
