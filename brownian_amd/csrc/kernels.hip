@@ -671,14 +671,16 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         // about it, FP32, were formed while the particles were on their way).
         // The relative energy sum |v - u|^2 (which the rotation leaves
         // unchanged) is sum |v|^2 - |sum v|^2 / N from the same sums, wherever
-        // that difference is known to 2^-26 of itself: its error is bounded by
-        // the sums' roundings (N 2^-29 for the |v|^2 terms, 2^-41 per velocity
-        // term) and the FP64 operations.  A cell where it is not (a cold cell,
-        // or one moving fast against its spread) takes the exact second pass:
-        // per particle |v - u|^2 in FP64, summed in the fixed point 2^(50 - e)
-        // with sum |v|^2 < 2^e, which bounds every term and their sum (each
-        // scaled term below 2^50 rounds to an integer by rint_small).  The
-        // thermostat factor is folded into M = scale R.
+        // that difference is known to 2^-26 of itself: its error is bounded
+        // (err, with a factor 2 of margin) by the sums' roundings (2^-29 per
+        // |v|^2 term, 2^-41 per velocity term) and the FP64 operations.  A
+        // cell where it is not (a cold cell, or one moving fast against its
+        // spread) takes the exact second pass: per particle |v - u|^2 in FP64,
+        // summed in the fixed point 2^(50 - e), where 2^e exceeds the one-pass
+        // value plus twice its error bound and so every term and their sum
+        // (each scaled term below 2^50 rounds to an integer by rint_small; the
+        // resolution is 2^-50 of the cell's relative energy, whatever |u|).
+        // The thermostat factor is folded into M = scale R.
         bool missing = false, second = false;
         for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
             const int N = f_cnt[c];
@@ -692,7 +694,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             f_u[3 * c] = sx * iN; f_u[3 * c + 1] = sy * iN; f_u[3 * c + 2] = sz * iN;
             const double S2 = (sx * sx + sy * sy) + sz * sz;
             const double var = E - S2 * iN;
-            const double err = N * 0x1p-29 + 0x1p-51 * E + 0x1p-39 * sqrt(S2) + 0x1p-50 * (S2 * iN);
+            const double err = N * 0x1p-28 + 0x1p-50 * E + 0x1p-38 * sqrt(S2) + 0x1p-49 * (S2 * iN);
             if (var > err * 0x1p26) {
                 const double scale = sqrt(3.0 * (N - 1) * ph.kbt / (ph.mass_fl * var));
                 float *m = f_m + 9 * c;
@@ -701,7 +703,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
                 f_exp[c] = -1;  // done
             } else {
                 int ex;
-                frexp(E + 0x1p-16, &ex);  // sum |v|^2 < 2^ex (2^-16 covers <= 4096 roundings of 2^-29)
+                frexp(fmax(var, 0.0) + 2.0 * err + 0x1p-40 * E + 0x1p-60, &ex);
                 f_exp[c] = 50 - ex;
                 second = true;
             }

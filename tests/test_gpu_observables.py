@@ -67,6 +67,42 @@ def test_thermostat_per_cell_closed_form(precision):
     assert np.abs(e[m] / (3.0 * (cnt[m] - 1)) - 1).max() < 10 * tol
 
 
+@pytest.mark.parametrize("spread", [1e-4, 1.0])
+def test_fp32_thermostat_closed_form_cold_and_drifting_cells(spread):
+    """FP32 storage with a common drift of (3, -2, 1) and a thermal spread of
+    1e-4 (every cell's relative energy far below what the exact cell sums
+    resolve, so each takes the exact per-particle pass) or 1 (most cells take
+    the one-pass form): each shifted cell with N > 1 keeps its momentum and
+    ends with relative kinetic energy 3 (N-1) kT / 2 (cpp/rotation_mpcd.cpp:37-49)."""
+    prm = dict(L=10, density=5, n_colloid=0, sig_colloid=2.5, sigma=2.0, mass_colloid=50.0, n_sub=10,
+               theta_deg=90.0, precision=B.BW_FP32, vel_law=B.BW_VEL_MAXWELL)
+    s = B.System(**prm)
+    s.startup()
+    pos, _, _ = s.download_fluid()
+    rng = np.random.default_rng(7)
+    vel = np.array([3.0, -2.0, 1.0]) + spread * rng.standard_normal(pos.shape)
+    s.upload_fluid(pos, vel)
+    _, v0, _ = s.download_fluid()
+    s.rotation_mpcd()
+    assert s.status() == 0
+    _, v1, _ = s.download_fluid()
+    cells = s.shifted_cells()
+    ncell = prm["L"] ** 3
+    cnt = np.bincount(cells, minlength=ncell)
+    for comp in range(3):
+        p0 = np.bincount(cells, weights=v0[:, comp], minlength=ncell)
+        p1 = np.bincount(cells, weights=v1[:, comp], minlength=ncell)
+        # one FP32 rounding of one particle per cell (the compensated residual)
+        assert (np.abs(p1 - p0) / np.maximum(cnt, 1)).max() < 2e-6
+    mean = np.stack([np.bincount(cells, weights=v1[:, k], minlength=ncell) for k in range(3)], 1)
+    mean /= np.maximum(cnt, 1)[:, None]
+    rel = ((v1 - mean[cells]) ** 2).sum(1)
+    e = np.bincount(cells, weights=rel, minlength=ncell)
+    m = cnt > 1
+    # FP32 velocities of magnitude ~3 around a relative speed ~1: ~1e-6 relative
+    assert np.abs(e[m] / (3.0 * (cnt[m] - 1)) - 1).max() < 2e-5
+
+
 def test_fluid_temperature_observable_matches_state():
     prm = dict(L=12, density=10, n_colloid=3, sig_colloid=4.2, sigma=4.0, mass_colloid=50.0, n_sub=20,
                theta_deg=90.0, v0=0.02, precision=B.BW_FP32, vel_law=B.BW_VEL_MAXWELL,
