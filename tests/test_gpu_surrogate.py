@@ -205,9 +205,12 @@ def test_surrogate_diffusivity_against_the_explicit_judge():
     5000-5007), run in a 64-replica ensemble of the batched configuration
     (seeds 1000+), plain and surrogate from one start-up state.  The paired
     pooled-MSD ratio and orientation-autocorrelation ratio at 500 steps lie
-    within 10 % of 1 (a regression bound: the 256-replica run on 2M pairs
-    measures an MSD ratio of 1.08 at 1000 steps, outside the 3 % target), and
-    the network drove most colloid-steps."""
+    within 10 % of 1 plus three of their paired standard errors (2-3 % for 64
+    replicas at 500 steps): a regression bound, not the 3 % target.  The
+    256-replica runs on 2M pairs measure MSD ratios of 1.07-1.09 at 1000 steps,
+    and networks trained on pairs that differ only in the fluid's last-bit
+    roundings differ by several per cent; a closure that loses momentum
+    (2.4x) fails it by far.  The network must drive most colloid-steps."""
     import bench
     common = dict(precision=B.BW_FP32, draws=B.BW_DRAWS_DEVICE, vel_law=B.BW_VEL_MAXWELL,
                   orient_law=B.BW_ORIENT_SPHERE, fluid_init=B.BW_INIT_DEVICE)
@@ -220,7 +223,7 @@ def test_surrogate_diffusivity_against_the_explicit_judge():
           [(m["ratio"], m["ratio_se"]) for m in out["msd"]], [(m["ratio"], m["ratio_se"]) for m in out["orient_corr"]])
     assert out["surrogate"]["surrogate_fraction"] > 0.9
     m, o = out["msd"][-1], out["orient_corr"][-1]
-    assert abs(m["ratio"] - 1) < 0.10, m
-    assert abs(o["ratio"] - 1) < 0.10, o
+    assert abs(m["ratio"] - 1) < 0.10 + 3 * m["ratio_se"], m
+    assert abs(o["ratio"] - 1) < 0.10 + 3 * o["ratio_se"], o
     # the colloids stay thermal under the closure
     assert abs(out["surrogate"]["t_colloid"] / out["plain"]["t_colloid"] - 1) < 0.1
