@@ -96,23 +96,61 @@ def common_candidates(Ls, density: int = 10, fp64: bool = False) -> list[tuple[i
     return sorted(set.intersection(*sets), key=lambda s: (s[0] * s[1] * s[2], s))
 
 
-def autotune_ensemble(replicas: list, steps: int = 5, warmup: int = 2, shapes=None, log=None) -> dict:
+def _time_ensemble(replicas: list, tiles: dict, steps: int, warmup: int) -> float:
+    """ms per fused step of the ensemble with tile shape tiles[L] for each box edge L."""
+    s = System(replicas=[dict(r, tile=tiles[r["L"]]) for r in replicas])
+    s.startup()
+    s.step(warmup)
+    s.status()
+    ms = s.time_steps(steps) / steps
+    s.status()
+    s.close()
+    return ms
+
+
+def autotune_ensemble(replicas: list, steps: int = 5, warmup: int = 2, shapes=None, per_box: bool = False,
+                      max_cells: int = 256, even: bool = False, log=None) -> dict:
     """Time `steps` fused steps of the whole ensemble (replica params as for
     System(replicas=...)) for each tile shape its box edges all accept;
-    returns {"best": shape, "ms_per_step": {shape: ms}}."""
+    returns {"best": shape, "ms_per_step": {shape: ms}}.
+
+    per_box: then refine the shape per box edge, one edge at a time from the
+    best common shape (every shape of at most max_cells cells that edge
+    accepts, the others held), keeping a change that is faster by more than
+    1 % (the timing's spread); adds {"per_box": {L: shape}, "per_box_ms": ms,
+    "per_box_tried": {"L:shape": ms}}.  The replicas then share one slot
+    capacity and SRD workgroup size (the largest tile's), which the timing
+    includes.  even: only shapes of even edges (the surrogate closure's 2x2x2
+    coarse cells need them)."""
     fp64 = replicas[0].get("precision", 0) == BW_FP64
-    shapes = shapes or common_candidates([r["L"] for r in replicas], replicas[0].get("density", 10), fp64)
+    density = replicas[0].get("density", 10)
+    shapes = shapes or common_candidates([r["L"] for r in replicas], density, fp64)
     times = {}
     for shape in shapes:
         t0 = time.time()
-        s = System(replicas=[dict(r, tile=shape) for r in replicas])
-        s.startup()
-        s.step(warmup)
-        s.status()
-        times[shape] = s.time_steps(steps) / steps
-        s.status()
-        s.close()
+        times[shape] = _time_ensemble(replicas, {r["L"]: shape for r in replicas}, steps, warmup)
         if log:
             log(f"ensemble tile {shape}: {times[shape]:.3f} ms/step ({time.time() - t0:.1f} s)")
     best = min(times, key=times.get)
-    return {"best": best, "ms_per_step": times}
+    out = {"best": best, "ms_per_step": times}
+    if per_box:
+        Ls = sorted(set(r["L"] for r in replicas))
+        assign = {L: best for L in Ls}
+        best_ms = times[best]
+        tried = {}
+        for L in Ls:
+            for shape in candidates(L, density, fp64):
+                if shape == assign[L] or shape[0] * shape[1] * shape[2] > max_cells:
+                    continue
+                if even and any(e % 2 for e in shape):
+                    continue
+                trial = dict(assign)
+                trial[L] = shape
+                ms = _time_ensemble(replicas, trial, steps, warmup)
+                tried[f"{L}:{shape}"] = ms
+                if log:
+                    log(f"ensemble tile {shape} for L={L}: {ms:.3f} ms/step")
+                if ms < 0.99 * best_ms:
+                    assign, best_ms = trial, ms
+        out.update(per_box=assign, per_box_ms=best_ms, per_box_tried=tried)
+    return out

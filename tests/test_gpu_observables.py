@@ -155,6 +155,37 @@ def test_autotuner_picks_a_candidate_with_identical_results():
     np.testing.assert_array_equal(va, vb)
 
 
+def test_ensemble_autotuner_per_box_shapes_give_identical_results():
+    """The ensemble tuner's per-box-edge refinement returns, for each edge, a
+    shape that edge accepts; the ensemble stepped with those shapes (one slot
+    capacity and SRD workgroup size for all, the largest tile's) gives the
+    common shape's trajectory bit for bit."""
+    from brownian_amd.autotune import autotune_ensemble, candidates
+    reps = [dict(L=L, density=10, n_colloid=nc, n_sub=20, sig_colloid=4.2, sigma=4.0, mass_colloid=50.0,
+                 theta_deg=90.0, v0=0.03, seed=1100 + i, precision=B.BW_FP32, vel_law=B.BW_VEL_MAXWELL,
+                 orient_law=B.BW_ORIENT_SPHERE)
+            for i, (L, nc) in enumerate([(16, 3), (24, 8), (16, 2)])]
+    res = autotune_ensemble(reps, steps=2, warmup=1, per_box=True, max_cells=128)
+    assert set(res["per_box"]) == {16, 24}
+    for L, shape in res["per_box"].items():
+        assert shape in candidates(L, 10)
+    # a mixed assignment, whatever the timing chose
+    mixed = {16: (4, 4, 4), 24: (8, 4, 4)}
+    a = B.System(replicas=[dict(r, tile=res["best"]) for r in reps])
+    b = B.System(replicas=[dict(r, tile=mixed[r["L"]]) for r in reps])
+    a.startup(); b.startup()
+    a.step(3); b.step(3)
+    assert a.status() == 0 and b.status() == 0
+    for r in range(len(reps)):
+        pa, va, _ = a.download_fluid(r)
+        pb, vb, _ = b.download_fluid(r)
+        np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_array_equal(va, vb)
+        ca, cb = a.download_colloid(r), b.download_colloid(r)
+        for k in ca:
+            np.testing.assert_array_equal(ca[k], cb[k], err_msg=f"replica {r} {k}")
+
+
 def test_small_ensemble_of_mixed_boxes_equals_single_runs():
     """Replicas of different edge, colloid count and v0 stepped together give
     each replica's single-system result bit for bit."""
