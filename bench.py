@@ -316,8 +316,8 @@ def main():
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.out, args.command)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; a multi-GPU figure is not produced "
-              "(the slab decomposition is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py: the step runs on one GPU; a multi-GPU figure is not produced "
+              "(replicas or box slabs would be split over the GPUs).", file=sys.stderr)
         sys.exit(2)
 
     import brownian_amd as B

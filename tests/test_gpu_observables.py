@@ -121,7 +121,7 @@ def test_fluid_temperature_observable_matches_state():
     assert -1.0 <= o["orient_corr"] <= 1.0 and o["msd"] > 0
 
 
-def test_fp32_error_report_within_north_star_tolerance():
+def test_fp32_error_report_within_tolerance():
     """FP32 storage vs an FP64 run from the same state: per-step errors far
     below 1e-5 relative, apart from discrete collision decisions within a
     rounding distance of a colloid surface (counted by the report)."""

@@ -198,8 +198,8 @@ def test_surrogate_ensemble_is_reproducible():
 
 
 @pytest.mark.timeout(300)
-def test_surrogate_diffusivity_against_the_explicit_judge():
-    """The closure against its judge, the explicit collisions
+def test_surrogate_diffusivity_against_the_explicit_collisions():
+    """The closure against the explicit collisions
     (cpp/fluid_colloid_collision.cpp:26-49).  Trained on 2M pairs from
     single-system-style runs (64^3 x 10, 300 colloids, v0 0.02, seeds
     5000-5007), run in a 64-replica ensemble of the batched configuration
