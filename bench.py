@@ -219,6 +219,17 @@ def ensemble_replicas(B, n_rep, tile=None, seed0=1000):
     return reps
 
 
+def parse_tiles(text):
+    """'16:8x4x4,48:6x6x4' -> {16: (8, 4, 4), 48: (6, 6, 4)}; None -> None."""
+    if not text:
+        return None
+    out = {}
+    for item in text.split(","):
+        L, shape = item.split(":")
+        out[int(L)] = tuple(int(x) for x in shape.split("x"))
+    return out
+
+
 def run_ensemble(B, n_rep, steps, warmup, tile=None):
     reps = ensemble_replicas(B, n_rep, tile)
     t0 = time.time()
@@ -294,6 +305,10 @@ def main():
     ap.add_argument("--ensemble", type=int, default=256, help="replicas of the batched run (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True)
+    ap.add_argument("--ensemble-tiles", metavar="L:BXxBYxBZ,...",
+                    help="the ensemble's tile shape per box edge (skips its autotuning), e.g. 16:8x4x4,48:6x6x4")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="time only the batched ensemble (one JSON line of its figures)")
     ap.add_argument("--tile", type=int, nargs=3, metavar=("BX", "BY", "BZ"),
                     help="tile shape (skips the autotuner)")
     ap.add_argument("--surrogate", action=argparse.BooleanOptionalAction, default=True)
@@ -321,6 +336,10 @@ def main():
         sys.exit(2)
 
     import brownian_amd as B
+    if args.ensemble_only:
+        ens = run_ensemble(B, args.ensemble, args.ensemble_steps, 5, parse_tiles(args.ensemble_tiles))
+        print(json.dumps({"ensemble": ens}), flush=True)
+        return
     prm = workload_params(args)
     t0 = time.time()
     mode = B.BW_STREAM_BLOCKED if args.streaming == "blocked" else B.BW_STREAM_PER_SUBSTEP
@@ -440,7 +459,9 @@ def main():
     if args.ensemble > 0:
         ens_tile, ens_tune = None, None
         try:
-            if args.autotune:
+            if args.ensemble_tiles:
+                ens_tile = parse_tiles(args.ensemble_tiles)
+            elif args.autotune:
                 from brownian_amd.autotune import autotune_ensemble
                 log("autotuning the ensemble's tile shape")
                 ens_tune = autotune_ensemble(ensemble_replicas(B, args.ensemble), per_box=True,
