@@ -31,8 +31,9 @@ def candidates(L: int, density: int, fp64: bool = False) -> list[tuple[int, int,
     """Tile shapes the library accepts for a box of edge L: each edge divides
     L and is at least 4, at most 512 cells, the slots and the SRD pass's LDS
     (per-cell FP64 sums, per-slot staging) fit, the region fits the masks.
-    The SRD workgroup follows the slots (4 per thread: 256, 512 or 1024
-    threads), so smaller tiles let several workgroups share a CU."""
+    The SRD workgroup follows the slots (3 or 4 per thread, 256, 512 or
+    1024 threads: the fewest slots that hold the tile), so smaller tiles let
+    several workgroups share a CU."""
     divs = [b for b in range(4, min(L, 16) + 1) if L % b == 0] or [L]
     rb = 8 if fp64 else 4
     out = []
@@ -109,7 +110,7 @@ def _time_ensemble(replicas: list, tiles: dict, steps: int, warmup: int) -> floa
 
 
 def autotune_ensemble(replicas: list, steps: int = 5, warmup: int = 2, shapes=None, per_box: bool = False,
-                      max_cells: int = 256, even: bool = False, log=None) -> dict:
+                      max_cells: int = 256, even: bool = False, passes: int = 2, log=None) -> dict:
     """Time `steps` fused steps of the whole ensemble (replica params as for
     System(replicas=...)) for each tile shape its box edges all accept;
     returns {"best": shape, "ms_per_step": {shape: ms}}.
@@ -117,8 +118,10 @@ def autotune_ensemble(replicas: list, steps: int = 5, warmup: int = 2, shapes=No
     per_box: then refine the shape per box edge, one edge at a time from the
     best common shape (every shape of at most max_cells cells that edge
     accepts, the others held), keeping a change that is faster by more than
-    1 % (the timing's spread); adds {"per_box": {L: shape}, "per_box_ms": ms,
-    "per_box_tried": {"L:shape": ms}}.  The replicas then share one slot
+    1 % (the timing's spread), in passes over the edges until one changes
+    nothing (at most `passes`: a change can pay only once another has been
+    made, when the largest tile sets every replica's slot capacity); adds
+    {"per_box": {L: shape}, "per_box_ms": ms, "per_box_tried": {"L:shape": ms}}.  The replicas then share one slot
     capacity and SRD workgroup size (the largest tile's), which the timing
     includes.  even: only shapes of even edges (the surrogate closure's 2x2x2
     coarse cells need them)."""
@@ -138,19 +141,23 @@ def autotune_ensemble(replicas: list, steps: int = 5, warmup: int = 2, shapes=No
         assign = {L: best for L in Ls}
         best_ms = times[best]
         tried = {}
-        for L in Ls:
-            for shape in candidates(L, density, fp64):
-                if shape == assign[L] or shape[0] * shape[1] * shape[2] > max_cells:
-                    continue
-                if even and any(e % 2 for e in shape):
-                    continue
-                trial = dict(assign)
-                trial[L] = shape
-                ms = _time_ensemble(replicas, trial, steps, warmup)
-                tried[f"{L}:{shape}"] = ms
-                if log:
-                    log(f"ensemble tile {shape} for L={L}: {ms:.3f} ms/step")
-                if ms < 0.99 * best_ms:
-                    assign, best_ms = trial, ms
+        for _ in range(passes):
+            changed = False
+            for L in Ls:
+                for shape in candidates(L, density, fp64):
+                    if shape == assign[L] or shape[0] * shape[1] * shape[2] > max_cells:
+                        continue
+                    if even and any(e % 2 for e in shape):
+                        continue
+                    trial = dict(assign)
+                    trial[L] = shape
+                    ms = _time_ensemble(replicas, trial, steps, warmup)
+                    tried[f"{L}:{shape}"] = ms
+                    if log:
+                        log(f"ensemble tile {shape} for L={L}: {ms:.3f} ms/step")
+                    if ms < 0.99 * best_ms:
+                        assign, best_ms, changed = trial, ms, True
+            if not changed:
+                break
         out.update(per_box=assign, per_box_ms=best_ms, per_box_tried=tried)
     return out

@@ -555,7 +555,15 @@ int create_impl(const bw_params *ps, int n_rep, bw_ctx **out) {
         return BW_ERR_ARG;
     }
     ph.cap = cap;
-    ph.srd_threads = cap <= 4 * 256 ? 256 : cap <= 4 * 512 ? 512 : 1024;
+    // the SRD workgroup: the fewest slots (threads x slots per thread) that
+    // hold the tile's capacity, 768 / 1024 / 1536 / 2048 / 3072 / 4096
+    {
+        static const int shape[6][2] = {{256, 3}, {256, 4}, {512, 3}, {512, 4}, {1024, 3}, {1024, 4}};
+        int k = 0;
+        while (k < 5 && cap > shape[k][0] * shape[k][1]) k++;
+        ph.srd_threads = shape[k][0];
+        ph.srd_pt = shape[k][1];
+    }
     ph.near_wrap = 0;
     for (auto &R : c->reps)
         if (srd_needs_wrap(R.L, std::max(R.B[0], std::max(R.B[1], R.B[2])))) ph.near_wrap = 1;

@@ -27,7 +27,6 @@ constexpr int CELL_BITS = 10;       // packed cell coordinates: L <= 1023
 constexpr int MD_CAP = 128;         // full MD neighbour list per colloid
 constexpr int MDC_CAP = 256;        // MD candidates (cutoff + skin) per colloid
 constexpr int MAX_CAP = 4096;       // slots per tile upper bound
-constexpr int SRD_PT = 4;           // slots per thread in the SRD kernel (256, 512 or 1024 threads)
 constexpr int STREAM_THREADS = 256;
 constexpr int NEAR_BLOCKS = 2048;    // near-particle pass: at most this many workgroups of 4 waves
 constexpr int DEPART_THREADS = 256;
@@ -79,7 +78,8 @@ struct Phys {
     int surrogate;                    // colloids may be driven by the surrogate impulses
     int n_sub;
     int cap;                          // slots per tile
-    int srd_threads;                  // SRD workgroup size (4 slots per thread)
+    int srd_threads;                  // SRD workgroup size
+    int srd_pt;                       // SRD slots per thread (srd_threads x srd_pt >= cap)
     int near_blocks;                  // grid of the near-particle pass
     int near_wrap;                    // the SRD near test needs the periodic wrap (a small box)
     int n_rep, n_tiles, n_colloid_total;

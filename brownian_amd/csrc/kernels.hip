@@ -361,7 +361,8 @@ constexpr size_t SRD_SUR_LDS = (size_t)TCL_CAP * (48 + 8 * SUR_ACC + 12 + 4 + 4)
 
 // WRAP: the near test applies the periodic wrap to its FP32 differences
 // (srd_needs_wrap); otherwise they stay below L/2, where the wrap is the identity.
-template <class Real, bool COUNT_SHELL, bool MAP, bool BLOCK, int TH, bool WRAP>
+// PT: slots per thread (3 or 4; TH x PT slots hold the tile's capacity).
+template <class Real, bool COUNT_SHELL, bool MAP, bool BLOCK, int TH, bool WRAP, int PT>
 __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Fluid<Real> dst) {
     constexpr bool NEAR_TEST = COUNT_SHELL && BLOCK;
     // FP32 storage: per-cell sums by exact fixed-point LDS atomics (no sort);
@@ -374,7 +375,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     __shared__ NearColl nc[1];  // used by the near test only
     __shared__ int s_shell[COUNT_SHELL ? TCL_CAP : 1];
     __shared__ AxisMask s_am[COUNT_SHELL ? 2 : 1];  // [0] run shells, [1] near test
-    __shared__ int s_cc[SRD_PT][TH / 64];   // near particles per chunk and wave
+    __shared__ int s_cc[PT][TH / 64];   // near particles per chunk and wave
     __shared__ int s_sn;
     const int T = blockIdx.x;
     Rep R;
@@ -507,12 +508,12 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     // particle (of slot 0 of an in-range index for an empty tile): the loads
     // are straight runs, each with one wait after it, and the values are
     // never used
-    int rcell[SRD_PT], rpid[SRD_PT], rkey[SRD_PT], rslot[SRD_PT];
-    Real rox[SRD_PT], roy[SRD_PT], roz[SRD_PT], rvx[SRD_PT], rvy[SRD_PT], rvz[SRD_PT];
+    int rcell[PT], rpid[PT], rkey[PT], rslot[PT];
+    Real rox[PT], roy[PT], roz[PT], rvx[PT], rvy[PT], rvz[PT];
     {
         const int g_last = max(n - 1, 0), idx_last = ph.n_tiles * cap - 1;
 #pragma unroll
-        for (int k = 0; k < SRD_PT; k++) {
+        for (int k = 0; k < PT; k++) {
             const int g = min((int)threadIdx.x + k * TH, g_last);
             int lo = 0;
 #pragma unroll
@@ -522,12 +523,12 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
             rslot[k] = min(rkey[k] + s_base[lo] + (g - s_pre[lo]), idx_last);
         }
 #pragma unroll
-        for (int k = 0; k < SRD_PT; k++) rslot[k] = d.dep_idx[rslot[k]];
+        for (int k = 0; k < PT; k++) rslot[k] = d.dep_idx[rslot[k]];
 #pragma unroll
-        for (int k = 0; k < SRD_PT; k++) rslot[k] += rkey[k];
+        for (int k = 0; k < PT; k++) rslot[k] += rkey[k];
     }
 #pragma unroll
-    for (int k = 0; k < SRD_PT; k++) rkey[k] = -1;
+    for (int k = 0; k < PT; k++) rkey[k] = -1;
     const double Ld = (double)L;
     if (cl_mine) {
         const int q = threadIdx.x;
@@ -548,7 +549,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     }
     if (COUNT_SHELL && !cl_stale && threadIdx.x == 0) tc.n = cl_n;
 #pragma unroll
-    for (int k = 0; k < SRD_PT; k++) {
+    for (int k = 0; k < PT; k++) {
         const int slot = rslot[k];
         rcell[k] = src.cell[slot];
         rox[k] = src.ox[slot]; roy[k] = src.oy[slot]; roz[k] = src.oz[slot];
@@ -624,7 +625,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     // shifted cells and the per-cell histogram
     bool drift = false;
 #pragma unroll
-    for (int k = 0; k < SRD_PT; k++) {
+    for (int k = 0; k < PT; k++) {
         int g = threadIdx.x + k * TH;
         if (g < n) {
             int ix, iy, iz;
@@ -711,7 +712,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         if (missing) atomicOr(d.status, BW_FLAG_DRAW_MISSING);
         if (__syncthreads_or(second)) {  // some cell of the tile needs the exact pass
 #pragma unroll
-            for (int k = 0; k < SRD_PT; k++) {
+            for (int k = 0; k < PT; k++) {
                 if (threadIdx.x + k * TH < n) {
                     const int c = rkey[k];
                     if (f_cnt[c] > 1 && f_exp[c] >= 0) {
@@ -744,14 +745,14 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     for (int c = threadIdx.x; c < ncell; c += blockDim.x) s_fill[c] = s_start[c];
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < SRD_PT; k++) {
+    for (int k = 0; k < PT; k++) {
         int g = threadIdx.x + k * TH;
         if (g < n) s_pk[atomicAdd(&s_fill[rkey[k]], 1)] = rpid[k];
     }
     __syncthreads();
     // rank of each particle in its cell by id: the cell's canonical order
 #pragma unroll
-    for (int k = 0; k < SRD_PT; k++) {
+    for (int k = 0; k < PT; k++) {
         int g = threadIdx.x + k * TH;
         if (g < n) {
             const int b = s_start[rkey[k]], e = s_start[rkey[k] + 1];
@@ -795,7 +796,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     __syncthreads();  // the velocities in LDS are no longer needed: energy terms overwrite them
     // per particle: rotated velocity relative to the mean, its squared norm (:37-41)
 #pragma unroll
-    for (int k = 0; k < SRD_PT; k++) {
+    for (int k = 0; k < PT; k++) {
         int g = threadIdx.x + k * TH;
         if (g < n) {
             const int c = rkey[k];
@@ -868,7 +869,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
     // apply; count run() shells; near test
     unsigned nearbits = 0;
 #pragma unroll
-    for (int k = 0; k < SRD_PT; k++) {
+    for (int k = 0; k < PT; k++) {
         int g = threadIdx.x + k * TH;
         if (g < n) {
             int c = rkey[k];
@@ -1074,14 +1075,14 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         }
     }
     // output slots: near particles first, each group in gather order
-    int nrk[SRD_PT];  // near particles of lower gather index
+    int nrk[PT];  // near particles of lower gather index
     int n_near = n;
     if (BLOCK) {
         const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const unsigned long long lt = (1ull << lane) - 1ull;
-        unsigned long long bal[SRD_PT];
+        unsigned long long bal[PT];
 #pragma unroll
-        for (int k = 0; k < SRD_PT; k++) {
+        for (int k = 0; k < PT; k++) {
             bal[k] = __ballot((threadIdx.x + k * TH < n) && (nearbits & (1u << k)));
             if (lane == 0) s_cc[k][w] = __popcll(bal[k]);
         }
@@ -1093,7 +1094,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         }
         int base = 0;
 #pragma unroll
-        for (int k = 0; k < SRD_PT; k++) {
+        for (int k = 0; k < PT; k++) {
             int before = 0, tot = 0;
             for (int v = 0; v < TH / 64; v++) {
                 int cnt = s_cc[k][v];
@@ -1112,7 +1113,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         // conserves it exactly)
         if (!BLOCK) __syncthreads();
 #pragma unroll
-        for (int k = 0; k < SRD_PT; k++) {
+        for (int k = 0; k < PT; k++) {
             if (threadIdx.x + k * TH < n) {
                 const int c = rkey[k];
                 if (f_cnt[c] > 1 && rpid[k] == f_minpid[c]) {
@@ -1139,7 +1140,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         // less than a cell in the step, so each axis crosses a cell face at
         // most once and the periodic wrap of the cell index comes once, after.
 #pragma unroll
-        for (int k = 0; k < SRD_PT; k++) {
+        for (int k = 0; k < PT; k++) {
             if (threadIdx.x + k * TH < n && !(nearbits & (1u << k))) {
                 int ix, iy, iz;
                 unpack_cell(rcell[k], ix, iy, iz);
@@ -1183,7 +1184,7 @@ __global__ __launch_bounds__(TH) void k_srd(Phys ph, Dev d, Fluid<Real> src, Flu
         for (int j = threadIdx.x; j < (n_near + 63) >> 6; j += 64) d.chunk_tile[chunk0 + j] = (T << 6) | j;
     }
 #pragma unroll
-    for (int k = 0; k < SRD_PT; k++) {
+    for (int k = 0; k < PT; k++) {
         int g = threadIdx.x + k * TH;
         if (g < n) {
             int slot_in_tile = g;
@@ -2545,10 +2546,13 @@ void launch_depart(const Phys &ph, const Dev &d, const Fluid<Real> &f, hipStream
 template <class Real, bool C, bool M, bool B, bool W>
 static void srd_th(const Phys &ph, const Dev &d, const Fluid<Real> &src, const Fluid<Real> &dst, size_t lds,
                    hipStream_t st) {
-    switch (ph.srd_threads) {
-        case 256: LAUNCH((k_srd<Real, C, M, B, 256, W>), ph.n_tiles, 256, lds, st, ph, d, src, dst); break;
-        case 512: LAUNCH((k_srd<Real, C, M, B, 512, W>), ph.n_tiles, 512, lds, st, ph, d, src, dst); break;
-        default: LAUNCH((k_srd<Real, C, M, B, 1024, W>), ph.n_tiles, 1024, lds, st, ph, d, src, dst); break;
+    switch (ph.srd_threads * 8 + ph.srd_pt) {
+        case 256 * 8 + 3: LAUNCH((k_srd<Real, C, M, B, 256, W, 3>), ph.n_tiles, 256, lds, st, ph, d, src, dst); break;
+        case 256 * 8 + 4: LAUNCH((k_srd<Real, C, M, B, 256, W, 4>), ph.n_tiles, 256, lds, st, ph, d, src, dst); break;
+        case 512 * 8 + 3: LAUNCH((k_srd<Real, C, M, B, 512, W, 3>), ph.n_tiles, 512, lds, st, ph, d, src, dst); break;
+        case 512 * 8 + 4: LAUNCH((k_srd<Real, C, M, B, 512, W, 4>), ph.n_tiles, 512, lds, st, ph, d, src, dst); break;
+        case 1024 * 8 + 3: LAUNCH((k_srd<Real, C, M, B, 1024, W, 3>), ph.n_tiles, 1024, lds, st, ph, d, src, dst); break;
+        default: LAUNCH((k_srd<Real, C, M, B, 1024, W, 4>), ph.n_tiles, 1024, lds, st, ph, d, src, dst); break;
     }
 }
 template <class Real, bool C, bool M, bool B>
