@@ -128,6 +128,10 @@ def autotune_ensemble(replicas: list, steps: int = 5, warmup: int = 2, shapes=No
     fp64 = replicas[0].get("precision", 0) == BW_FP64
     density = replicas[0].get("density", 10)
     shapes = shapes or common_candidates([r["L"] for r in replicas], density, fp64)
+    if even:
+        shapes = [s for s in shapes if not any(e % 2 for e in s)]
+        if not shapes:
+            raise ValueError("no tile shape with even edges divides every box edge of the ensemble")
     times = {}
     for shape in shapes:
         t0 = time.time()

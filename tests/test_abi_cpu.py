@@ -82,3 +82,20 @@ def test_create_rejects_bad_parameters_without_gpu_state():
     # L < 7 would give duplicate stencil cells (cpp/neighbour_list.cpp:12-25)
     with pytest.raises(B.BrownianError):
         B.System(L=6, density=2, n_colloid=0)
+
+
+def test_autotuner_candidate_shapes_divide_every_box_edge():
+    """The tile shapes the autotuners try: each divides its box edge (one
+    shape for all the edges of an ensemble), and with `even` the ensemble
+    search refuses edge sets with no even-edged common shape before it starts
+    any system (the surrogate's 2x2x2 coarse cells need even edges)."""
+    from brownian_amd.autotune import autotune_ensemble, candidates, common_candidates
+    for L in (16, 24, 32, 40, 48):
+        for shape in candidates(L, 10):
+            assert all(L % e == 0 for e in shape), (L, shape)
+    common = common_candidates([16, 24, 32, 40, 48], 10)
+    assert common and all(all(L % e == 0 for e in s) for s in common for L in (16, 24, 32, 40, 48))
+    assert all(e % 2 == 0 for s in common for e in s)
+    assert common_candidates([20, 30], 10) and any(e % 2 for s in common_candidates([20, 30], 10) for e in s)
+    with pytest.raises(ValueError):
+        autotune_ensemble([dict(L=20), dict(L=30)], even=True)
