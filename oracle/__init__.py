@@ -30,6 +30,10 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 
+FLANG = "/opt/rocm/llvm/bin/flang"
+TWIN_EXE = os.path.join(HERE, "_ref", "fortran_twin")
+
+
 def build(with_reference: bool | None = None) -> None:
     """Compile the oracle (and the reference when its sources are present)."""
     subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
@@ -37,6 +41,9 @@ def build(with_reference: bool | None = None) -> None:
         with_reference = os.path.isdir("/root/reference/cpp")
     if with_reference:
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        # the Fortran twin, where its sources and a Fortran compiler are present
+        if os.path.isdir("/root/reference/fortran") and os.path.exists(FLANG):
+            subprocess.run(["make", "-s", "-C", HERE, "twin", f"FC={FLANG}"], check=True)
 
 
 class Params(C.Structure):

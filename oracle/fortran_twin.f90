@@ -1,0 +1,45 @@
+! Driver for the reference's Fortran twin (fortran/*.f90), compiled with its
+! sources where they lie (oracle/Makefile, target `twin`) into oracle/_ref/.
+! Test infrastructure only: it lets the tests compare the Fortran program's
+! generator and colloid stencil with the C++ reference and the oracle.
+!
+!   fortran_twin ran1 <seed> <n> <out>   n draws of ran1 (fortran/ran1.f90) from
+!                                        <seed>, as raw float64
+!   fortran_twin create_box <out>        nbox (int32), then box_neigh(1:nbox, cell)
+!                                        for every cell of the twin's 30^3 box
+!                                        (fortran/create_box.f90), raw int32,
+!                                        1-based, cells in the twin's order
+program fortran_twin
+    use all_parameters
+    implicit none
+    double precision :: ran1
+    character(len=256) :: mode, a1, a2, path
+    integer :: seed, ndraw, i, u
+    double precision, allocatable :: draws(:)
+
+    call get_command_argument(1, mode)
+    if (trim(mode) == 'ran1') then
+        call get_command_argument(2, a1)
+        call get_command_argument(3, a2)
+        call get_command_argument(4, path)
+        read (a1, *) seed
+        read (a2, *) ndraw
+        allocate (draws(ndraw))
+        do i = 1, ndraw
+            draws(i) = ran1(seed)
+        end do
+        open (newunit=u, file=trim(path), access='stream', form='unformatted', status='replace')
+        write (u) draws
+        close (u)
+    else if (trim(mode) == 'create_box') then
+        call get_command_argument(2, path)
+        call create_box
+        open (newunit=u, file=trim(path), access='stream', form='unformatted', status='replace')
+        write (u) nbox
+        write (u) box_neigh(1:nbox, 1:lx*ly*lz)
+        close (u)
+    else
+        write (*, *) 'usage: fortran_twin ran1 <seed> <n> <out> | create_box <out>'
+        stop 2
+    end if
+end program fortran_twin

@@ -139,3 +139,30 @@ def test_overlapping_colloids_sequential_semantics():
     o.step()
     state_eq(o, g, "step1_")
     assert o.potential() == g["step1_potential"][0]
+
+
+def test_fortran_twin_generator_and_stencil(tmp_path):
+    """The reference's Fortran twin as a second oracle (fortran/ran1.f90 and
+    fortran/create_box.f90, compiled from the reference's sources with the
+    flags of fortran/Makefile:3 by oracle/Makefile's `twin` target): its
+    generator gives the C++ reference's stream bit for bit (the golden draws
+    from seeds 77777 and -12345, cpp/parameters.hpp:16-42), and its colloid
+    stencil of the 30^3 box is the oracle's create_box list, entry for entry
+    (cpp/neighbour_list.cpp:3-25: the 342 cells of the 7^3 cube but the
+    colloid's own, in the same order)."""
+    import subprocess
+    if not os.path.exists(O.TWIN_EXE):
+        pytest.skip("Fortran twin not built (needs the reference's fortran/ sources and flang)")
+    g, _ = load("rng")
+    for seed, key in ((77777, "draws_77777"), (-12345, "draws_m12345")):
+        out = tmp_path / f"ran1_{seed}.bin"
+        subprocess.run([O.TWIN_EXE, "ran1", str(seed), str(len(g[key])), str(out)], check=True)
+        np.testing.assert_array_equal(np.fromfile(out, dtype="<f8"), g[key])
+    out = tmp_path / "box.bin"
+    subprocess.run([O.TWIN_EXE, "create_box", str(out)], check=True)
+    raw = np.fromfile(out, dtype="<i4")
+    nbox = int(raw[0])
+    o = O.Oracle(L=30, density=1, n_colloid=1)
+    o.create_box()
+    assert nbox == 342
+    np.testing.assert_array_equal(raw[1:].reshape(30 ** 3, nbox) - 1, o.box_neigh())
