@@ -9,6 +9,18 @@
 !                                        for every cell of the twin's 30^3 box
 !                                        (fortran/create_box.f90), raw int32,
 !                                        1-based, cells in the twin's order
+!   fortran_twin stream_bin <in> <out>   reads pos_fl, vel_fl (3 per fluid
+!                                        particle) and pos_colloid (3 per
+!                                        colloid) as raw float64, streams one
+!                                        substep (fortran/update_pos_mpcd.f90)
+!                                        and bins (fortran/neighbour_list_mpcd.f90);
+!                                        writes pos_fl (float64), then no_neigh
+!                                        and neigh_fl(1:10000, colloid) (int32)
+!   fortran_twin rotate <in> <out>       reads pos_fl and vel_fl, applies one SRD
+!                                        rotation with its thermostat
+!                                        (fortran/rotation_mpcd.f90; the shift and
+!                                        axes drawn by ran1 from the twin's seed
+!                                        77777), writes vel_fl
 program fortran_twin
     use all_parameters
     implicit none
@@ -38,8 +50,37 @@ program fortran_twin
         write (u) nbox
         write (u) box_neigh(1:nbox, 1:lx*ly*lz)
         close (u)
+    else if (trim(mode) == 'stream_bin') then
+        call get_command_argument(2, a1)
+        call get_command_argument(3, path)
+        open (newunit=u, file=trim(a1), access='stream', form='unformatted', status='old')
+        read (u) pos_fl
+        read (u) vel_fl
+        read (u) pos_colloid
+        close (u)
+        call create_box
+        call update_pos_mpcd
+        call neighbour_list_mpcd
+        open (newunit=u, file=trim(path), access='stream', form='unformatted', status='replace')
+        write (u) pos_fl
+        write (u) no_neigh
+        write (u) neigh_fl
+        close (u)
+    else if (trim(mode) == 'rotate') then
+        call get_command_argument(2, a1)
+        call get_command_argument(3, path)
+        open (newunit=u, file=trim(a1), access='stream', form='unformatted', status='old')
+        read (u) pos_fl
+        read (u) vel_fl
+        close (u)
+        nn = 1
+        call rotation_mpcd
+        open (newunit=u, file=trim(path), access='stream', form='unformatted', status='replace')
+        write (u) vel_fl
+        close (u)
     else
-        write (*, *) 'usage: fortran_twin ran1 <seed> <n> <out> | create_box <out>'
+        write (*, *) 'usage: fortran_twin ran1 <seed> <n> <out> | create_box <out> | stream_bin <in> <out>'
+        write (*, *) '       | rotate <in> <out>'
         stop 2
     end if
 end program fortran_twin

@@ -141,6 +141,17 @@ def test_overlapping_colloids_sequential_semantics():
     assert o.potential() == g["step1_potential"][0]
 
 
+def _run_twin(*args):
+    """Run the Fortran twin with a 256 MiB stack: its binning and rotation keep
+    cell lists of 100 x 27,000 entries in local arrays."""
+    import resource
+    import subprocess
+
+    def big_stack():
+        resource.setrlimit(resource.RLIMIT_STACK, (256 << 20, resource.RLIM_INFINITY))
+    subprocess.run([O.TWIN_EXE, *map(str, args)], check=True, preexec_fn=big_stack)
+
+
 def test_fortran_twin_generator_and_stencil(tmp_path):
     """The reference's Fortran twin as a second oracle (fortran/ran1.f90 and
     fortran/create_box.f90, compiled from the reference's sources with the
@@ -150,19 +161,78 @@ def test_fortran_twin_generator_and_stencil(tmp_path):
     stencil of the 30^3 box is the oracle's create_box list, entry for entry
     (cpp/neighbour_list.cpp:3-25: the 342 cells of the 7^3 cube but the
     colloid's own, in the same order)."""
-    import subprocess
     if not os.path.exists(O.TWIN_EXE):
         pytest.skip("Fortran twin not built (needs the reference's fortran/ sources and flang)")
     g, _ = load("rng")
     for seed, key in ((77777, "draws_77777"), (-12345, "draws_m12345")):
         out = tmp_path / f"ran1_{seed}.bin"
-        subprocess.run([O.TWIN_EXE, "ran1", str(seed), str(len(g[key])), str(out)], check=True)
+        _run_twin("ran1", seed, len(g[key]), out)
         np.testing.assert_array_equal(np.fromfile(out, dtype="<f8"), g[key])
     out = tmp_path / "box.bin"
-    subprocess.run([O.TWIN_EXE, "create_box", str(out)], check=True)
+    _run_twin("create_box", out)
     raw = np.fromfile(out, dtype="<i4")
     nbox = int(raw[0])
     o = O.Oracle(L=30, density=1, n_colloid=1)
     o.create_box()
     assert nbox == 342
     np.testing.assert_array_equal(raw[1:].reshape(30 ** 3, nbox) - 1, o.box_neigh())
+
+
+def test_fortran_twin_streaming_binning_and_rotation(tmp_path):
+    """The Fortran twin's streaming with the periodic wrap
+    (fortran/update_pos_mpcd.f90), its per-colloid fluid lists
+    (fortran/neighbour_list_mpcd.f90) and its SRD rotation at 130 degrees with
+    the thermostat (fortran/rotation_mpcd.f90, shift and axes drawn by ran1
+    from seed 77777) equal the oracle's cpp restatement bit for bit on the
+    twin's compile-time system (30^3 cells, 5 particles per cell, 10
+    colloids), from one random state with particles crossing the faces."""
+    if not os.path.exists(O.TWIN_EXE):
+        pytest.skip("Fortran twin not built (needs the reference's fortran/ sources and flang)")
+    rng = np.random.default_rng(5)
+    L, dens, nc = 30, 5, 10
+    nf = L ** 3 * dens
+    pos = rng.uniform(0, L, (nf, 3))
+    vel = rng.uniform(-1.7, 1.7, (nf, 3))
+    pc = rng.uniform(0, L, (nc, 3))
+    pos[:200, 0] = rng.uniform(0, 0.01, 200)  # these leave through x = 0
+    vel[:200, 0] = -1.5
+    pos[200:400, 1] = rng.uniform(L - 0.01, L, 200)  # and these through y = L
+    vel[200:400, 1] = 1.5
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as fh:
+        pos.astype("<f8").tofile(fh)
+        vel.astype("<f8").tofile(fh)
+        pc.astype("<f8").tofile(fh)
+    _run_twin("stream_bin", inp, out)
+    raw = open(out, "rb").read()
+    f_pos = np.frombuffer(raw[: nf * 24], "<f8").reshape(nf, 3)
+    f_cnt = np.frombuffer(raw[nf * 24: nf * 24 + nc * 4], "<i4")
+    f_lst = np.frombuffer(raw[nf * 24 + nc * 4:], "<i4").reshape(nc, 10000)
+    o = O.Oracle(**O.reference_defaults(density=dens, n_colloid=nc))
+    o.set_fluid(pos, vel)
+    z = np.zeros((nc, 3))
+    o.set_colloid(pc, z, z, z, z, np.tile([1.0, 0.0, 0.0], (nc, 1)))
+    o.create_box()
+    o.update_pos_mpcd()
+    o.neighbour_list_mpcd()
+    o_pos, _ = o.get_fluid()
+    np.testing.assert_array_equal(o_pos, f_pos)
+    cnt, lst = o.neigh_fl()
+    np.testing.assert_array_equal(cnt, f_cnt)
+    off = 0
+    for j in range(nc):
+        np.testing.assert_array_equal(lst[off: off + cnt[j]], f_lst[j, : f_cnt[j]] - 1)
+        off += cnt[j]
+    # one SRD rotation of a fresh state
+    pos = rng.uniform(0, L, (nf, 3))
+    vel = rng.uniform(-1.7, 1.7, (nf, 3))
+    with open(inp, "wb") as fh:
+        pos.astype("<f8").tofile(fh)
+        vel.astype("<f8").tofile(fh)
+    _run_twin("rotate", inp, out)
+    f_vel = np.fromfile(out, "<f8").reshape(nf, 3)
+    o.set_fluid(pos, vel)
+    o.rng_reset(77777)
+    o.rotation_mpcd()
+    _, o_vel = o.get_fluid()
+    np.testing.assert_array_equal(o_vel, f_vel)
