@@ -21,6 +21,11 @@
 !                                        (fortran/rotation_mpcd.f90; the shift and
 !                                        axes drawn by ran1 from the twin's seed
 !                                        77777), writes vel_fl
+!   fortran_twin force <in> <out>        reads pos_colloid, builds the colloid
+!                                        pair list and the WCA forces
+!                                        (fortran/neighbour_list_md.f90,
+!                                        fortran/compute_force_md.f90), writes f
+!                                        (3 per colloid), then potential_colloid
 program fortran_twin
     use all_parameters
     implicit none
@@ -78,9 +83,21 @@ program fortran_twin
         open (newunit=u, file=trim(path), access='stream', form='unformatted', status='replace')
         write (u) vel_fl
         close (u)
+    else if (trim(mode) == 'force') then
+        call get_command_argument(2, a1)
+        call get_command_argument(3, path)
+        open (newunit=u, file=trim(a1), access='stream', form='unformatted', status='old')
+        read (u) pos_colloid
+        close (u)
+        call neighbour_list_md
+        call compute_force_md
+        open (newunit=u, file=trim(path), access='stream', form='unformatted', status='replace')
+        write (u) f
+        write (u) potential_colloid
+        close (u)
     else
         write (*, *) 'usage: fortran_twin ran1 <seed> <n> <out> | create_box <out> | stream_bin <in> <out>'
-        write (*, *) '       | rotate <in> <out>'
+        write (*, *) '       | rotate <in> <out> | force <in> <out>'
         stop 2
     end if
 end program fortran_twin
