@@ -26,6 +26,13 @@
 !                                        (fortran/neighbour_list_md.f90,
 !                                        fortran/compute_force_md.f90), writes f
 !                                        (3 per colloid), then potential_colloid
+!   fortran_twin collide <in> <out>      reads pos_fl, vel_fl, pos_colloid,
+!                                        vel_colloid and ang_vel_colloid, bins
+!                                        the fluid and applies the stochastic
+!                                        reflections (fortran/fluid_colloid_collision.f90,
+!                                        stochastic_reflection.f90, gauss.f90;
+!                                        draws by ran1 from seed 77777); writes
+!                                        pos_fl, vel_fl, vel_colloid, ang_vel_colloid
 program fortran_twin
     use all_parameters
     implicit none
@@ -95,9 +102,28 @@ program fortran_twin
         write (u) f
         write (u) potential_colloid
         close (u)
+    else if (trim(mode) == 'collide') then
+        call get_command_argument(2, a1)
+        call get_command_argument(3, path)
+        open (newunit=u, file=trim(a1), access='stream', form='unformatted', status='old')
+        read (u) pos_fl
+        read (u) vel_fl
+        read (u) pos_colloid
+        read (u) vel_colloid
+        read (u) ang_vel_colloid
+        close (u)
+        call create_box
+        call neighbour_list_mpcd
+        call fluid_colloid_collision
+        open (newunit=u, file=trim(path), access='stream', form='unformatted', status='replace')
+        write (u) pos_fl
+        write (u) vel_fl
+        write (u) vel_colloid
+        write (u) ang_vel_colloid
+        close (u)
     else
         write (*, *) 'usage: fortran_twin ran1 <seed> <n> <out> | create_box <out> | stream_bin <in> <out>'
-        write (*, *) '       | rotate <in> <out> | force <in> <out>'
+        write (*, *) '       | rotate <in> <out> | force <in> <out> | collide <in> <out>'
         stop 2
     end if
 end program fortran_twin

@@ -276,3 +276,47 @@ def test_fortran_twin_colloid_forces(tmp_path):
     assert (mag > 0).sum() >= 8  # the pairs interact
     assert (np.abs(o_f - f_f).max(axis=1) <= 1e-13 * (mag + 1.0)).all()
     assert abs(o.potential() - f_pot) <= 1e-13 * abs(f_pot)
+
+
+def test_fortran_twin_stochastic_reflections(tmp_path):
+    """The Fortran twin's colloid-fluid collisions
+    (fortran/fluid_colloid_collision.f90 with stochastic_reflection.f90 and
+    the polar Box-Muller of gauss.f90, draws by ran1 from seed 77777) equal
+    the oracle's cpp restatement (cpp/fluid_colloid_collision.cpp:7-49) bit
+    for bit: every reflected particle's position and velocity and every
+    colloid's velocity and angular velocity after the impulses, on the twin's
+    system with 10 colloids dropped into a random fluid (over a thousand
+    reflections, overlapping colloids included)."""
+    if not os.path.exists(O.TWIN_EXE):
+        pytest.skip("Fortran twin not built (needs the reference's fortran/ sources and flang)")
+    rng = np.random.default_rng(13)
+    L, dens, nc = 30, 5, 10
+    nf = L ** 3 * dens
+    pos = rng.uniform(0, L, (nf, 3))
+    vel = rng.normal(0, 1.0, (nf, 3))
+    pc = rng.uniform(0, L, (nc, 3))
+    vc = rng.normal(0, 0.05, (nc, 3))
+    wc = rng.normal(0, 0.05, (nc, 3))
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as fh:
+        for a in (pos, vel, pc, vc, wc):
+            a.astype("<f8").tofile(fh)
+    _run_twin("collide", inp, out)
+    raw = np.fromfile(out, "<f8")
+    f_pos, f_vel = raw[: 3 * nf].reshape(nf, 3), raw[3 * nf: 6 * nf].reshape(nf, 3)
+    f_vc, f_wc = raw[6 * nf: 6 * nf + 3 * nc].reshape(nc, 3), raw[6 * nf + 3 * nc:].reshape(nc, 3)
+    o = O.Oracle(**O.reference_defaults(density=dens, n_colloid=nc))
+    o.set_fluid(pos, vel)
+    z = np.zeros((nc, 3))
+    o.set_colloid(pc, vc, wc, z, z, np.tile([1.0, 0.0, 0.0], (nc, 1)))
+    o.rng_reset(77777)
+    o.create_box()
+    o.neighbour_list_mpcd()
+    o.fluid_colloid_collision()
+    o_pos, o_vel = o.get_fluid()
+    c = o.get_colloid()
+    assert (np.abs(o_vel - vel).max(axis=1) > 0).sum() > 1000  # reflections happened
+    np.testing.assert_array_equal(o_pos, f_pos)
+    np.testing.assert_array_equal(o_vel, f_vel)
+    np.testing.assert_array_equal(c["vel"], f_vc)
+    np.testing.assert_array_equal(c["omega"], f_wc)
