@@ -33,6 +33,13 @@
 !                                        stochastic_reflection.f90, gauss.f90;
 !                                        draws by ran1 from seed 77777); writes
 !                                        pos_fl, vel_fl, vel_colloid, ang_vel_colloid
+!   fortran_twin colloid <in> <out>      reads pos_fl, vel_fl, pos_colloid,
+!                                        vel_colloid, ang_vel_colloid, ra, f and
+!                                        old_force, bins the fluid, then run
+!                                        (the active kick, fortran/run.f90),
+!                                        update_activity_direction, update_pos_md
+!                                        and update_velocity_colloid; writes
+!                                        vel_fl, pos_colloid, vel_colloid, ra
 program fortran_twin
     use all_parameters
     implicit none
@@ -121,9 +128,34 @@ program fortran_twin
         write (u) vel_colloid
         write (u) ang_vel_colloid
         close (u)
+    else if (trim(mode) == 'colloid') then
+        call get_command_argument(2, a1)
+        call get_command_argument(3, path)
+        open (newunit=u, file=trim(a1), access='stream', form='unformatted', status='old')
+        read (u) pos_fl
+        read (u) vel_fl
+        read (u) pos_colloid
+        read (u) vel_colloid
+        read (u) ang_vel_colloid
+        read (u) ra
+        read (u) f
+        read (u) old_force
+        close (u)
+        call create_box
+        call neighbour_list_mpcd
+        call run
+        call update_activity_direction
+        call update_pos_md
+        call update_velocity_colloid
+        open (newunit=u, file=trim(path), access='stream', form='unformatted', status='replace')
+        write (u) vel_fl
+        write (u) pos_colloid
+        write (u) vel_colloid
+        write (u) ra
+        close (u)
     else
         write (*, *) 'usage: fortran_twin ran1 <seed> <n> <out> | create_box <out> | stream_bin <in> <out>'
-        write (*, *) '       | rotate <in> <out> | force <in> <out> | collide <in> <out>'
+        write (*, *) '       | rotate <in> <out> | force <in> <out> | collide <in> <out> | colloid <in> <out>'
         stop 2
     end if
 end program fortran_twin

@@ -320,3 +320,54 @@ def test_fortran_twin_stochastic_reflections(tmp_path):
     np.testing.assert_array_equal(o_vel, f_vel)
     np.testing.assert_array_equal(c["vel"], f_vc)
     np.testing.assert_array_equal(c["omega"], f_wc)
+
+
+def test_fortran_twin_colloid_updates(tmp_path):
+    """The Fortran twin's colloid side of a step -- run() with its rear-shell
+    payment (fortran/run.f90), update_activity_direction, update_pos_md with
+    the periodic wrap and update_velocity_colloid -- against the oracle's cpp
+    restatement (cpp/run.cpp:10-26, cpp/update.cpp:34-67) at the twin's
+    v0 = 0.04: the kicked fluid, colloid positions (one crossing two faces),
+    velocities and orientations agree bit for bit."""
+    if not os.path.exists(O.TWIN_EXE):
+        pytest.skip("Fortran twin not built (needs the reference's fortran/ sources and flang)")
+    rng = np.random.default_rng(17)
+    L, dens, nc = 30, 5, 10
+    nf = L ** 3 * dens
+    pos = rng.uniform(0, L, (nf, 3))
+    vel = rng.normal(0, 1.0, (nf, 3))
+    pc = rng.uniform(0, L, (nc, 3))
+    vc = rng.normal(0, 0.05, (nc, 3))
+    wc = rng.normal(0, 0.5, (nc, 3))
+    ra = rng.normal(size=(nc, 3))
+    ra /= np.linalg.norm(ra, axis=1, keepdims=True)
+    fc = rng.normal(0, 5, (nc, 3))
+    of = rng.normal(0, 5, (nc, 3))
+    pc[0] = [29.9995, 0.0004, 15.0]  # leaves through x = L and y = 0
+    vc[0] = [0.2, -0.2, 0.0]
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as fh:
+        for a in (pos, vel, pc, vc, wc, ra, fc, of):
+            a.astype("<f8").tofile(fh)
+    _run_twin("colloid", inp, out)
+    raw = np.fromfile(out, "<f8")
+    k = 3 * nf
+    f_vel = raw[:k].reshape(nf, 3)
+    f_pc, f_vc, f_ra = (raw[k + 3 * nc * i: k + 3 * nc * (i + 1)].reshape(nc, 3) for i in range(3))
+    o = O.Oracle(**O.reference_defaults(density=dens, n_colloid=nc, v0=0.04))
+    o.set_fluid(pos, vel)
+    o.set_colloid(pc, vc, wc, fc, of, ra)
+    o.create_box()
+    o.neighbour_list_mpcd()
+    o.run()
+    o.update_activity_direction()
+    o.update_pos_md()
+    o.update_velocity_colloid()
+    _, o_vel = o.get_fluid()
+    c = o.get_colloid()
+    assert (np.abs(o_vel - vel).max(axis=1) > 0).sum() > 1000  # shells paid the kicks
+    assert c["pos"][0][0] < 1.0 and c["pos"][0][1] > 29.0  # wrapped
+    np.testing.assert_array_equal(o_vel, f_vel)
+    np.testing.assert_array_equal(c["pos"], f_pc)
+    np.testing.assert_array_equal(c["vel"], f_vc)
+    np.testing.assert_array_equal(c["ra"], f_ra)
