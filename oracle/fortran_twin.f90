@@ -40,12 +40,22 @@
 !                                        update_activity_direction, update_pos_md
 !                                        and update_velocity_colloid; writes
 !                                        vel_fl, pos_colloid, vel_colloid, ra
+!   fortran_twin startup <out>           the twin's start-up from its seed 77777:
+!                                        initialize_colloid, initialize_fluid,
+!                                        tumble; writes pos_fl, vel_fl,
+!                                        pos_colloid, vel_colloid,
+!                                        ang_vel_colloid, ra
+!   fortran_twin steps <k> <out>         the start-up, the initial lists, forces
+!                                        and orientations, then k MPCD steps in
+!                                        the order of fortran/active_brownian.f90:7-30
+!                                        (rotation, run, n substeps); writes the
+!                                        same arrays as startup
 program fortran_twin
     use all_parameters
     implicit none
     double precision :: ran1
     character(len=256) :: mode, a1, a2, path
-    integer :: seed, ndraw, i, u
+    integer :: seed, ndraw, i, u, ksteps, l
     double precision, allocatable :: draws(:)
 
     call get_command_argument(1, mode)
@@ -153,9 +163,57 @@ program fortran_twin
         write (u) vel_colloid
         write (u) ra
         close (u)
+    else if (trim(mode) == 'startup') then
+        call get_command_argument(2, path)
+        call initialize_colloid
+        call initialize_fluid
+        call tumble
+        open (newunit=u, file=trim(path), access='stream', form='unformatted', status='replace')
+        write (u) pos_fl
+        write (u) vel_fl
+        write (u) pos_colloid
+        write (u) vel_colloid
+        write (u) ang_vel_colloid
+        write (u) ra
+        close (u)
+    else if (trim(mode) == 'steps') then
+        call get_command_argument(2, a1)
+        call get_command_argument(3, path)
+        read (a1, *) ksteps
+        call initialize_colloid
+        call initialize_fluid
+        call create_box
+        call neighbour_list_mpcd
+        call neighbour_list_md
+        call compute_force_md
+        call tumble
+        do nn = 1, ksteps
+            call rotation_mpcd
+            call run
+            do l = 1, n
+                old_force = f
+                call update_pos_md
+                call neighbour_list_md
+                call update_pos_mpcd
+                call neighbour_list_mpcd
+                call fluid_colloid_collision
+                call update_activity_direction
+                call compute_force_md
+                call update_velocity_colloid
+            end do
+        end do
+        open (newunit=u, file=trim(path), access='stream', form='unformatted', status='replace')
+        write (u) pos_fl
+        write (u) vel_fl
+        write (u) pos_colloid
+        write (u) vel_colloid
+        write (u) ang_vel_colloid
+        write (u) ra
+        close (u)
     else
         write (*, *) 'usage: fortran_twin ran1 <seed> <n> <out> | create_box <out> | stream_bin <in> <out>'
         write (*, *) '       | rotate <in> <out> | force <in> <out> | collide <in> <out> | colloid <in> <out>'
+        write (*, *) '       | startup <out> | steps <k> <out>'
         stop 2
     end if
 end program fortran_twin
