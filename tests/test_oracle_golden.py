@@ -148,7 +148,11 @@ def _run_twin(*args):
     import subprocess
 
     def big_stack():
-        resource.setrlimit(resource.RLIMIT_STACK, (256 << 20, resource.RLIM_INFINITY))
+        hard = resource.getrlimit(resource.RLIMIT_STACK)[1]
+        want = 256 << 20
+        if hard != resource.RLIM_INFINITY:
+            want = min(want, hard)
+        resource.setrlimit(resource.RLIMIT_STACK, (want, hard))
     subprocess.run([O.TWIN_EXE, *map(str, args)], check=True, preexec_fn=big_stack)
 
 
